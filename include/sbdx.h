@@ -1,0 +1,149 @@
+/*
+ * sbdx.h -- C ABI of the MI355X (gfx950) implementation of the sbd-pub
+ * tensor-product-basis (TPB) Hamiltonian path: excitation tables, the
+ * diagonal, sigma = H*C, the single-root Davidson solver and the 1-/2-RDM.
+ *
+ * Every entry point returns an int status (SBDX_OK == 0) and never throws or
+ * exits; sbdx_last_error() gives the message of the last failure on the
+ * calling thread.  Caller buffers are plain host pointers unless the name
+ * ends in _device; they are never retained past the call.  Device memory is
+ * owned by the opaque handle.  One host thread per handle.
+ *
+ * Vectors are alpha-major, index ia*n_bdet + ib, in the reference's
+ * determinant convention (interleaved spin orbitals, chemistry/basic/
+ * determinants.h:96-142), so they are interchangeable with the reference's
+ * own vectors.  Citations are relative to the sbd-pub source tree.
+ */
+#ifndef SBDX_H
+#define SBDX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SBDX_OK 0
+#define SBDX_ERR_ARG 1           /* invalid argument (sizes, pointers)       */
+#define SBDX_ERR_HIP 2           /* HIP runtime or kernel launch failure     */
+#define SBDX_ERR_UNSUPPORTED 3   /* instance outside supported limits        */
+#define SBDX_ERR_ALLOC 4         /* device allocation failed                 */
+#define SBDX_ERR_NOT_CONVERGED 5 /* solver stopped before reaching eps       */
+#define SBDX_ERR_IO 6            /* file could not be read or parsed         */
+
+typedef struct sbdx_context *sbdx_handle_t;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *sbdx_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int sbdx_version(void);
+
+/*
+ * Build an operator on the current HIP device.
+ * Replaces MultTPBThrust::Init (chemistry/tpb/mult_thrust.h:59-94, 98-277)
+ * together with MakeHelpers/RemakeHelpers (chemistry/tpb/helper.h:710-854,
+ * 916-1226) for one rank: the excitation tables are built on the GPU.
+ *   norb          spatial orbitals (1..64)
+ *   bit_length    bits per size_t word of the string arrays (reference
+ *                 default 20, chemistry/tpb/sbdiag.h:41)
+ *   adet_words    n_adet strings x ceil(norb/bit_length) words, sorted and
+ *                 unique as LoadAlphaDets leaves them
+ *                 (chemistry/basic/makedeterminants.h:160-185)
+ *   I0, I1, I2    core energy, oneInt::store ((2L)^2, row-major) and
+ *                 twoInt::store (8-fold compressed, n_I2 = B(B+1)/2 with
+ *                 B = L(L+1)/2), exactly as SetupIntegrals fills them
+ *                 (chemistry/basic/makeintegrals.h:123-174)
+ */
+int sbdx_create(sbdx_handle_t *out, int norb, size_t bit_length,
+                const uint64_t *adet_words, size_t n_adet,
+                const uint64_t *bdet_words, size_t n_bdet,
+                double I0, const double *I1, const double *I2, size_t n_I2);
+int sbdx_destroy(sbdx_handle_t h);
+
+/* n_adet, n_bdet and n_det = n_adet*n_bdet of the handle. */
+int sbdx_dims(sbdx_handle_t h, size_t *n_adet, size_t *n_bdet, size_t *n_det);
+
+/* Diagonal hii = ZeroExcite(det) including I0.
+ * Replaces makeQChamDiagTerms (chemistry/tpb/qcham.h:289-340;
+ * MultTPBThrust::makeQChamDiagTerms, mult_thrust.h:1585-1664). */
+int sbdx_diag(sbdx_handle_t h, double *hii);
+
+/* sigma: Wb (+)= H*Wk.  With accumulate != 0 the product is added to Wb, as
+ * MultBase::run does (chemistry/basic/mult.h:19-103; CPU sbd::mult,
+ * chemistry/tpb/mult.h:625-881); with accumulate == 0 Wb is overwritten.
+ * The diagonal term uses the handle's own hii. */
+int sbdx_sigma(sbdx_handle_t h, const double *Wk, double *Wb, int accumulate);
+/* Same with device pointers on the handle's stream (synchronous on return). */
+int sbdx_sigma_device(sbdx_handle_t h, const double *d_Wk, double *d_Wb, int accumulate);
+
+/*
+ * Single-root Davidson, chemistry/tpb/davidson.h:667-930 and
+ * chemistry/basic/davidson_thrust.h:85-373: restarts every nb vectors,
+ * preconditioner R/(E0-d) with the 1e-12 guard, modified Gram-Schmidt,
+ * Rayleigh-Ritz on the device, stop when |E0 W - H W| < eps, or when the
+ * predicted end of the next step passes max_time seconds.
+ * W (host, n_det) holds the initial vector on entry and the normalized
+ * solution on exit.  *energy receives the lowest Ritz value of the last
+ * step and *n_steps the number of sigma steps.  Returns
+ * SBDX_ERR_NOT_CONVERGED (with valid outputs) if eps was not reached.
+ */
+int sbdx_davidson(sbdx_handle_t h, double *W, int max_it, int nb, double eps,
+                  double max_time, double *energy, int *n_steps);
+
+/* Split form of sbdx_davidson for callers that keep the vector resident:
+ * upload the start vector, solve on the device, download the solution. */
+int sbdx_set_vector(sbdx_handle_t h, const double *W);
+int sbdx_solve(sbdx_handle_t h, int max_it, int nb, double eps, double max_time,
+               double *energy, int *n_steps);
+int sbdx_get_vector(sbdx_handle_t h, double *W);
+
+/* History of the last solve: trace[2*k] = E0, trace[2*k+1] = |r| of step k,
+ * the values the reference prints per Davidson step (davidson.h:839-851). */
+int sbdx_trace(sbdx_handle_t h, double *trace, int cap, int *n);
+
+/* Energy <W|H W> of a normalized host vector W (chemistry/tpb/sbdiag.h:431-469). */
+int sbdx_energy(sbdx_handle_t h, const double *W, double *energy);
+
+/*
+ * 1-/2-RDM of a host vector W, chemistry/tpb/correlation.h:378-662 with the
+ * element rules of chemistry/basic/correlation.h:16-229:
+ *   one[s*L*L + i + L*j]                           s = 0 alpha, 1 beta
+ *   two[(s+2t)*L^4 + i + L*j + L^2*k + L^3*l]
+ * one holds 2*L*L doubles and two 4*L^4 doubles (caller-allocated); both are
+ * overwritten.
+ */
+int sbdx_rdm(sbdx_handle_t h, const double *W, double *one, double *two);
+
+/*
+ * Excitation tables as the reference's TaskHelpers lists
+ * (chemistry/tpb/helper.h:185-323): for bra string i, the ket strings of
+ * the same spin set that differ by one (kind 1) or two (kind 2) orbitals, in
+ * ascending index, with cran = [cr..., an...] spin orbitals (cr bra-only,
+ * an ket-only; 2*orb for alpha, 2*orb+1 for beta).  spin 0 alpha, 1 beta.
+ * sbdx_table_count gives the entry count; export fills offsets[n+1],
+ * kets[count] and cran[count*2*kind].
+ */
+int sbdx_table_count(sbdx_handle_t h, int spin, int kind, size_t *count);
+int sbdx_table_export(sbdx_handle_t h, int spin, int kind, uint64_t *offsets,
+                      uint32_t *kets, int32_t *cran);
+
+/* Per-kernel timing with HIP events on the handle's stream.
+ * enable != 0 starts recording (and clears earlier records). */
+int sbdx_profile(sbdx_handle_t h, int enable);
+/* Number of distinct kernels recorded so far. */
+int sbdx_profile_count(sbdx_handle_t h, int *n);
+/* Name, launch count and summed duration (ms) of recorded kernel idx. */
+int sbdx_profile_get(sbdx_handle_t h, int idx, char *name, size_t name_cap,
+                     long *launches, double *total_ms);
+
+/* Algorithmic work of one sigma for this instance (roofline numerators):
+ * counts[0..3] = alpha singles, alpha doubles, beta singles, beta doubles
+ * table entries; counts[4] = sum over alpha rows of (alpha singles x sum of
+ * beta singles) = the number of mixed terms; counts[5] = n_det. */
+int sbdx_work_counts(sbdx_handle_t h, double *counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SBDX_H */

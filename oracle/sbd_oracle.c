@@ -695,6 +695,7 @@ double orc_hij_bra(void *h, uint64_t a_bra, uint64_t b_bra, uint64_t a_ket, uint
   u128 cr = Db & ~Dk, an = Dk & ~Db;
   int nc = popc128(cr);
   if (nc == 0) return zero_excite(&o->g, Db);
+  if (nc > 2) return 0.0;
   int c[2], d[2], k = 0, m = 0;
   for (int i = 0; i < 2 * L; i++) {
     if (bit128(cr, i)) c[k++] = i;

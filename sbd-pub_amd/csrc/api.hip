@@ -1,0 +1,490 @@
+// C ABI of libsbdx (declared in include/sbdx.h): handle lifecycle, the
+// boundary conversion between the reference's interleaved determinant basis
+// and the device's alpha-first basis, profiling, and error reporting.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "../../include/sbdx.h"
+#include "common.h"
+
+namespace sbdx {
+
+static thread_local std::string g_err;
+
+void *dalloc(Context &c, size_t bytes) {
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes ? bytes : 8);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(SBDX_ERR_ALLOC, "hipMalloc(" + std::to_string(bytes) + " bytes) failed: " +
+                                    hipGetErrorString(e));
+  }
+  c.allocs.push_back(p);
+  return p;
+}
+
+void dfree_all(Context &c) {
+  for (void *p : c.allocs) (void)hipFree(p);
+  c.allocs.clear();
+}
+
+TmpBuf::TmpBuf(size_t bytes) {
+  hipError_t e = hipMalloc(&p, bytes ? bytes : 8);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;
+    throw Error(SBDX_ERR_ALLOC, "hipMalloc(" + std::to_string(bytes) + " bytes) failed");
+  }
+}
+TmpBuf::~TmpBuf() {
+  if (p) (void)hipFree(p);
+}
+
+void check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Error(SBDX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+ProfScope::ProfScope(Context &cx, const char *nm) : c(cx), name(nm) {
+  if (!c.prof) return;
+  SBDX_HIP(hipEventCreate(&b));
+  SBDX_HIP(hipEventCreate(&e));
+  SBDX_HIP(hipEventRecord(b, c.stream));
+}
+ProfScope::~ProfScope() {
+  if (!b) return;
+  if (hipEventRecord(e, c.stream) != hipSuccess) return;
+  auto it = c.prof_slots.find(name);
+  if (it == c.prof_slots.end()) {
+    c.prof_order.push_back(name);
+    it = c.prof_slots.emplace(name, ProfSlot{}).first;
+  }
+  it->second.ev.emplace_back(b, e);
+  it->second.launches++;
+}
+
+static void prof_collect(Context &c) {
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  for (auto &kv : c.prof_slots) {
+    for (auto &pr : kv.second.ev) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) kv.second.total_ms += ms;
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+    kv.second.ev.clear();
+  }
+}
+
+static void prof_clear(Context &c) {
+  for (auto &kv : c.prof_slots)
+    for (auto &pr : kv.second.ev) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  c.prof_slots.clear();
+  c.prof_order.clear();
+}
+
+static size_t i2_size(int L) {
+  const size_t B = (size_t)L * (L + 1) / 2;
+  return B * (B + 1) / 2;
+}
+static inline size_t tri(size_t a, size_t b) {
+  const size_t mx = a > b ? a : b, mn = a > b ? b : a;
+  return mx * (mx + 1) / 2 + mn;
+}
+
+static std::vector<uint64_t> unpack_words(const uint64_t *w, size_t n, size_t bl, int L) {
+  const size_t nw = (L + bl - 1) / bl;
+  std::vector<uint64_t> out(n);
+  const uint64_t lowmask = bl >= 64 ? ~0ull : ((1ull << bl) - 1ull);
+  const uint64_t lmask = L >= 64 ? ~0ull : ((1ull << L) - 1ull);
+  for (size_t i = 0; i < n; i++) {
+    uint64_t m = 0;
+    for (size_t k = 0; k < nw; k++) m |= (w[i * nw + k] & lowmask) << (k * bl);
+    out[i] = m & lmask;
+  }
+  return out;
+}
+
+static void upload(Context &c, double *d, const double *h) {
+  SBDX_HIP(hipMemcpyAsync(d, h, sizeof(double) * c.N, hipMemcpyHostToDevice, c.stream));
+}
+static void download(Context &c, double *h, const double *d) {
+  SBDX_HIP(hipMemcpyAsync(h, d, sizeof(double) * c.N, hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+}
+
+static Context *create(int L, size_t bl, const uint64_t *aw, size_t na, const uint64_t *bw, size_t nb,
+                       double I0, const double *I1, const double *I2, size_t nI2) {
+  if (L < 1 || L > 64) throw Error(SBDX_ERR_UNSUPPORTED, "norb must be in [1, 64]");
+  if (bl < 1 || bl > 64) throw Error(SBDX_ERR_ARG, "bit_length must be in [1, 64]");
+  if (!aw || !bw || !I1 || !I2) throw Error(SBDX_ERR_ARG, "null input pointer");
+  if (na < 1 || nb < 1) throw Error(SBDX_ERR_ARG, "empty string set");
+  if (na > 0xffffffffull || nb > 0xffffffffull)
+    throw Error(SBDX_ERR_UNSUPPORTED, "string sets are limited to 2^32-1 strings");
+  if (nI2 != i2_size(L))
+    throw Error(SBDX_ERR_ARG, "n_I2 must be B(B+1)/2 with B = L(L+1)/2 (" + std::to_string(i2_size(L)) + ")");
+  auto ctx = std::make_unique<Context>();
+  Context &c = *ctx;
+  try {
+    SBDX_HIP(hipGetDevice(&c.device));
+    int lds = 0, ncu = 0;
+    SBDX_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device));
+    SBDX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
+    c.lds_limit = (size_t)lds;
+    c.n_cu = ncu;
+    SBDX_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.L = L;
+    c.K = L * L;
+    c.na = na;
+    c.nb = nb;
+    c.N = na * nb;
+    c.I0 = I0;
+    // integrals: spatial one-body from the alpha block of I1, dense (pq|rs)
+    const size_t K = (size_t)c.K;
+    std::vector<double> h1(K), eri(K * K);
+    for (int p = 0; p < L; p++)
+      for (int q = 0; q < L; q++) h1[p * L + q] = I1[(size_t)(2 * p) * (2 * L) + 2 * q];
+    for (int p = 0; p < L; p++)
+      for (int q = 0; q < L; q++) {
+        const size_t pq = tri(p, q);
+        for (int r = 0; r < L; r++)
+          for (int s = 0; s < L; s++) eri[(p * L + q) * K + r * L + s] = I2[tri(pq, tri(r, s))];
+      }
+    c.h1 = (double *)dalloc(c, sizeof(double) * K);
+    c.eri = (double *)dalloc(c, sizeof(double) * K * K);
+    SBDX_HIP(hipMemcpyAsync(c.h1, h1.data(), sizeof(double) * K, hipMemcpyHostToDevice, c.stream));
+    SBDX_HIP(hipMemcpyAsync(c.eri, eri.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, c.stream));
+    const auto as = unpack_words(aw, na, bl, L);
+    const auto bs = unpack_words(bw, nb, bl, L);
+    build_spin_tables(c, c.A, as);
+    build_dress(c, c.A);
+    if (as == bs) {
+      c.B = c.A;
+      c.B.alias = true;
+    } else {
+      build_spin_tables(c, c.B, bs);
+      build_dress(c, c.B);
+    }
+    c.hii = (double *)dalloc(c, sizeof(double) * c.N);
+    compute_diag(c);
+    c.vin = (double *)dalloc(c, sizeof(double) * c.N);
+    c.vout = (double *)dalloc(c, sizeof(double) * c.N);
+    c.T1 = (double *)dalloc(c, sizeof(double) * c.N);
+    c.T2 = (double *)dalloc(c, sizeof(double) * c.N);
+    c.W = (double *)dalloc(c, sizeof(double) * c.N);
+    c.part_cap = 1024 * (size_t)kNbMax;
+    c.part = (double *)dalloc(c, sizeof(double) * c.part_cap);
+    c.dsmall = (double *)dalloc(c, sizeof(double) * kSmallDoubles);
+    SBDX_HIP(hipMemsetAsync(c.dsmall, 0, sizeof(double) * kSmallDoubles, c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+  } catch (...) {
+    dfree_all(c);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    throw;
+  }
+  return ctx.release();
+}
+
+static void destroy(Context *c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->stream);
+  prof_clear(*c);
+  dfree_all(*c);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // namespace sbdx
+
+using namespace sbdx;
+
+#define SBDX_GUARD_BEGIN try {
+#define SBDX_GUARD_END                                        \
+  }                                                           \
+  catch (const sbdx::Error &e) {                              \
+    g_err = e.what();                                         \
+    return e.code;                                            \
+  }                                                           \
+  catch (const std::bad_alloc &) {                            \
+    g_err = "host allocation failed";                         \
+    return SBDX_ERR_ALLOC;                                    \
+  }                                                           \
+  catch (const std::exception &e) {                           \
+    g_err = e.what();                                         \
+    return SBDX_ERR_ARG;                                      \
+  }                                                           \
+  catch (...) {                                               \
+    g_err = "unknown error";                                  \
+    return SBDX_ERR_ARG;                                      \
+  }
+
+static Context &ctx_of(sbdx_handle_t h) {
+  if (!h) throw Error(SBDX_ERR_ARG, "null handle");
+  return *reinterpret_cast<Context *>(h);
+}
+
+extern "C" {
+
+const char *sbdx_last_error(void) { return g_err.c_str(); }
+int sbdx_version(void) { return 100; }
+
+int sbdx_create(sbdx_handle_t *out, int norb, size_t bit_length, const uint64_t *adet_words,
+                size_t n_adet, const uint64_t *bdet_words, size_t n_bdet, double I0,
+                const double *I1, const double *I2, size_t n_I2) {
+  SBDX_GUARD_BEGIN
+  if (!out) throw Error(SBDX_ERR_ARG, "null output handle pointer");
+  *out = nullptr;
+  Context *c = create(norb, bit_length, adet_words, n_adet, bdet_words, n_bdet, I0, I1, I2, n_I2);
+  *out = reinterpret_cast<sbdx_handle_t>(c);
+  g_err.clear();
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_destroy(sbdx_handle_t h) {
+  SBDX_GUARD_BEGIN
+  destroy(reinterpret_cast<Context *>(h));
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_dims(sbdx_handle_t h, size_t *na, size_t *nb, size_t *nd) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (na) *na = c.na;
+  if (nb) *nb = c.nb;
+  if (nd) *nd = c.N;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_diag(sbdx_handle_t h, double *hii) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!hii) throw Error(SBDX_ERR_ARG, "null hii");
+  download(c, hii, c.hii);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_sigma_device(sbdx_handle_t h, const double *d_in, double *d_out, int accumulate) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!d_in || !d_out) throw Error(SBDX_ERR_ARG, "null vector");
+  sign_convert(c, d_in, c.vin);
+  sigma_alpha_first(c, c.vin, c.vout);
+  if (accumulate)
+    sign_convert_add(c, c.vout, d_out);
+  else
+    sign_convert(c, c.vout, d_out);
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_sigma(sbdx_handle_t h, const double *Wk, double *Wb, int accumulate) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!Wk || !Wb) throw Error(SBDX_ERR_ARG, "null vector");
+  upload(c, c.vout, Wk);
+  sign_convert(c, c.vout, c.vin);
+  sigma_alpha_first(c, c.vin, c.vout);
+  if (accumulate) {
+    upload(c, c.vin, Wb);
+    sign_convert_add(c, c.vout, c.vin);
+    download(c, Wb, c.vin);
+  } else {
+    sign_convert(c, c.vout, c.vin);
+    download(c, Wb, c.vin);
+  }
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_set_vector(sbdx_handle_t h, const double *W) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!W) throw Error(SBDX_ERR_ARG, "null vector");
+  upload(c, c.vin, W);
+  sign_convert(c, c.vin, c.W);
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  c.have_vector = true;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_get_vector(sbdx_handle_t h, double *W) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!W) throw Error(SBDX_ERR_ARG, "null vector");
+  if (!c.have_vector) throw Error(SBDX_ERR_ARG, "no vector set");
+  sign_convert(c, c.W, c.vout);
+  download(c, W, c.vout);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_solve(sbdx_handle_t h, int max_it, int nb, double eps, double max_time, double *energy,
+               int *n_steps) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!c.have_vector) throw Error(SBDX_ERR_ARG, "no start vector (sbdx_set_vector)");
+  if (max_it < 1) throw Error(SBDX_ERR_ARG, "max_it must be >= 1");
+  const int rc = davidson_solve(c, max_it, nb, eps, max_time, energy, n_steps);
+  if (rc != 0) g_err = "Davidson did not reach eps";
+  return rc;
+  SBDX_GUARD_END
+}
+
+int sbdx_davidson(sbdx_handle_t h, double *W, int max_it, int nb, double eps, double max_time,
+                  double *energy, int *n_steps) {
+  int rc = sbdx_set_vector(h, W);
+  if (rc != SBDX_OK) return rc;
+  rc = sbdx_solve(h, max_it, nb, eps, max_time, energy, n_steps);
+  if (rc != SBDX_OK && rc != SBDX_ERR_NOT_CONVERGED) return rc;
+  const int rc2 = sbdx_get_vector(h, W);
+  return rc2 != SBDX_OK ? rc2 : rc;
+}
+
+int sbdx_trace(sbdx_handle_t h, double *trace, int cap, int *n) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  const int steps = (int)(c.trace.size() / 2);
+  if (n) *n = steps;
+  if (trace)
+    for (int k = 0; k < std::min(cap, steps); k++) {
+      trace[2 * k] = c.trace[2 * k];
+      trace[2 * k + 1] = c.trace[2 * k + 1];
+    }
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_energy(sbdx_handle_t h, const double *W, double *energy) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!W || !energy) throw Error(SBDX_ERR_ARG, "null pointer");
+  upload(c, c.vin, W);
+  sign_convert(c, c.vin, c.vout);
+  sigma_alpha_first(c, c.vout, c.vin);
+  *energy = device_dot(c, c.vout, c.vin, c.N);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_rdm(sbdx_handle_t h, const double *W, double *one, double *two) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!W || !one || !two) throw Error(SBDX_ERR_ARG, "null pointer");
+  const size_t L2 = (size_t)c.L * c.L;
+  upload(c, c.vin, W);
+  sign_convert(c, c.vin, c.vout);
+  TmpBuf d1(sizeof(double) * 2 * L2), d2(sizeof(double) * 4 * L2 * L2);
+  compute_rdm(c, c.vout, (double *)d1.p, (double *)d2.p);
+  SBDX_HIP(hipMemcpyAsync(one, d1.p, sizeof(double) * 2 * L2, hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipMemcpyAsync(two, d2.p, sizeof(double) * 4 * L2 * L2, hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_table_count(sbdx_handle_t h, int spin, int kind, size_t *count) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!count || (spin != 0 && spin != 1) || (kind != 1 && kind != 2))
+    throw Error(SBDX_ERR_ARG, "bad table selector");
+  const SpinTables &t = spin ? c.B : c.A;
+  *count = kind == 1 ? t.ns : t.nd;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_table_export(sbdx_handle_t h, int spin, int kind, uint64_t *offsets, uint32_t *kets,
+                      int32_t *cran) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!offsets || !kets || !cran || (spin != 0 && spin != 1) || (kind != 1 && kind != 2))
+    throw Error(SBDX_ERR_ARG, "bad table export arguments");
+  const SpinTables &t = spin ? c.B : c.A;
+  const size_t n = t.n;
+  if (kind == 1) {
+    std::vector<uint64_t> ent(t.ns);
+    SBDX_HIP(hipMemcpyAsync(offsets, t.s_off, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipMemcpyAsync(ent.data(), t.s_ent, sizeof(uint64_t) * t.ns, hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+    for (size_t e = 0; e < t.ns; e++) {
+      const uint32_t pq = (uint32_t)((ent[e] >> 32) & 0x7fffffffull);
+      kets[e] = (uint32_t)(ent[e] & 0xffffffffull);
+      cran[2 * e] = 2 * (int)(pq / c.L) + spin;
+      cran[2 * e + 1] = 2 * (int)(pq % c.L) + spin;
+    }
+  } else {
+    std::vector<uint32_t> orb(t.nd);
+    SBDX_HIP(hipMemcpyAsync(offsets, t.d_off, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipMemcpyAsync(kets, t.d_ket, sizeof(uint32_t) * t.nd, hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipMemcpyAsync(orb.data(), t.d_orb, sizeof(uint32_t) * t.nd, hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+    for (size_t e = 0; e < t.nd; e++)
+      for (int k = 0; k < 4; k++) cran[4 * e + k] = 2 * (int)((orb[e] >> (8 * k)) & 255u) + spin;
+  }
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_profile(sbdx_handle_t h, int enable) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  prof_clear(c);
+  c.prof = enable != 0;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_profile_count(sbdx_handle_t h, int *n) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  prof_collect(c);
+  if (n) *n = (int)c.prof_order.size();
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_profile_get(sbdx_handle_t h, int idx, char *name, size_t cap, long *launches,
+                     double *total_ms) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  prof_collect(c);
+  if (idx < 0 || idx >= (int)c.prof_order.size()) throw Error(SBDX_ERR_ARG, "profile index out of range");
+  const std::string &nm = c.prof_order[idx];
+  const ProfSlot &s = c.prof_slots[nm];
+  if (name && cap) {
+    std::strncpy(name, nm.c_str(), cap - 1);
+    name[cap - 1] = 0;
+  }
+  if (launches) *launches = s.launches;
+  if (total_ms) *total_ms = s.total_ms;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_work_counts(sbdx_handle_t h, double *counts) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!counts) throw Error(SBDX_ERR_ARG, "null counts");
+  counts[0] = (double)c.A.ns;
+  counts[1] = (double)c.A.nd;
+  counts[2] = (double)c.B.ns;
+  counts[3] = (double)c.B.nd;
+  counts[4] = (double)c.A.ns * (double)c.B.ns;
+  counts[5] = (double)c.N;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+}  // extern "C"

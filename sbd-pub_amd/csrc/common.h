@@ -1,0 +1,148 @@
+// Internal declarations of libsbdx (HIP, gfx950 only).
+//
+// Device-side conventions
+//  * strings: one uint64 per string (norb <= 64), orbital k at bit k;
+//  * vectors: alpha-major [ia*nb + ib] in the ALPHA-FIRST determinant basis
+//    (alpha creators ascending, then beta creators ascending).  The API
+//    converts from/to the reference's interleaved basis with the diagonal
+//    sign S(Ia,Ib) = (-1)^#{(p in Ia, r in Ib): r < p}; in the alpha-first
+//    basis the phase of an alpha excitation depends on the alpha string only
+//    and that of a beta excitation on the beta string only;
+//  * dense ERI: eri[pq*K + rs] = (pq|rs), pq = p*L + q, K = L*L.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sbdx {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define SBDX_HIP(call)                                                            \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      throw ::sbdx::Error(2, std::string(#call) + ": " + hipGetErrorString(e_) +  \
+                                 " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
+  } while (0)
+
+// Excitation tables of one spin set.  Singles entry: ket | pqidx<<32 |
+// (sign<0)<<63 with pqidx = p*L + q (p bra-only, q ket-only); s_val = sign*F,
+// F = h_pq + sum_{k in bra} ((pq|kk) - (pk|kq)).  Doubles: ket, orbitals
+// i | j<<8 | a<<16 | b<<24 (i<j bra-only, a<b ket-only), signed value.
+struct SpinTables {
+  int n = 0;
+  uint64_t *str = nullptr;     // [n]
+  uint64_t *s_off = nullptr;   // [n+1]
+  uint64_t *s_ent = nullptr;   // [ns]
+  double *s_val = nullptr;     // [ns]
+  uint64_t *d_off = nullptr;   // [n+1]
+  uint32_t *d_ket = nullptr;   // [nd]
+  uint32_t *d_orb = nullptr;   // [nd]
+  double *d_val = nullptr;     // [nd]
+  double *dress = nullptr;     // [K][n]: dress[pq*n + x] = sum_{k in str[x]} (pq|kk)
+  double *e1 = nullptr;        // [n]: sum_p h_pp + sum_{p<q}((pp|qq)-(pq|qp)) of the string
+  double *jsum = nullptr;      // [n][L]: sum_{p in str} (pp|rr)
+  size_t ns = 0, nd = 0;
+  size_t max_s = 0;            // longest singles row
+  bool alias = false;          // shares arrays with the other spin
+};
+
+struct ProfSlot {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  long launches = 0;
+  double total_ms = 0.0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int L = 0, K = 0;
+  size_t na = 0, nb = 0, N = 0;
+  double I0 = 0.0;
+  double *eri = nullptr;   // [K*K]
+  double *h1 = nullptr;    // [K] spatial one-body
+  SpinTables A, B;
+  double *hii = nullptr;   // [N]
+  // work vectors
+  double *vin = nullptr, *vout = nullptr;   // API staging (alpha-first)
+  double *T1 = nullptr, *T2 = nullptr;      // transpose scratch [N]
+  double *part = nullptr;                   // reduction partials
+  size_t part_cap = 0;
+  // Davidson state
+  int dav_nb = 0;
+  std::vector<double *> C, HC;
+  double *W = nullptr, *R = nullptr;
+  double *dsmall = nullptr;                 // small device matrices/scalars
+  std::vector<double> trace;
+  bool have_vector = false;
+  size_t lds_limit = 65536;
+  int n_cu = 256;
+  // profiling
+  bool prof = false;
+  std::map<std::string, ProfSlot> prof_slots;
+  std::vector<std::string> prof_order;
+  std::vector<void *> allocs;
+};
+
+// small-matrix device layout inside Context::dsmall (doubles)
+// [0, nbmax^2)            H (subspace matrix, column-major, ld = nbmax)
+// [nbmax^2, 2 nbmax^2)    U (eigenvectors)
+// [2 nbmax^2, +nbmax)     E (eigenvalues ascending)
+// then scalars: norms, residual
+constexpr int kNbMax = 64;
+constexpr size_t kSmallDoubles = 2 * kNbMax * kNbMax + kNbMax + 64;
+constexpr int kScalarOffset = 2 * kNbMax * kNbMax + kNbMax;
+
+// ---------------------------------------------------------------- helpers
+void *dalloc(Context &c, size_t bytes);
+void dfree_all(Context &c);
+
+// Device scratch released at scope exit (callers synchronize first).
+struct TmpBuf {
+  void *p = nullptr;
+  explicit TmpBuf(size_t bytes);
+  ~TmpBuf();
+  TmpBuf(const TmpBuf &) = delete;
+  TmpBuf &operator=(const TmpBuf &) = delete;
+};
+
+struct ProfScope {
+  Context &c;
+  hipEvent_t b = nullptr, e = nullptr;
+  const char *name;
+  ProfScope(Context &cx, const char *nm);
+  ~ProfScope();
+};
+
+void check_launch(const char *what);
+
+// tables.hip
+void build_spin_tables(Context &c, SpinTables &t, const std::vector<uint64_t> &strs);
+void build_dress(Context &c, SpinTables &t);
+void compute_diag(Context &c);
+void sign_convert(Context &c, const double *in, double *out);      // out = S*in
+void sign_convert_add(Context &c, const double *in, double *out);  // out += S*in
+
+// sigma.hip
+void sigma_alpha_first(Context &c, const double *x, double *y);  // y = H' x (overwrite)
+void transpose_into(Context &c, const double *in, double *out, int R, int Cc);  // out = in^T
+
+// davidson.hip
+int davidson_solve(Context &c, int max_it, int nb, double eps, double max_time, double *energy,
+                   int *n_steps);
+double device_dot(Context &c, const double *x, const double *y, size_t n);
+
+// rdm.hip
+void compute_rdm(Context &c, const double *Wp, double *one, double *two);
+
+}  // namespace sbdx

@@ -1,0 +1,396 @@
+// 1-/2-RDM, chemistry/tpb/correlation.h:378-662 with the element rules of
+// chemistry/basic/correlation.h:16-229 (ZeroDiff/OneDiff/TwoDiffCorrelation).
+//
+// Layout: one[s*L^2 + i + L*j], two[(s+2t)*L^4 + i + L*j + L^2*k + L^3*l].
+// Work is done on the alpha-first vector W' = S*W: every contribution of the
+// reference, w_bra*w_ket*(interleaved sign), equals w'_bra*w'_ket times the
+// one-spin table sign (alpha single, beta single, or their product for the
+// mixed term), so the per-pair sums below factor over the other spin:
+//   ZeroDiff  -> row/column sums of w'^2 and their occupation-weighted sums
+//   OneDiff   -> per single pair: T = s*sum_m w'[I,m]w'[J,m] and
+//                Tk[k] = s*sum_m w'[I,m]w'[J,m] n_k(m) (other-spin spectators)
+//   TwoDiff   -> per same-spin double pair: T; mixed: T[pq][rs] summed over
+//                all (alpha single, beta single) pairs, then placed with the
+//                reference's spin-orbital ordering rules.
+#include "common.h"
+
+namespace sbdx {
+
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+__device__ __forceinline__ double parity1r(uint64_t s, int a, int b) {
+  int lo = a < b ? a : b, hi = a < b ? b : a;
+  uint64_t mask = ((1ull << hi) - 1ull) ^ ((1ull << lo) - 1ull);
+  int n = __popcll(s & mask) + (int)((s >> lo) & 1ull);
+  return (n & 1) ? -1.0 : 1.0;
+}
+
+__device__ __forceinline__ double blk_sum256(double v, double *sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ int row_of(const uint64_t *off, int n, uint64_t e) {
+  int lo = 0, hi = n;  // off[lo] <= e < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Per row r of w (rows x M): rho[r] = sum_m w^2 and mrow[r][k] = sum_m w^2 n_k(m)
+__global__ void __launch_bounds__(256) k_rdm_diag_rows(int M, int L, const uint64_t *__restrict__ ostr,
+                                                       const double *__restrict__ w,
+                                                       double *__restrict__ rho,
+                                                       double *__restrict__ mrow) {
+  __shared__ double sh[256];
+  __shared__ double acc[64];
+  const int r = blockIdx.x;
+  if (threadIdx.x < 64) acc[threadIdx.x] = 0.0;
+  __syncthreads();
+  double s = 0.0;
+  for (int m = threadIdx.x; m < M; m += 256) {
+    const double v = w[(size_t)r * M + m];
+    const double v2 = v * v;
+    s += v2;
+    if (mrow) {
+      uint64_t occ = ostr[m];
+      while (occ) {
+        const int k = __ffsll((unsigned long long)occ) - 1;
+        occ &= occ - 1;
+        atomicAdd(&acc[k], v2);
+      }
+    }
+  }
+  const double t = blk_sum256(s, sh);
+  if (threadIdx.x == 0) rho[r] = t;
+  if (mrow && threadIdx.x < L) mrow[(size_t)r * L + threadIdx.x] = acc[threadIdx.x];
+}
+
+// ZeroDiff contributions from rho (both spins) and M[p][r] = sum_I n_p(I) mrow[I][r]
+__global__ void k_rdm_diag_scatter(int L, int na, int nb, const uint64_t *__restrict__ astr,
+                                   const uint64_t *__restrict__ bstr,
+                                   const double *__restrict__ rho_a, const double *__restrict__ rho_b,
+                                   const double *__restrict__ mrow, double *__restrict__ one,
+                                   double *__restrict__ two) {
+  // one thread per (p, r) pair of orbitals
+  const int p = blockIdx.x, r = threadIdx.x;
+  if (r >= L) return;
+  const size_t L2 = (size_t)L * L, L4 = L2 * L2;
+  double m = 0.0, aa = 0.0, bb = 0.0;
+  for (int I = 0; I < na; I++) {
+    const uint64_t s = astr[I];
+    if ((s >> p) & 1ull) {
+      m += mrow[(size_t)I * L + r];
+      if (r != p && ((s >> r) & 1ull)) aa += rho_a[I];
+    }
+  }
+  for (int ib = 0; ib < nb; ib++) {
+    const uint64_t s = bstr[ib];
+    if (((s >> p) & 1ull) && r != p && ((s >> r) & 1ull)) bb += rho_b[ib];
+  }
+  // alpha-beta pairs: two[2][p,r,p,r] and two[1][r,p,r,p]
+  two[2 * L4 + p + L * r + L2 * p + L2 * L * r] += m;
+  two[1 * L4 + r + L * p + L2 * r + L2 * L * p] += m;
+  // same-spin ordered pairs (p,r), p != r: [p,r,p,r] += w^2 and [p,r,r,p] -= w^2
+  if (r != p) {
+    two[0 * L4 + p + L * r + L2 * p + L2 * L * r] += aa;
+    two[0 * L4 + p + L * r + L2 * r + L2 * L * p] -= aa;
+    two[3 * L4 + p + L * r + L2 * p + L2 * L * r] += bb;
+    two[3 * L4 + p + L * r + L2 * r + L2 * L * p] -= bb;
+  }
+  if (r == 0) {
+    double da = 0.0, db = 0.0;
+    for (int I = 0; I < na; I++)
+      if ((astr[I] >> p) & 1ull) da += rho_a[I];
+    for (int ib = 0; ib < nb; ib++)
+      if ((bstr[ib] >> p) & 1ull) db += rho_b[ib];
+    one[p + L * p] += da;
+    one[L2 + p + L * p] += db;
+  }
+}
+
+// Single pairs of one spin: block per pair e; rows of w are this spin's strings.
+//   out[e*(L+1)]     = s * sum_m w[I,m] w[J,m]
+//   out[e*(L+1)+1+k] = s * sum_m w[I,m] w[J,m] n_k(m)    (other-spin strings)
+__global__ void __launch_bounds__(256) k_rdm_single_pairs(int n, int M, int L,
+                                                          const uint64_t *__restrict__ s_off,
+                                                          const uint64_t *__restrict__ s_ent,
+                                                          const uint64_t *__restrict__ ostr,
+                                                          const double *__restrict__ w,
+                                                          double *__restrict__ out) {
+  __shared__ double sh[256];
+  __shared__ double acc[64];
+  const uint64_t ns = s_off[n];
+  for (uint64_t e = blockIdx.x; e < ns; e += gridDim.x) {
+  const int I = row_of(s_off, n, e);
+  const uint64_t ent = s_ent[e];
+  const size_t J = (uint32_t)(ent & 0xffffffffull);
+  const double sg = (ent >> 63) ? -1.0 : 1.0;
+  if (threadIdx.x < 64) acc[threadIdx.x] = 0.0;
+  __syncthreads();
+  double s = 0.0;
+  for (int m = threadIdx.x; m < M; m += 256) {
+    const double v = w[(size_t)I * M + m] * w[J * M + m];
+    s += v;
+    if (v != 0.0) {
+      uint64_t occ = ostr[m];
+      while (occ) {
+        const int k = __ffsll((unsigned long long)occ) - 1;
+        occ &= occ - 1;
+        atomicAdd(&acc[k], v);
+      }
+    }
+  }
+  const double t = blk_sum256(s, sh);
+  double *o = out + e * (size_t)(L + 1);
+  if (threadIdx.x == 0) o[0] = sg * t;
+  if (threadIdx.x < L) o[1 + threadIdx.x] = sg * acc[threadIdx.x];
+  __syncthreads();
+  }
+}
+
+// Double pairs of one spin: out[e] = sign * sum_m w[I,m] w[J,m]
+__global__ void __launch_bounds__(256) k_rdm_double_pairs(int n, int M,
+                                                          const uint64_t *__restrict__ d_off,
+                                                          const uint32_t *__restrict__ d_ket,
+                                                          const uint32_t *__restrict__ d_orb,
+                                                          const uint64_t *__restrict__ str,
+                                                          const double *__restrict__ w,
+                                                          double *__restrict__ out) {
+  __shared__ double sh[256];
+  const uint64_t nd = d_off[n];
+  for (uint64_t e = blockIdx.x; e < nd; e += gridDim.x) {
+  const int I = row_of(d_off, n, e);
+  const size_t J = d_ket[e];
+  double s = 0.0;
+  for (int m = threadIdx.x; m < M; m += 256) s += w[(size_t)I * M + m] * w[J * M + m];
+  const double t = blk_sum256(s, sh);
+  if (threadIdx.x == 0) {
+    const uint32_t o = d_orb[e];
+    const int i = o & 255, j = (o >> 8) & 255, a = (o >> 16) & 255, b = (o >> 24) & 255;
+    const uint64_t bra = str[I];
+    double sg = parity1r(bra, i, a) * parity1r(bra, j, b);
+    if (a > j || b < i) sg = -sg;
+    out[e] = sg * t;
+  }
+  }
+}
+
+// OneDiff scatter for the single pairs of one spin (si = spin).
+__global__ void k_rdm_single_scatter(int n, int L, int spin, size_t ns,
+                                     const uint64_t *__restrict__ s_off,
+                                     const uint64_t *__restrict__ s_ent,
+                                     const uint64_t *__restrict__ str,
+                                     const double *__restrict__ T, double *__restrict__ one,
+                                     double *__restrict__ two) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ns) return;
+  const size_t L2 = (size_t)L * L, L4 = L2 * L2, L3 = L2 * L;
+  const int I = row_of(s_off, n, e);
+  const uint32_t pq = (uint32_t)((s_ent[e] >> 32) & 0x7fffffffull);
+  const int p = pq / L, q = pq % L;  // p = bra-only (cr), q = ket-only (an)
+  const double *t = T + e * (size_t)(L + 1);
+  const double t0 = t[0];
+  atomicAdd(&one[spin * L2 + p + (size_t)L * q], t0);
+  const size_t bs = (size_t)(3 * spin) * L4;  // same-spin block
+  uint64_t occ = str[I];
+  while (occ) {
+    const int k = __ffsll((unsigned long long)occ) - 1;
+    occ &= occ - 1;
+    if (k == p) continue;  // the excited orbital's own terms cancel exactly
+    atomicAdd(&two[bs + q + L * k + L2 * p + L3 * k], t0);
+    atomicAdd(&two[bs + k + L * q + L2 * k + L3 * p], t0);
+    atomicAdd(&two[bs + q + L * k + L2 * k + L3 * p], -t0);
+    atomicAdd(&two[bs + k + L * q + L2 * p + L3 * k], -t0);
+  }
+  const size_t b1 = (size_t)(spin + 2 * (1 - spin)) * L4;  // [si + 2 sj]
+  const size_t b2 = (size_t)((1 - spin) + 2 * spin) * L4;  // [sj + 2 si]
+  for (int k = 0; k < L; k++) {
+    const double tk = t[1 + k];
+    if (tk == 0.0) continue;
+    atomicAdd(&two[b1 + q + L * k + L2 * p + L3 * k], tk);
+    atomicAdd(&two[b2 + k + L * q + L2 * k + L3 * p], tk);
+  }
+}
+
+// TwoDiff scatter for same-spin double pairs (block 0 alpha, 3 beta).
+__global__ void k_rdm_double_scatter(int L, int spin, size_t nd, const uint32_t *__restrict__ d_orb,
+                                     const double *__restrict__ T, double *__restrict__ two) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nd) return;
+  const size_t L2 = (size_t)L * L, L4 = L2 * L2;
+  const uint32_t o = d_orb[e];
+  const size_t i = o & 255, j = (o >> 8) & 255, a = (o >> 16) & 255, b = (o >> 24) & 255;
+  const double t = T[e];
+  const size_t bs = (size_t)(3 * spin) * L4;
+  atomicAdd(&two[bs + a + L * b + L2 * (i + L * j)], t);
+  atomicAdd(&two[bs + b + L * a + L2 * (j + L * i)], t);
+  atomicAdd(&two[bs + a + L * b + L2 * (j + L * i)], -t);
+  atomicAdd(&two[bs + b + L * a + L2 * (i + L * j)], -t);
+}
+
+// Mixed term: Tm[pq*K + rs] += sum_{e in S_a(I)} s_e sum_{ib} w[I,ib] sum_{f in S_b(ib)} s_f w[J_e, jb_f]
+template <int NT>
+__global__ void __launch_bounds__(1024) k_rdm_mixed(int nb, int K, const uint64_t *__restrict__ a_off,
+                                                    const uint64_t *__restrict__ a_ent,
+                                                    const uint64_t *__restrict__ b_off,
+                                                    const uint64_t *__restrict__ b_ent,
+                                                    const double *__restrict__ w,
+                                                    double *__restrict__ Tm) {
+  extern __shared__ double lds[];
+  double *wrow = lds;
+  double *trow = lds + nb;
+  const int I = blockIdx.x;
+  const int tid = threadIdx.x;
+  double wi[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int ib = tid + t * 1024;
+    wi[t] = ib < nb ? w[(size_t)I * nb + ib] : 0.0;
+  }
+  for (uint64_t e = a_off[I]; e < a_off[I + 1]; e++) {
+    const uint64_t ent = a_ent[e];
+    const size_t J = (uint32_t)(ent & 0xffffffffull);
+    const double sa = (ent >> 63) ? -1.0 : 1.0;
+    const size_t pq = (ent >> 32) & 0x7fffffffull;
+    __syncthreads();
+    for (int i = tid; i < nb; i += 1024) wrow[i] = w[J * nb + i];
+    for (int i = tid; i < K; i += 1024) trow[i] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int ib = tid + t * 1024;
+      if (ib < nb && wi[t] != 0.0) {
+        for (uint64_t f = b_off[ib]; f < b_off[ib + 1]; f++) {
+          const uint64_t be = b_ent[f];
+          const double v = wi[t] * wrow[(uint32_t)(be & 0xffffffffull)];
+          atomicAdd(&trow[(be >> 32) & 0x7fffffffull], (be >> 63) ? -v : v);
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < K; i += 1024) {
+      const double v = trow[i];
+      if (v != 0.0) atomicAdd(&Tm[pq * K + i], sa * v);
+    }
+  }
+}
+
+// Place Tm[pq][rs] (alpha p->q, beta r->s) with TwoDiffCorrelation's rules on
+// the sorted spin orbitals I<J (bra-only 2p, 2r+1) and A<B (ket-only 2q, 2s+1).
+__global__ void k_rdm_mixed_place(int L, const double *__restrict__ Tm, double *__restrict__ two) {
+  const size_t K = (size_t)L * L;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= K * K) return;
+  const double t = Tm[idx];
+  if (t == 0.0) return;
+  const int pq = (int)(idx / K), rs = (int)(idx % K);
+  const int p = pq / L, q = pq % L, r = rs / L, s = rs % L;
+  const int i = 2 * p, j = 2 * r + 1, a = 2 * q, b = 2 * s + 1;
+  const int I = i < j ? i : j, J = i < j ? j : i, A = a < b ? a : b, B = a < b ? b : a;
+  const size_t oi = I >> 1, si = I & 1, oa = A >> 1, sa = A & 1;
+  const size_t oj = J >> 1, sj = J & 1, ob = B >> 1;
+  const size_t L2 = K, L4 = K * K;
+  // distinct (pq, rs) can land on one element, hence the atomics
+  if (si == sa) {
+    atomicAdd(&two[(si + 2 * sj) * L4 + oa + L * ob + L2 * (oi + L * oj)], t);
+    atomicAdd(&two[(sj + 2 * si) * L4 + ob + L * oa + L2 * (oj + L * oi)], t);
+  } else {
+    atomicAdd(&two[(si + 2 * sj) * L4 + oa + L * ob + L2 * (oj + L * oi)], t);
+    atomicAdd(&two[(sj + 2 * si) * L4 + ob + L * oa + L2 * (oi + L * oj)], t);
+  }
+}
+
+void transpose_into(Context &c, const double *in, double *out, int R, int Cc);
+
+template <int NT>
+static void launch_rdm_mixed(Context &c, const double *w, double *Tm, size_t lds) {
+  auto fn = k_rdm_mixed<NT>;
+  SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.A.s_off,
+                     c.A.s_ent, c.B.s_off, c.B.s_ent, w, Tm);
+  check_launch("k_rdm_mixed");
+}
+
+void compute_rdm(Context &c, const double *Wp, double *one, double *two) {
+  const int L = c.L;
+  const size_t L2 = (size_t)L * L, L4 = L2 * L2;
+  const int na = (int)c.na, nb = (int)c.nb;
+  ProfScope ps(c, "rdm");
+  SBDX_HIP(hipMemsetAsync(one, 0, sizeof(double) * 2 * L2, c.stream));
+  SBDX_HIP(hipMemsetAsync(two, 0, sizeof(double) * 4 * L4, c.stream));
+  // W'^T (beta-major) for the beta-side sums
+  transpose_into(c, Wp, c.T1, na, nb);
+  // ZeroDiff
+  TmpBuf b_rho_a(sizeof(double) * na), b_rho_b(sizeof(double) * nb),
+      b_mrow(sizeof(double) * (size_t)na * L);
+  double *rho_a = (double *)b_rho_a.p, *rho_b = (double *)b_rho_b.p, *mrow = (double *)b_mrow.p;
+  hipLaunchKernelGGL(k_rdm_diag_rows, dim3(na), dim3(256), 0, c.stream, nb, L, c.B.str, Wp, rho_a, mrow);
+  check_launch("k_rdm_diag_rows");
+  hipLaunchKernelGGL(k_rdm_diag_rows, dim3(nb), dim3(256), 0, c.stream, na, L, c.A.str, c.T1, rho_b,
+                     (double *)nullptr);
+  check_launch("k_rdm_diag_rows");
+  hipLaunchKernelGGL(k_rdm_diag_scatter, dim3(L), dim3(64), 0, c.stream, L, na, nb, c.A.str, c.B.str,
+                     rho_a, rho_b, mrow, one, two);
+  check_launch("k_rdm_diag_scatter");
+  // same-spin singles and doubles, alpha (rows of W') then beta (rows of W'^T)
+  for (int spin = 0; spin < 2; spin++) {
+    const SpinTables &t = spin ? c.B : c.A;
+    const SpinTables &o = spin ? c.A : c.B;
+    const double *w = spin ? c.T1 : Wp;
+    const int M = spin ? na : nb;
+    if (t.ns > 0) {
+      TmpBuf bts(sizeof(double) * t.ns * (L + 1));
+      double *Ts = (double *)bts.p;
+      hipLaunchKernelGGL(k_rdm_single_pairs, dim3((unsigned)std::min<size_t>(t.ns, 1u << 20)), dim3(256), 0, c.stream, t.n, M, L,
+                         t.s_off, t.s_ent, o.str, w, Ts);
+      check_launch("k_rdm_single_pairs");
+      hipLaunchKernelGGL(k_rdm_single_scatter, dim3(cdiv(t.ns, 256)), dim3(256), 0, c.stream, t.n, L,
+                         spin, t.ns, t.s_off, t.s_ent, t.str, Ts, one, two);
+      check_launch("k_rdm_single_scatter");
+      SBDX_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (t.nd > 0) {
+      TmpBuf btd(sizeof(double) * t.nd);
+      double *Td = (double *)btd.p;
+      hipLaunchKernelGGL(k_rdm_double_pairs, dim3((unsigned)std::min<size_t>(t.nd, 1u << 20)), dim3(256), 0, c.stream, t.n, M,
+                         t.d_off, t.d_ket, t.d_orb, t.str, w, Td);
+      check_launch("k_rdm_double_pairs");
+      hipLaunchKernelGGL(k_rdm_double_scatter, dim3(cdiv(t.nd, 256)), dim3(256), 0, c.stream, L, spin,
+                         t.nd, t.d_orb, Td, two);
+      check_launch("k_rdm_double_scatter");
+      SBDX_HIP(hipStreamSynchronize(c.stream));
+    }
+  }
+  // mixed alpha-single x beta-single
+  const size_t K = (size_t)c.K;
+  TmpBuf btm(sizeof(double) * K * K);
+  double *Tm = (double *)btm.p;
+  SBDX_HIP(hipMemsetAsync(Tm, 0, sizeof(double) * K * K, c.stream));
+  const size_t lds = sizeof(double) * ((size_t)nb + K);
+  const int nt = (nb + 1023) / 1024;
+  if (lds > c.lds_limit || nt > 32)
+    throw Error(3, "rdm: beta string count too large for the LDS-staged mixed RDM kernel");
+  if (nt <= 1) launch_rdm_mixed<1>(c, Wp, Tm, lds);
+  else if (nt <= 2) launch_rdm_mixed<2>(c, Wp, Tm, lds);
+  else if (nt <= 4) launch_rdm_mixed<4>(c, Wp, Tm, lds);
+  else if (nt <= 8) launch_rdm_mixed<8>(c, Wp, Tm, lds);
+  else if (nt <= 16) launch_rdm_mixed<16>(c, Wp, Tm, lds);
+  else launch_rdm_mixed<32>(c, Wp, Tm, lds);
+  hipLaunchKernelGGL(k_rdm_mixed_place, dim3(cdiv(K * K, 256)), dim3(256), 0, c.stream, L, Tm, two);
+  check_launch("k_rdm_mixed_place");
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace sbdx

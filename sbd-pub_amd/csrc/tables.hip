@@ -1,0 +1,269 @@
+// Excitation tables, dressing tables, the diagonal and the basis sign.
+//
+// Table construction follows the semantics of GenerateExcitation
+// (chemistry/tpb/helper.h:185-323): for every bra string, every ket string of
+// the same set at Hamming distance 2 (single) or 4 (double), in ascending ket
+// index.  One 64-lane wavefront owns one bra string and sweeps the kets 64 at
+// a time; popcount of the XOR gives the distance and the wave ballot + a
+// masked popcount give each match its slot, so the order is ascending
+// without sorting.
+#include "common.h"
+
+namespace sbdx {
+
+__device__ __forceinline__ double parity1(uint64_t s, int a, int b) {
+  // Reference parity rule (chemistry/basic/determinants.h:230-281) on one
+  // string: bits in [lo, hi), one more flip when bit lo is set.
+  int lo = a < b ? a : b, hi = a < b ? b : a;
+  uint64_t mask = ((1ull << hi) - 1ull) ^ ((1ull << lo) - 1ull);
+  int n = __popcll(s & mask) + (int)((s >> lo) & 1ull);
+  return (n & 1) ? -1.0 : 1.0;
+}
+
+__global__ void __launch_bounds__(256) k_table_count(const uint64_t *__restrict__ s, int n,
+                                                     uint32_t *__restrict__ cs,
+                                                     uint32_t *__restrict__ cd) {
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (wave >= n) return;  // uniform per wave
+  const uint64_t me = s[wave];
+  uint32_t a = 0, b = 0;
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    const int j = j0 + lane;
+    const int d = (j < n) ? __popcll(me ^ s[j]) : 0;
+    a += (uint32_t)__popcll(__ballot(d == 2));
+    b += (uint32_t)__popcll(__ballot(d == 4));
+  }
+  if (lane == 0) {
+    cs[wave] = a;
+    cd[wave] = b;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_table_fill(
+    const uint64_t *__restrict__ s, int n, int L, const uint64_t *__restrict__ s_off,
+    const uint64_t *__restrict__ d_off, const double *__restrict__ eri,
+    const double *__restrict__ h1, uint64_t *__restrict__ s_ent, double *__restrict__ s_val,
+    uint32_t *__restrict__ d_ket, uint32_t *__restrict__ d_orb, double *__restrict__ d_val) {
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (wave >= n) return;
+  const int K = L * L;
+  const uint64_t me = s[wave];
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint64_t ps = s_off[wave], pd = d_off[wave];
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    const int j = j0 + lane;
+    const uint64_t sj = (j < n) ? s[j] : me;
+    const int d = __popcll(me ^ sj);
+    const uint64_t m2 = __ballot(d == 2), m4 = __ballot(d == 4);
+    if (d == 2) {
+      const uint64_t pos = ps + (uint64_t)__popcll(m2 & below);
+      const int p = __ffsll((unsigned long long)(me & ~sj)) - 1;
+      const int q = __ffsll((unsigned long long)(sj & ~me)) - 1;
+      const double sg = parity1(me, p, q);
+      const int pq = p * L + q;
+      double F = h1[pq];
+      uint64_t occ = me;
+      while (occ) {
+        const int k = __ffsll((unsigned long long)occ) - 1;
+        occ &= occ - 1;
+        F += eri[(size_t)pq * K + k * L + k] - eri[(size_t)(p * L + k) * K + k * L + q];
+      }
+      s_ent[pos] = (uint64_t)j | ((uint64_t)pq << 32) | (sg < 0 ? (1ull << 63) : 0ull);
+      s_val[pos] = sg * F;
+    } else if (d == 4) {
+      const uint64_t pos = pd + (uint64_t)__popcll(m4 & below);
+      const uint64_t cr = me & ~sj, an = sj & ~me;
+      const int i = __ffsll((unsigned long long)cr) - 1, jj = 63 - __clzll(cr);
+      const int a = __ffsll((unsigned long long)an) - 1, b = 63 - __clzll(an);
+      double sg = parity1(me, i, a) * parity1(me, jj, b);
+      if (a > jj || b < i) sg = -sg;
+      const double v = eri[(size_t)(a * L + i) * K + b * L + jj] -
+                       eri[(size_t)(a * L + jj) * K + b * L + i];
+      d_ket[pos] = (uint32_t)j;
+      d_orb[pos] = (uint32_t)i | ((uint32_t)jj << 8) | ((uint32_t)a << 16) | ((uint32_t)b << 24);
+      d_val[pos] = sg * v;
+    }
+    ps += (uint64_t)__popcll(m2);
+    pd += (uint64_t)__popcll(m4);
+  }
+}
+
+// dress[pq*n + x] = sum_{k in str[x]} (pq|kk): the part of an other-spin
+// single excitation p->q that depends on this spin's string x.
+__global__ void __launch_bounds__(256) k_dress(const uint64_t *__restrict__ s, int n, int L,
+                                               const double *__restrict__ eri,
+                                               double *__restrict__ dress) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pq = blockIdx.y;
+  if (x >= n) return;
+  const int K = L * L;
+  const double *row = eri + (size_t)pq * K;
+  uint64_t occ = s[x];
+  double acc = 0.0;
+  while (occ) {
+    const int k = __ffsll((unsigned long long)occ) - 1;
+    occ &= occ - 1;
+    acc += row[k * L + k];
+  }
+  dress[(size_t)pq * n + x] = acc;
+}
+
+// Per-string pieces of the diagonal: e1[x] = sum_p h_pp + sum_{p<q}
+// ((pp|qq) - (pq|qp)) and jsum[x][r] = sum_{p in x} (pp|rr).
+__global__ void __launch_bounds__(256) k_string_energy(const uint64_t *__restrict__ s, int n,
+                                                       int L, const double *__restrict__ eri,
+                                                       const double *__restrict__ h1,
+                                                       double *__restrict__ e1,
+                                                       double *__restrict__ jsum) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const int K = L * L;
+  const uint64_t str = s[x];
+  double e = 0.0;
+  for (int p = 0; p < L; p++) {
+    if (!((str >> p) & 1ull)) continue;
+    e += h1[p * L + p];
+    for (int q = p + 1; q < L; q++) {
+      if (!((str >> q) & 1ull)) continue;
+      e += eri[(size_t)(p * L + p) * K + q * L + q] - eri[(size_t)(p * L + q) * K + q * L + p];
+    }
+  }
+  e1[x] = e;
+  for (int r = 0; r < L; r++) {
+    double j = 0.0;
+    for (int p = 0; p < L; p++)
+      if ((str >> p) & 1ull) j += eri[(size_t)(p * L + p) * K + r * L + r];
+    jsum[(size_t)x * L + r] = j;
+  }
+}
+
+// hii[ia*nb+ib] = I0 + e1a[ia] + e1b[ib] + sum_{r in beta(ib)} jsum_a[ia][r]
+// (ZeroExcite, chemistry/basic/determinants.h:434-457, split by spin).
+__global__ void __launch_bounds__(256) k_diag(int nb, int nchunk, int L, double I0,
+                                              const uint64_t *__restrict__ bstr,
+                                              const double *__restrict__ e1a,
+                                              const double *__restrict__ e1b,
+                                              const double *__restrict__ jsa,
+                                              double *__restrict__ hii) {
+  const int ia = blockIdx.x / nchunk;
+  const int ib = (blockIdx.x % nchunk) * blockDim.x + threadIdx.x;
+  if (ib >= nb) return;
+  const double *j = jsa + (size_t)ia * L;
+  uint64_t occ = bstr[ib];
+  double acc = 0.0;
+  while (occ) {
+    const int r = __ffsll((unsigned long long)occ) - 1;
+    occ &= occ - 1;
+    acc += j[r];
+  }
+  hii[(size_t)ia * nb + ib] = I0 + e1a[ia] + e1b[ib] + acc;
+}
+
+// out = S * in, S(ia,ib) = (-1)^#{(p in alpha, r in beta): r < p}: the sign
+// between the interleaved (reference) and the alpha-first basis.
+template <bool ACC>
+__global__ void __launch_bounds__(256) k_sign(int nb, int nchunk, const uint64_t *__restrict__ astr,
+                                              const uint64_t *__restrict__ bstr,
+                                              const double *__restrict__ in,
+                                              double *__restrict__ out) {
+  const int ia = blockIdx.x / nchunk;
+  const int ib = (blockIdx.x % nchunk) * blockDim.x + threadIdx.x;
+  if (ib >= nb) return;
+  uint64_t a = astr[ia];
+  const uint64_t b = bstr[ib];
+  int cnt = 0;
+  while (a) {
+    const int p = __ffsll((unsigned long long)a) - 1;
+    a &= a - 1;
+    cnt += __popcll(b & ((1ull << p) - 1ull));
+  }
+  const size_t idx = (size_t)ia * nb + ib;
+  const double v = (cnt & 1) ? -in[idx] : in[idx];
+  out[idx] = ACC ? out[idx] + v : v;
+}
+
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+void build_spin_tables(Context &c, SpinTables &t, const std::vector<uint64_t> &strs) {
+  const int n = (int)strs.size();
+  t.n = n;
+  t.str = (uint64_t *)dalloc(c, sizeof(uint64_t) * n);
+  SBDX_HIP(hipMemcpyAsync(t.str, strs.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice, c.stream));
+  uint32_t *cnt = (uint32_t *)dalloc(c, sizeof(uint32_t) * 2 * (size_t)n);
+  {
+    ProfScope ps(c, "table_count");
+    hipLaunchKernelGGL(k_table_count, dim3(cdiv((size_t)n * 64, 256)), dim3(256), 0, c.stream,
+                       t.str, n, cnt, cnt + n);
+    check_launch("k_table_count");
+  }
+  std::vector<uint32_t> hc(2 * (size_t)n);
+  SBDX_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(uint32_t) * 2 * n, hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  std::vector<uint64_t> so(n + 1, 0), dof(n + 1, 0);
+  size_t mx = 0;
+  for (int i = 0; i < n; i++) {
+    so[i + 1] = so[i] + hc[i];
+    dof[i + 1] = dof[i] + hc[n + i];
+    if (hc[i] > mx) mx = hc[i];
+  }
+  t.ns = so[n];
+  t.nd = dof[n];
+  t.max_s = mx;
+  t.s_off = (uint64_t *)dalloc(c, sizeof(uint64_t) * (n + 1));
+  t.d_off = (uint64_t *)dalloc(c, sizeof(uint64_t) * (n + 1));
+  SBDX_HIP(hipMemcpyAsync(t.s_off, so.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(t.d_off, dof.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, c.stream));
+  t.s_ent = (uint64_t *)dalloc(c, sizeof(uint64_t) * (t.ns + 1));
+  t.s_val = (double *)dalloc(c, sizeof(double) * (t.ns + 1));
+  t.d_ket = (uint32_t *)dalloc(c, sizeof(uint32_t) * (t.nd + 1));
+  t.d_orb = (uint32_t *)dalloc(c, sizeof(uint32_t) * (t.nd + 1));
+  t.d_val = (double *)dalloc(c, sizeof(double) * (t.nd + 1));
+  {
+    ProfScope ps(c, "table_fill");
+    hipLaunchKernelGGL(k_table_fill, dim3(cdiv((size_t)n * 64, 256)), dim3(256), 0, c.stream, t.str,
+                       n, c.L, t.s_off, t.d_off, c.eri, c.h1, t.s_ent, t.s_val, t.d_ket, t.d_orb,
+                       t.d_val);
+    check_launch("k_table_fill");
+  }
+  t.e1 = (double *)dalloc(c, sizeof(double) * n);
+  t.jsum = (double *)dalloc(c, sizeof(double) * (size_t)n * c.L);
+  hipLaunchKernelGGL(k_string_energy, dim3(cdiv(n, 256)), dim3(256), 0, c.stream, t.str, n, c.L,
+                     c.eri, c.h1, t.e1, t.jsum);
+  check_launch("k_string_energy");
+}
+
+void build_dress(Context &c, SpinTables &t) {
+  t.dress = (double *)dalloc(c, sizeof(double) * (size_t)c.K * t.n);
+  ProfScope ps(c, "dress");
+  hipLaunchKernelGGL(k_dress, dim3(cdiv(t.n, 256), c.K), dim3(256), 0, c.stream, t.str, t.n, c.L,
+                     c.eri, t.dress);
+  check_launch("k_dress");
+}
+
+void compute_diag(Context &c) {
+  const unsigned nchunk = cdiv(c.nb, 256);
+  ProfScope ps(c, "diag");
+  hipLaunchKernelGGL(k_diag, dim3((unsigned)(c.na * nchunk)), dim3(256), 0, c.stream, (int)c.nb,
+                     (int)nchunk, c.L, c.I0, c.B.str, c.A.e1, c.B.e1, c.A.jsum, c.hii);
+  check_launch("k_diag");
+}
+
+void sign_convert(Context &c, const double *in, double *out) {
+  const unsigned nchunk = cdiv(c.nb, 256);
+  ProfScope ps(c, "sign");
+  hipLaunchKernelGGL(k_sign<false>, dim3((unsigned)(c.na * nchunk)), dim3(256), 0, c.stream,
+                     (int)c.nb, (int)nchunk, c.A.str, c.B.str, in, out);
+  check_launch("k_sign");
+}
+
+void sign_convert_add(Context &c, const double *in, double *out) {
+  const unsigned nchunk = cdiv(c.nb, 256);
+  ProfScope ps(c, "sign");
+  hipLaunchKernelGGL(k_sign<true>, dim3((unsigned)(c.na * nchunk)), dim3(256), 0, c.stream,
+                     (int)c.nb, (int)nchunk, c.A.str, c.B.str, in, out);
+  check_launch("k_sign");
+}
+
+}  // namespace sbdx

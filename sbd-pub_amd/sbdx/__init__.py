@@ -1,0 +1,256 @@
+"""sbdx -- MI355X implementation of the sbd-pub TPB Hamiltonian path.
+
+Python mirror of the reference's operator and solver interface, bound to
+the C ABI of libsbdx.so (include/sbdx.h):
+
+* ``TPBOperator``  ~ ``MultTPBThrust`` (chemistry/tpb/mult_thrust.h:23-95):
+  ``run`` (MultBase::run, Wb += H Wk), ``make_diag`` (makeQChamDiagTerms),
+  ``correlation`` (1-/2-RDM).
+* ``davidson``     ~ ``sbd::Davidson(hii, W, mult, max_it, nb, eps, max_time)``
+  (chemistry/basic/davidson_thrust.h:85-92).
+* ``diag``         ~ ``sbd::tpb::diag`` (chemistry/tpb/sbdiag.h:139-151):
+  energy, density, 1-/2-RDM from an FCIDUMP and string sets.
+
+There is no CPU fallback: if libsbdx.so is missing or no GPU is visible, the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import io
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsbdx.so")
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+ERRORS = {1: "invalid argument", 2: "HIP failure", 3: "unsupported instance",
+          4: "device allocation failed", 5: "not converged", 6: "I/O error"}
+
+# every entry point of include/sbdx.h
+ABI_SYMBOLS = [
+    "sbdx_last_error", "sbdx_version", "sbdx_create", "sbdx_destroy", "sbdx_dims", "sbdx_diag",
+    "sbdx_sigma", "sbdx_sigma_device", "sbdx_davidson", "sbdx_set_vector", "sbdx_solve",
+    "sbdx_get_vector", "sbdx_trace", "sbdx_energy", "sbdx_rdm", "sbdx_table_count",
+    "sbdx_table_export", "sbdx_profile", "sbdx_profile_count", "sbdx_profile_get",
+    "sbdx_work_counts",
+]
+
+
+class SbdxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sbdx error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libsbdx.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not built: run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    L.sbdx_last_error.restype = C.c_char_p
+    L.sbdx_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_size_t, _u64p, C.c_size_t, _u64p,
+                              C.c_size_t, C.c_double, _dp, _dp, C.c_size_t]
+    L.sbdx_destroy.argtypes = [C.c_void_p]
+    L.sbdx_dims.argtypes = [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
+                            C.POINTER(C.c_size_t)]
+    L.sbdx_diag.argtypes = [C.c_void_p, _dp]
+    L.sbdx_sigma.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+    L.sbdx_sigma_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    L.sbdx_davidson.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.sbdx_set_vector.argtypes = [C.c_void_p, _dp]
+    L.sbdx_solve.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                             C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.sbdx_get_vector.argtypes = [C.c_void_p, _dp]
+    L.sbdx_trace.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    L.sbdx_energy.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_double)]
+    L.sbdx_rdm.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.sbdx_table_count.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_size_t)]
+    L.sbdx_table_export.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]
+    L.sbdx_profile.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_profile_count.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.sbdx_profile_get.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_size_t,
+                                   C.POINTER(C.c_long), C.POINTER(C.c_double)]
+    L.sbdx_work_counts.argtypes = [C.c_void_p, _dp]
+    _LIB = L
+    return L
+
+
+def _check(rc, allow=()):
+    if rc != 0 and rc not in allow:
+        raise SbdxError(rc, lib().sbdx_last_error().decode())
+    return rc
+
+
+class TPBOperator:
+    """Device-resident TPB Hamiltonian of one (alpha strings x beta strings)
+    space; builds the excitation tables and the diagonal on construction
+    (MultTPBThrust::Init + MakeHelpers for one rank)."""
+
+    def __init__(self, norb, I0, I1, I2, astr, bstr=None, bit_length=20):
+        astr = np.ascontiguousarray(astr, dtype=np.uint64)
+        bstr = astr if bstr is None else np.ascontiguousarray(bstr, dtype=np.uint64)
+        self.norb = int(norb)
+        self.na, self.nb = astr.size, bstr.size
+        self.ndet = self.na * self.nb
+        aw = np.ascontiguousarray(io.masks_to_words(astr, norb, bit_length).reshape(-1))
+        bw = np.ascontiguousarray(io.masks_to_words(bstr, norb, bit_length).reshape(-1))
+        I1 = np.ascontiguousarray(I1, dtype=np.float64).reshape(-1)
+        I2 = np.ascontiguousarray(I2, dtype=np.float64).reshape(-1)
+        h = C.c_void_p()
+        _check(lib().sbdx_create(C.byref(h), self.norb, bit_length, aw, self.na, bw, self.nb,
+                                 float(I0), I1, I2, I2.size))
+        self.h = h
+
+    @classmethod
+    def from_instance(cls, inst, bit_length=20):
+        return cls(inst["norb"], inst["I0"], inst["I1"], inst["I2"], inst["astr"], inst["bstr"],
+                   bit_length)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sbdx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def make_diag(self):
+        out = np.empty(self.ndet)
+        _check(lib().sbdx_diag(self.h, out))
+        return out
+
+    def run(self, Wk, Wb=None, accumulate=True):
+        """Wb += H*Wk (MultBase::run semantics); returns Wb."""
+        Wk = np.ascontiguousarray(Wk, dtype=np.float64)
+        if Wb is None:
+            Wb = np.zeros(self.ndet)
+            accumulate = False
+        _check(lib().sbdx_sigma(self.h, Wk, Wb, 1 if accumulate else 0))
+        return Wb
+
+    def sigma(self, Wk):
+        return self.run(Wk, None, accumulate=False)
+
+    def energy(self, W):
+        e = C.c_double()
+        _check(lib().sbdx_energy(self.h, np.ascontiguousarray(W, dtype=np.float64), C.byref(e)))
+        return e.value
+
+    def correlation(self, W):
+        """(one, two): one[s*L*L + i + L*j], two[(s+2t)*L^4 + i + L*j + L^2*k + L^3*l]."""
+        L = self.norb
+        one = np.zeros(2 * L * L)
+        two = np.zeros(4 * L ** 4)
+        _check(lib().sbdx_rdm(self.h, np.ascontiguousarray(W, dtype=np.float64), one, two))
+        return one, two
+
+    def set_vector(self, W):
+        _check(lib().sbdx_set_vector(self.h, np.ascontiguousarray(W, dtype=np.float64)))
+
+    def get_vector(self):
+        out = np.empty(self.ndet)
+        _check(lib().sbdx_get_vector(self.h, out))
+        return out
+
+    def solve(self, max_it=100, nb=12, eps=1e-8, max_time=1e30):
+        e = C.c_double()
+        n = C.c_int()
+        rc = _check(lib().sbdx_solve(self.h, max_it, nb, eps, max_time, C.byref(e), C.byref(n)),
+                    allow=(5,))
+        return e.value, n.value, rc == 0
+
+    def trace(self):
+        n = C.c_int()
+        _check(lib().sbdx_trace(self.h, None, 0, C.byref(n)))
+        buf = np.zeros(2 * max(n.value, 1))
+        _check(lib().sbdx_trace(self.h, buf.ctypes.data, n.value, C.byref(n)))
+        return buf[: 2 * n.value].reshape(n.value, 2)
+
+    def table(self, spin, kind):
+        n = self.na if spin == 0 else self.nb
+        cnt = C.c_size_t()
+        _check(lib().sbdx_table_count(self.h, spin, kind, C.byref(cnt)))
+        off = np.zeros(n + 1, dtype=np.uint64)
+        ket = np.zeros(max(cnt.value, 1), dtype=np.uint32)
+        cran = np.zeros(max(cnt.value, 1) * 2 * kind, dtype=np.int32)
+        _check(lib().sbdx_table_export(self.h, spin, kind, off.ctypes.data, ket.ctypes.data,
+                                       cran.ctypes.data))
+        c = cnt.value
+        return off.astype(np.int64), ket[:c], cran[: c * 2 * kind]
+
+    def work_counts(self):
+        out = np.zeros(6)
+        _check(lib().sbdx_work_counts(self.h, out))
+        return dict(alpha_singles=out[0], alpha_doubles=out[1], beta_singles=out[2],
+                    beta_doubles=out[3], mixed_terms=out[4], ndet=out[5])
+
+    def profile(self, enable=True):
+        _check(lib().sbdx_profile(self.h, 1 if enable else 0))
+
+    def profile_stats(self):
+        n = C.c_int()
+        _check(lib().sbdx_profile_count(self.h, C.byref(n)))
+        out = {}
+        name = C.create_string_buffer(128)
+        for i in range(n.value):
+            cnt = C.c_long()
+            ms = C.c_double()
+            _check(lib().sbdx_profile_get(self.h, i, name, 128, C.byref(cnt), C.byref(ms)))
+            out[name.value.decode()] = (cnt.value, ms.value)
+        return out
+
+
+def basis_init_vector(ndet):
+    """W = e_0 (the Hartree-Fock determinant is index 0 after sorting),
+    BasisInitVector with init == 0 (chemistry/tpb/davidson.h:104-144)."""
+    W = np.zeros(ndet)
+    W[0] = 1.0
+    return W
+
+
+def davidson(op: TPBOperator, W=None, max_it=100, nb=12, eps=1e-8, max_time=1e30):
+    """Davidson on the device; returns (energy, W, n_steps, converged)."""
+    if W is None:
+        W = basis_init_vector(op.ndet)
+    W = np.ascontiguousarray(W, dtype=np.float64).copy()
+    e = C.c_double()
+    n = C.c_int()
+    rc = _check(lib().sbdx_davidson(op.h, W, max_it, nb, eps, max_time, C.byref(e), C.byref(n)),
+                allow=(5,))
+    return e.value, W, n.value, rc == 0
+
+
+def diag(fcidump_path, adet_path, bdet_path=None, max_it=100, nb=12, eps=1e-8, do_rdm=True,
+         bit_length=20):
+    """FCIDUMP + string files -> energy <W|HW>, density and RDMs, like
+    sbd::tpb::diag (chemistry/tpb/sbdiag.h:139-763) on one GPU."""
+    fd = io.read_fcidump(fcidump_path)
+    I0, I1, I2 = io.setup_integrals(fd)
+    L = fd["norb"]
+    a = io.read_strings(adet_path, L)
+    b = a if bdet_path is None else io.read_strings(bdet_path, L)
+    op = TPBOperator(L, I0, I1, I2, a, b, bit_length)
+    _, W, steps, conv = davidson(op, max_it=max_it, nb=nb, eps=eps)
+    E = op.energy(W)
+    out = dict(energy=E, W=W, steps=steps, converged=conv, norb=L)
+    if do_rdm:
+        one, two = op.correlation(W)
+        out["one"], out["two"] = one, two
+        out["density"] = np.array([one[s * L * L + i + L * i] for i in range(L) for s in range(2)])
+    op.close()
+    return out

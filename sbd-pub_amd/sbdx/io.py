@@ -21,9 +21,15 @@ Formats (reference citations relative to the sbd-pub tree):
 """
 from __future__ import annotations
 
+import gzip
+
 import numpy as np
 
 MAX_NORB = 64
+
+
+def _open_text(path):
+    return gzip.open(path, "rt") if str(path).endswith(".gz") else open(path, "r")
 
 
 def tri(a, b):
@@ -47,7 +53,7 @@ def read_fcidump(path):
     vals = []
     idx = []
     in_header = True
-    with open(path, "r") as f:
+    with _open_text(path) as f:
         for line in f:
             line = line.rstrip("\n")
             if not line or line[0] == "!":
@@ -177,7 +183,7 @@ def read_strings(path, norb):
     if norb > MAX_NORB:
         raise RuntimeError(f"norb={norb} > {MAX_NORB} is not supported")
     vals = []
-    with open(path) as f:
+    with _open_text(path) as f:
         for line in f:
             if line.strip():
                 vals.append(parse_string(line, norb))

@@ -1,0 +1,112 @@
+"""Checks against golden vectors written by the reference implementation
+itself (tests/golden/make_golden.py runs the reference CPU path built from
+its own sources).  The oracle is checked on the CPU, libsbdx on the GPU."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, molecule
+
+CASES = ["h2o_1em3", "n2_1em3", "n2_3em4", "syn10"]
+# Cases whose Davidson result is compared.  syn10's ground state has exactly
+# zero Hartree-Fock weight (another spin-symmetry sector): from the HF start
+# vector the reference reaches it only through rounding noise after ~250
+# steps, so its solver result is not a reproducible target; its tables,
+# diagonal and sigma still are.
+SOLVE = {"h2o_1em3", "n2_1em3", "n2_3em4"}
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLDEN, f"ref_{name}.npz"), allow_pickle=False))
+
+
+def instance(name):
+    from sbdx import instances
+    if name == "syn10":
+        return instances.make_instance(norb=10, nelec_half=3, nstr=80, nchol=10, seed=77)
+    if name == "tiny16":
+        return instances.workload("tiny16")
+    mol, sub = name.split("_")
+    return molecule(mol, sub)
+
+
+def seeded_vector(g):
+    v = np.random.default_rng(int(g["seed"])).standard_normal(int(g["ndet"]))
+    return v / np.linalg.norm(v)
+
+
+def table_sha(off, ket, cran):
+    blob = np.concatenate([np.diff(off).astype(np.int64), ket.astype(np.int64), cran.astype(np.int64)])
+    return np.frombuffer(hashlib.sha256(blob.tobytes()).digest(), dtype=np.uint8)
+
+
+def check_all(g, tables, hii, sigma, davidson, rdm, tol_e=1e-10):
+    for nm, (spin, kind) in {"sa": (0, 1), "da": (0, 2), "sb": (1, 1), "db": (1, 2)}.items():
+        off, ket, cran = tables(spin, kind)
+        assert int(g[f"table_{nm}_count"]) == ket.size, nm
+        assert np.array_equal(table_sha(off, ket, cran), g[f"table_{nm}_sha"]), nm
+    h = hii()
+    assert np.max(np.abs(h[g["hii_idx"]] - g["hii_val"])) < 1e-12 * np.abs(g["hii_val"]).max()
+    assert abs(h.sum() - g["hii_sum"]) < 1e-11 * abs(g["hii_sum"])
+    v = seeded_vector(g)
+    s = sigma(v)
+    scale = np.abs(g["sigma_val"]).max()
+    assert np.max(np.abs(s[g["sigma_idx"]] - g["sigma_val"])) < 1e-12 * scale
+    assert abs(np.linalg.norm(s) - g["sigma_norm"]) < 1e-12 * g["sigma_norm"]
+    assert abs(v @ s - g["sigma_vhv"]) < 1e-11 * max(1.0, abs(g["sigma_vhv"]))
+    if davidson is None:
+        return
+    E, W, steps = davidson()
+    assert abs(E - float(g["energy"])) < tol_e, (E, float(g["energy"]))
+    traj = g["trajectory"]
+    assert abs(steps - traj.shape[0]) <= 2, (steps, traj.shape)
+    assert abs((W * W).sum() - g["W_sq"]) < 1e-12
+    assert np.max(np.abs(np.abs(W[g["W_idx"]]) - g["W_abs"])) < 1e-6
+    if rdm is not None and "rdm1" in g:
+        one, two = rdm(W)
+        assert np.max(np.abs(one - g["rdm1"])) < 1e-9
+        two = two.reshape(4, -1)
+        assert np.max(np.abs(two[:, g["rdm2_idx"]] - g["rdm2_val"])) < 1e-9
+        assert np.max(np.abs(two.sum(axis=1) - g["rdm2_sum"])) < 1e-8
+        assert np.max(np.abs(np.linalg.norm(two, axis=1) - g["rdm2_norm"])) < 1e-8
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_matches_reference(oracle_mod, name):
+    g = load(name)
+    inst = instance(name)
+    orc = oracle_mod.Oracle.from_instance(inst)
+    hii = orc.diag()
+
+    def dav():
+        r = orc.davidson(nb=12 if name == "syn10" else 10, eps=1e-8, max_it=100, hii=hii)
+        return r["E"], r["W"], r["steps"]
+
+    def rdm(W):
+        o1, o2 = orc.rdm(W)
+        return o1.ravel(), o2.ravel()
+
+    solve = name in SOLVE and name != "n2_3em4"  # keep the CPU suite short
+    check_all(g, orc.table, lambda: hii, lambda v: orc.sigma(v, hii), dav if solve else None,
+              rdm if solve else None)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_matches_reference(name):
+    import sbdx
+    g = load(name)
+    inst = instance(name)
+    print(f"[golden {name}] ndet={int(g['ndet'])}", flush=True)
+    op = sbdx.TPBOperator.from_instance(inst)
+
+    def dav():
+        E, W, steps, conv = sbdx.davidson(op, nb=12 if name == "syn10" else 10, eps=1e-8, max_it=100)
+        assert conv
+        return op.energy(W), W, steps
+
+    check_all(g, op.table, op.make_diag, op.sigma, dav if name in SOLVE else None,
+              op.correlation)
+    op.close()
