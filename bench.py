@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark of the TPB path on one MI355X.
+
+Workload (default c36): norb=36, n_alpha=n_beta=8, 8000 x 8000 strings
+(Hartree-Fock + singles + seeded doubles), 64M determinants, integrals from
+150 seeded Cholesky vectors; plain FP64.  A step is one Davidson solve from
+the start vector to |E0 W - H W| < eps, with the vector resident in HBM when
+the timed region starts.  value = time-to-E0 in seconds (lower is better);
+sigma throughput, per-kernel roofline figures and a CPU baseline measured on
+a bounded sample with the oracle are reported beside it.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "sbd-pub_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "time-to-E0 (|ΔE|<1e-8 Ha) & sigma GB/s: mixed+learned vs plain FP64, 1 GPU"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_bytes(wc, na, nb):
+    """Algorithmic HBM bytes per launch of each sigma kernel group (FP64).
+    same-spin: every (entry x other-spin column) reads x[J,m] (+ the dressing
+    dress[pq,m] for singles), plus diagonal, hii and output streams;
+    mixed (direct form): one x gather per (alpha single x beta single) term
+    plus reading and writing sigma; transposes: one read + one write."""
+    e = 8.0
+    N = na * nb
+    ab = wc["alpha_singles"] * wc["beta_singles"]
+    return {
+        "same_alpha": e * (nb * (2 * wc["alpha_singles"] + wc["alpha_doubles"]) + 3 * N)
+        + 16.0 * (wc["alpha_singles"] + wc["alpha_doubles"]),
+        "same_beta": e * (na * (2 * wc["beta_singles"] + wc["beta_doubles"]) + 2 * N)
+        + 16.0 * (wc["beta_singles"] + wc["beta_doubles"]),
+        "mixed": e * (ab + 2 * N) + 8.0 * (wc["alpha_singles"] + wc["beta_singles"]),
+        "transpose": 2 * e * N,
+        "transpose_add": 3 * e * N,
+    }
+
+
+def cpu_baseline(inst, n_steps, seconds, nthreads):
+    """Oracle sigma on a random sample of alpha rows, scaled to one full
+    sigma and multiplied by the GPU solve's sigma count."""
+    from oracle import oracle
+    orc = oracle.Oracle.from_instance(inst, nthreads)
+    n = orc.ndet
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(n)
+    hii = np.zeros(n)
+    y = np.zeros(n)
+    rows = rng.permutation(orc.na)
+    # calibrate on a few rows, then time a sample sized to ~`seconds`
+    t0 = time.perf_counter()
+    orc.sigma_rowlist(x, hii, rows[:nthreads], y)
+    t_cal = (time.perf_counter() - t0) / nthreads
+    k = int(max(nthreads, min(orc.na, seconds * nthreads / max(t_cal, 1e-6))))
+    k = max(nthreads, (k // nthreads) * nthreads)
+    sample = rows[nthreads:nthreads + k]
+    t0 = time.perf_counter()
+    orc.sigma_rowlist(x, hii, sample, y)
+    t = time.perf_counter() - t0
+    orc.close()
+    t_sigma = t * orc.na / sample.size
+    return dict(value=t_sigma * n_steps, unit="s", cores=nthreads, kind="port",
+                sample=f"oracle sigma on {sample.size} of {orc.na} alpha rows (random, seed 0) "
+                       f"in {t:.1f} s with {nthreads} threads, scaled x{orc.na / sample.size:.1f} "
+                       f"to one sigma ({t_sigma:.1f} s) and x{n_steps} sigma steps of the GPU solve",
+                sigma_s=t_sigma)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1, help="timed Davidson solves")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed sigma applications")
+    ap.add_argument("--workload", default="c36")
+    ap.add_argument("--nb", type=int, default=12)
+    ap.add_argument("--eps", type=float, default=1e-8)
+    ap.add_argument("--max-it", type=int, default=200)
+    ap.add_argument("--max-time", type=float, default=900.0)
+    ap.add_argument("--init", choices=["hf", "random"], default="hf")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    import sbdx
+    from sbdx import instances
+
+    spec = instances.WORKLOADS[args.workload]
+    log(f"building instance {args.workload}: {spec}")
+    inst = instances.workload(args.workload)
+    t0 = time.perf_counter()
+    op = sbdx.TPBOperator.from_instance(inst)
+    t_build = time.perf_counter() - t0
+    na, nb = op.na, op.nb
+    wc = op.work_counts()
+    log(f"operator built in {t_build:.2f} s: ndet={op.ndet} counts={wc}")
+
+    if args.init == "hf":
+        W0 = sbdx.basis_init_vector(op.ndet)
+    else:
+        W0 = np.random.default_rng(1).standard_normal(op.ndet)
+        W0 /= np.linalg.norm(W0)
+    for _ in range(args.warmup):
+        op.sigma(W0)
+    log("warmup done")
+
+    op.profile(True)
+    times, energies, nsteps, conv = [], [], [], []
+    for k in range(args.steps):
+        op.set_vector(W0)
+        # The solver restarts from W every nb steps (davidson.h:918-922), so
+        # running it one restart cycle per call is the same computation; each
+        # cycle's result is printed as it completes.
+        t0 = time.perf_counter()
+        n_tot, ok, cyc = 0, False, 0
+        while cyc < args.max_it:
+            E, n, ok = op.solve(max_it=1, nb=args.nb, eps=args.eps,
+                                max_time=max(args.max_time - (time.perf_counter() - t0), 0.0))
+            n_tot += n
+            cyc += 1
+            el = time.perf_counter() - t0
+            tr = op.trace()
+            log(f"solve {k} cycle {cyc}: steps={n_tot} E0={E:.12f} |r|={tr[-1, 1]:.3e} t={el:.2f} s")
+            if ok or n < args.nb or el > args.max_time:
+                break
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        energies.append(E)
+        nsteps.append(n_tot)
+        conv.append(ok)
+        log(f"solve {k}: E0={E:.12f} steps={n_tot} converged={ok} time={dt:.3f} s")
+    stats = op.profile_stats()
+    op.profile(False)
+
+    kb = kernel_bytes(wc, na, nb)
+    kernels = {}
+    for name, (cnt, ms) in stats.items():
+        avg = ms / max(cnt, 1)
+        ent = dict(launches=cnt, total_ms=round(ms, 3), avg_ms=round(avg, 4))
+        if name in kb and avg > 0:
+            ent["alg_bytes"] = kb[name]
+            ent["GBps"] = round(kb[name] / (avg * 1e-3) / 1e9, 1)
+        kernels[name] = ent
+    sig_names = ["same_alpha", "transpose", "same_beta", "transpose_add", "mixed"]
+    sig_ms = sum(kernels[n]["avg_ms"] for n in sig_names if n in kernels)
+    sig_bytes = sum(kb[n] for n in sig_names)
+    dom = max((n for n in sig_names if n in kernels), key=lambda n: kernels[n]["total_ms"])
+    dk = kernels[dom]
+    roofline = dict(kernel=dom, bound="hbm", achieved=dk["GBps"], peak=HBM_PEAK_GBPS, unit="GB/s",
+                    frac=round(dk["GBps"] / HBM_PEAK_GBPS, 4), traffic=None)
+
+    T = float(np.median(times))
+    steps_med = int(np.median(nsteps))
+    out = {
+        "metric": METRIC,
+        "value": round(T, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * T, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: norb={spec['norb']}, n_alpha=n_beta={spec['nelec_half']}, "
+                        f"{spec['nstr']}x{spec['nstr']} strings, {op.ndet} determinants, "
+                        f"{spec['nchol']} Cholesky vectors, seed {spec['seed']}; plain FP64 Davidson "
+                        f"(nb={args.nb}, eps={args.eps:g}, start={args.init})",
+            "solver": "plain-fp64",
+            "davidson_steps": steps_med,
+            "converged": all(conv),
+            "E0": energies[-1],
+        },
+        "sigma_GBps": round(sig_bytes / (sig_ms * 1e-3) / 1e9, 1) if sig_ms > 0 else None,
+        "sigma_ms": round(sig_ms, 3),
+        "roofline": roofline,
+        "kernels": kernels,
+        "build_s": round(t_build, 3),
+    }
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline: ~{args.cpu_seconds:.0f} s oracle sample on {args.cpu_threads} threads")
+        cb = cpu_baseline(inst, steps_med, args.cpu_seconds, args.cpu_threads)
+        out["cpu_baseline"] = {k: (round(v, 3) if isinstance(v, float) else v)
+                               for k, v in cb.items() if k != "sigma_s"}
+        out["cpu_baseline"]["sigma_s"] = round(cb["sigma_s"], 3)
+    op.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

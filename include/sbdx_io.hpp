@@ -1,0 +1,134 @@
+// sbdx_io.hpp -- header-only readers for the reference's input formats, for
+// C++ callers that use the adapter without the reference's own framework.
+//
+//   load_fcidump     framework/fcidump.h:29-141 (header until &END, then
+//                    "value i j k l" rows, 1-based; 0 0 0 0 = core energy)
+//   setup_integrals  chemistry/basic/makeintegrals.h:123-174 (RHF): oneInt
+//                    store (2L)^2 row-major, twoInt 8-fold compressed store
+//   load_strings     chemistry/basic/makedeterminants.h:160-185 with
+//                    framework/bit_manipulation.h:215-229 (rightmost char is
+//                    orbital 0) and the sort/unique of :149-168, 381-395
+#ifndef SBDX_IO_HPP
+#define SBDX_IO_HPP
+
+#include <algorithm>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace sbdx {
+
+struct FCIDump {
+  std::map<std::string, std::string> header;
+  std::vector<std::tuple<double, int, int, int, int>> integrals;
+};
+
+inline std::string trim(const std::string &s) {
+  const auto b = s.find_first_not_of(" \t\r");
+  if (b == std::string::npos) return "";
+  const auto e = s.find_last_not_of(" \t\r");
+  return s.substr(b, e - b + 1);
+}
+
+inline FCIDump load_fcidump(const std::string &path) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error("sbdx: cannot open FCIDUMP " + path);
+  FCIDump fd;
+  std::string line, head;
+  bool in_header = true;
+  while (std::getline(in, line)) {
+    if (line.empty() || line[0] == '!') continue;
+    if (in_header) {
+      if (line.find("&END") != std::string::npos) {
+        in_header = false;
+        continue;
+      }
+      const auto p = line.find("&FCI");
+      if (p != std::string::npos) line.erase(p, 4);
+      head += line;
+      continue;
+    }
+    std::stringstream ss(line);
+    double v;
+    int i, j, k, l;
+    if (ss >> v >> i >> j >> k >> l) fd.integrals.emplace_back(v, i, j, k, l);
+  }
+  std::stringstream hs(head);
+  std::string tok;
+  while (std::getline(hs, tok, ',')) {
+    const auto eq = tok.find('=');
+    if (eq != std::string::npos) fd.header[trim(tok.substr(0, eq))] = trim(tok.substr(eq + 1));
+  }
+  if (!fd.header.count("NORB")) throw std::runtime_error("sbdx: FCIDUMP without NORB");
+  return fd;
+}
+
+struct Integrals {
+  struct Store {
+    std::vector<double> store;
+  };
+  int L = 0, N = 0;
+  double I0 = 0.0;
+  Store I1, I2;  // same `store` layouts as the reference's oneInt / twoInt
+};
+
+inline Integrals setup_integrals(const FCIDump &fd) {
+  Integrals g;
+  g.L = std::atoi(fd.header.at("NORB").c_str());
+  if (fd.header.count("NELEC")) g.N = std::atoi(fd.header.at("NELEC").c_str());
+  const int L = g.L, L2 = 2 * L;
+  const size_t B = (size_t)L * (L + 1) / 2;
+  g.I1.store.assign((size_t)L2 * L2, 0.0);
+  g.I2.store.assign(B * (B + 1) / 2, 0.0);
+  auto tri = [](size_t a, size_t b) { return a > b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; };
+  for (const auto &t : fd.integrals) {
+    const double v = std::get<0>(t);
+    const int i = std::get<1>(t), j = std::get<2>(t), k = std::get<3>(t), l = std::get<4>(t);
+    if (i == 0 && j == 0 && k == 0 && l == 0) {
+      g.I0 = v;
+    } else if (k == 0 && l == 0) {
+      const int a = 2 * (i - 1), b = 2 * (j - 1);
+      g.I1.store[(size_t)a * L2 + b] = v;
+      g.I1.store[(size_t)(a + 1) * L2 + b + 1] = v;
+      g.I1.store[(size_t)b * L2 + a] = v;
+      g.I1.store[(size_t)(b + 1) * L2 + a + 1] = v;
+    } else {
+      g.I2.store[tri(tri(i - 1, j - 1), tri(k - 1, l - 1))] = v;
+    }
+  }
+  return g;
+}
+
+inline std::vector<std::vector<size_t>> load_strings(const std::string &path, size_t bit_length,
+                                                     size_t L) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error("sbdx: cannot open string file " + path);
+  const size_t nw = (L + bit_length - 1) / bit_length;
+  std::vector<std::vector<size_t>> out;
+  std::string s;
+  while (std::getline(in, s)) {
+    s = trim(s);
+    if (s.empty()) continue;
+    if (s.size() < L) throw std::runtime_error("sbdx: string shorter than norb: " + s);
+    std::vector<size_t> w(nw, 0);
+    for (size_t i = 0; i < L; i++)
+      if (s[s.size() - 1 - i] == '1') w[i / bit_length] |= (size_t)1 << (i % bit_length);
+    out.push_back(w);
+  }
+  std::sort(out.begin(), out.end(), [](const std::vector<size_t> &a, const std::vector<size_t> &b) {
+    for (size_t n = a.size(); n > 0; n--)
+      if (a[n - 1] != b[n - 1]) return a[n - 1] < b[n - 1];
+    return false;
+  });
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+  return out;
+}
+
+}  // namespace sbdx
+
+#endif  // SBDX_IO_HPP

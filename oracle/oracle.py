@@ -43,6 +43,9 @@ def lib():
         L.orc_diag.argtypes = [C.c_void_p, _dp]
         L.orc_sigma.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_sigma_rows.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_size_t, C.c_size_t]
+        L.orc_sigma_rowlist.argtypes = [C.c_void_p, _dp, _dp, _dp,
+                                        np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                        C.c_size_t]
         L.orc_davidson.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_int, C.c_double,
                                    C.POINTER(C.c_double), C.POINTER(C.c_int), _dp, C.c_int]
         L.orc_davidson.restype = C.c_int
@@ -121,6 +124,11 @@ class Oracle:
 
     def sigma_rows(self, C_vec, hii, lo, hi, out):
         lib().orc_sigma_rows(self.h, hii, C_vec, out, lo, hi)
+        return out
+
+    def sigma_rowlist(self, C_vec, hii, rows, out):
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        lib().orc_sigma_rowlist(self.h, hii, C_vec, out, rows, rows.size)
         return out
 
     def davidson(self, W=None, max_it=100, nb=12, eps=1e-8, hii=None, trace_cap=4096):

@@ -332,71 +332,77 @@ void orc_diag(void *h, double *hii) {
 /* sigma, chemistry/tpb/mult.h:625-881 on one rank: Wb += hii*T, then   */
 /* the three task types in the order alpha (S then D), beta (S then D), */
 /* alpha-single x beta-single.  sigma is ACCUMULATED (caller zeroes).   */
-/* row_lo/row_hi restrict the bra alpha rows (for bounded samples).     */
+/* Every element of a bra row receives its contributions in exactly that */
+/* order, so one row at a time gives the same sums as the task loops.   */
 /* ------------------------------------------------------------------ */
-void orc_sigma_rows(void *h, const double *hii, const double *T, double *W,
-                    size_t row_lo, size_t row_hi) {
-  orc_t *o = h;
-  size_t nb = o->nb;
+static void sigma_row(const orc_t *o, const double *hii, const double *T, double *W, size_t ia) {
+  const size_t nb = o->nb;
   const ints_t *g = &o->g;
   const table_t *ta = &o->ta, *tb = &o->tb;
-  if (row_hi > o->na) row_hi = o->na;
-#pragma omp parallel for schedule(dynamic, 1)
-  for (size_t ia = row_lo; ia < row_hi; ia++) {
-    for (size_t ib = 0; ib < nb; ib++) W[ia * nb + ib] += hii[ia * nb + ib] * T[ia * nb + ib];
-    /* type 2: alpha singles and doubles */
-    for (size_t ib = 0; ib < nb; ib++) {
-      u128 D = interleave(o->astr[ia], o->bstr[ib], o->L);
-      size_t bra = ia * nb + ib;
-      for (size_t p = ta->s_off[ia]; p < ta->s_off[ia + 1]; p++) {
-        double e = one_excite(g, D, ta->s_cran[2 * p], ta->s_cran[2 * p + 1]);
-        W[bra] += e * T[(size_t)ta->s_ket[p] * nb + ib];
-      }
-      for (size_t p = ta->d_off[ia]; p < ta->d_off[ia + 1]; p++) {
-        const int *c = ta->d_cran + 4 * p;
-        double e = two_excite(g, D, c[0], c[1], c[2], c[3]);
-        W[bra] += e * T[(size_t)ta->d_ket[p] * nb + ib];
-      }
+  for (size_t ib = 0; ib < nb; ib++) W[ia * nb + ib] += hii[ia * nb + ib] * T[ia * nb + ib];
+  /* type 2: alpha singles and doubles */
+  for (size_t ib = 0; ib < nb; ib++) {
+    u128 D = interleave(o->astr[ia], o->bstr[ib], o->L);
+    size_t bra = ia * nb + ib;
+    for (size_t p = ta->s_off[ia]; p < ta->s_off[ia + 1]; p++) {
+      double e = one_excite(g, D, ta->s_cran[2 * p], ta->s_cran[2 * p + 1]);
+      W[bra] += e * T[(size_t)ta->s_ket[p] * nb + ib];
+    }
+    for (size_t p = ta->d_off[ia]; p < ta->d_off[ia + 1]; p++) {
+      const int *c = ta->d_cran + 4 * p;
+      double e = two_excite(g, D, c[0], c[1], c[2], c[3]);
+      W[bra] += e * T[(size_t)ta->d_ket[p] * nb + ib];
     }
   }
-#pragma omp parallel for schedule(dynamic, 1)
-  for (size_t ia = row_lo; ia < row_hi; ia++) {
-    /* type 1: beta singles and doubles */
-    for (size_t ib = 0; ib < nb; ib++) {
-      u128 D = interleave(o->astr[ia], o->bstr[ib], o->L);
-      size_t bra = ia * nb + ib;
-      for (size_t p = tb->s_off[ib]; p < tb->s_off[ib + 1]; p++) {
-        double e = one_excite(g, D, tb->s_cran[2 * p], tb->s_cran[2 * p + 1]);
-        W[bra] += e * T[ia * nb + tb->s_ket[p]];
-      }
-      for (size_t p = tb->d_off[ib]; p < tb->d_off[ib + 1]; p++) {
-        const int *c = tb->d_cran + 4 * p;
-        double e = two_excite(g, D, c[0], c[1], c[2], c[3]);
-        W[bra] += e * T[ia * nb + tb->d_ket[p]];
-      }
+  /* type 1: beta singles and doubles */
+  for (size_t ib = 0; ib < nb; ib++) {
+    u128 D = interleave(o->astr[ia], o->bstr[ib], o->L);
+    size_t bra = ia * nb + ib;
+    for (size_t p = tb->s_off[ib]; p < tb->s_off[ib + 1]; p++) {
+      double e = one_excite(g, D, tb->s_cran[2 * p], tb->s_cran[2 * p + 1]);
+      W[bra] += e * T[ia * nb + tb->s_ket[p]];
+    }
+    for (size_t p = tb->d_off[ib]; p < tb->d_off[ib + 1]; p++) {
+      const int *c = tb->d_cran + 4 * p;
+      double e = two_excite(g, D, c[0], c[1], c[2], c[3]);
+      W[bra] += e * T[ia * nb + tb->d_ket[p]];
     }
   }
-#pragma omp parallel for schedule(dynamic, 1)
-  for (size_t ia = row_lo; ia < row_hi; ia++) {
-    /* type 0: alpha single x beta single */
-    for (size_t ib = 0; ib < nb; ib++) {
-      u128 D = interleave(o->astr[ia], o->bstr[ib], o->L);
-      size_t bra = ia * nb + ib;
-      for (size_t p = ta->s_off[ia]; p < ta->s_off[ia + 1]; p++) {
-        size_t ja = ta->s_ket[p];
-        int cra = ta->s_cran[2 * p], ana = ta->s_cran[2 * p + 1];
-        for (size_t q = tb->s_off[ib]; q < tb->s_off[ib + 1]; q++) {
-          double e = two_excite(g, D, cra, tb->s_cran[2 * q], ana, tb->s_cran[2 * q + 1]);
-          W[bra] += e * T[ja * nb + tb->s_ket[q]];
-        }
+  /* type 0: alpha single x beta single */
+  for (size_t ib = 0; ib < nb; ib++) {
+    u128 D = interleave(o->astr[ia], o->bstr[ib], o->L);
+    size_t bra = ia * nb + ib;
+    for (size_t p = ta->s_off[ia]; p < ta->s_off[ia + 1]; p++) {
+      size_t ja = ta->s_ket[p];
+      int cra = ta->s_cran[2 * p], ana = ta->s_cran[2 * p + 1];
+      for (size_t q = tb->s_off[ib]; q < tb->s_off[ib + 1]; q++) {
+        double e = two_excite(g, D, cra, tb->s_cran[2 * q], ana, tb->s_cran[2 * q + 1]);
+        W[bra] += e * T[ja * nb + tb->s_ket[q]];
       }
     }
   }
 }
 
+/* rows [row_lo, row_hi) only (bounded samples) */
+void orc_sigma_rows(void *h, const double *hii, const double *T, double *W,
+                    size_t row_lo, size_t row_hi) {
+  orc_t *o = h;
+  if (row_hi > o->na) row_hi = o->na;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t ia = row_lo; ia < row_hi; ia++) sigma_row(o, hii, T, W, ia);
+}
+
 void orc_sigma(void *h, const double *hii, const double *T, double *W) {
   orc_t *o = h;
   orc_sigma_rows(h, hii, T, W, 0, o->na);
+}
+
+/* sigma restricted to a list of distinct bra alpha rows (CPU timing samples) */
+void orc_sigma_rowlist(void *h, const double *hii, const double *T, double *W,
+                       const int64_t *rows, size_t nrows) {
+  orc_t *o = h;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t k = 0; k < nrows; k++) sigma_row(o, hii, T, W, (size_t)rows[k]);
 }
 
 /* ------------------------------------------------------------------ */

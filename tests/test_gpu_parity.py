@@ -86,22 +86,24 @@ def test_sigma_accumulate_and_linearity(pair):
 def test_davidson_energy(pair):
     name, inst, op, orc = pair
     import sbdx
-    # a seeded dense start vector overlaps every eigenstate, so both solvers
-    # target the true ground state even when it has no Hartree-Fock weight
-    W0 = random_vector(op.ndet, 99)
+    molecule_case = name.startswith("h2o")
+    # Molecules start from the Hartree-Fock determinant (the reference's
+    # init 0) and converge in a few steps, so the trajectories must agree.
+    # The synthetic cases start from a seeded dense vector that overlaps every
+    # eigenstate (their ground state may carry no HF weight); their long
+    # restarted runs amplify rounding, so only the converged state is compared.
+    W0 = sbdx.basis_init_vector(op.ndet) if molecule_case else random_vector(op.ndet, 99)
     r = orc.davidson(W=W0, nb=10, eps=1e-8)
     E, W, steps, conv = sbdx.davidson(op, W=W0, nb=10, eps=1e-8)
-    assert conv
+    assert conv and r["converged"]
     assert abs(E - r["E"]) < 1e-10, (E, r["E"])
     # the final <W|HW> the reference reports
     assert abs(op.energy(W) - r["E"]) < 1e-10
-    # same trajectory: identical early steps; later steps only drift by the
-    # amplified rounding of restarted subspaces
-    tr = op.trace()
-    assert abs(tr.shape[0] - r["steps"]) <= 2, (tr.shape, r["steps"])
-    n = min(tr.shape[0], r["steps"])
-    assert np.max(np.abs(tr[:4, 0] - r["trace"][:4, 0])) < 1e-8
-    assert np.max(np.abs(tr[:n, 0] - r["trace"][:n, 0])) < 1e-6
+    if molecule_case:
+        tr = op.trace()
+        assert abs(tr.shape[0] - r["steps"]) <= 1, (tr.shape, r["steps"])
+        n = min(tr.shape[0], r["steps"])
+        assert np.max(np.abs(tr[:n, 0] - r["trace"][:n, 0])) < 1e-9
     # same eigenvector up to a global sign
     s = np.sign(W @ r["W"])
     assert np.max(np.abs(W - s * r["W"])) < 1e-6
