@@ -32,21 +32,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_bytes(wc, na, nb):
-    """Algorithmic HBM bytes per launch of each sigma kernel group (FP64).
-    same-spin: every (entry x other-spin column) reads x[J,m] (+ the dressing
-    dress[pq,m] for singles), plus diagonal, hii and output streams;
-    mixed (direct form): one x gather per (alpha single x beta single) term
-    plus reading and writing sigma; transposes: one read + one write."""
-    e = 8.0
+def kernel_bytes(wc, na, nb, e=8.0):
+    """Algorithmic bytes per launch of each sigma kernel group, element size
+    e (8 FP64, 4 FP32).  same-spin: every (entry x other-spin column) reads
+    x[J,m] (+ the dressing dress[pq,m] for singles), plus diagonal, hii and
+    output streams, plus the table entries once; mixed (direct form): one x
+    gather per (alpha single x beta single) term plus reading and writing
+    sigma, plus the tables once; transposes: one read + one write."""
     N = na * nb
     ab = wc["alpha_singles"] * wc["beta_singles"]
     return {
         "same_alpha": e * (nb * (2 * wc["alpha_singles"] + wc["alpha_doubles"]) + 3 * N)
-        + 16.0 * (wc["alpha_singles"] + wc["alpha_doubles"]),
+        + (8.0 + e) * wc["alpha_singles"] + (4.0 + e) * wc["alpha_doubles"],
         "same_beta": e * (na * (2 * wc["beta_singles"] + wc["beta_doubles"]) + 2 * N)
-        + 16.0 * (wc["beta_singles"] + wc["beta_doubles"]),
-        "mixed": e * (ab + 2 * N) + 8.0 * (wc["alpha_singles"] + wc["beta_singles"]),
+        + (8.0 + e) * wc["beta_singles"] + (4.0 + e) * wc["beta_doubles"],
+        "mixed": e * (ab + 2 * N) + 8.0 * wc["alpha_singles"] + 4.0 * wc["beta_singles"],
         "transpose": 2 * e * N,
         "transpose_add": 3 * e * N,
     }
@@ -63,16 +63,14 @@ def cpu_baseline(inst, n_steps, seconds, nthreads):
     hii = np.zeros(n)
     y = np.zeros(n)
     rows = rng.permutation(orc.na)
-    # calibrate on a few rows, then time a sample sized to ~`seconds`
+    # random rows in batches of one row per thread until the time budget is used
+    done = 0
     t0 = time.perf_counter()
-    orc.sigma_rowlist(x, hii, rows[:nthreads], y)
-    t_cal = (time.perf_counter() - t0) / nthreads
-    k = int(max(nthreads, min(orc.na, seconds * nthreads / max(t_cal, 1e-6))))
-    k = max(nthreads, (k // nthreads) * nthreads)
-    sample = rows[nthreads:nthreads + k]
-    t0 = time.perf_counter()
-    orc.sigma_rowlist(x, hii, sample, y)
+    while done < orc.na and (time.perf_counter() - t0) < seconds:
+        orc.sigma_rowlist(x, hii, rows[done:done + nthreads], y)
+        done += nthreads
     t = time.perf_counter() - t0
+    sample = rows[:min(done, orc.na)]
     orc.close()
     t_sigma = t * orc.na / sample.size
     return dict(value=t_sigma * n_steps, unit="s", cores=nthreads, kind="port",
@@ -93,6 +91,11 @@ def main():
     ap.add_argument("--max-it", type=int, default=200)
     ap.add_argument("--max-time", type=float, default=900.0)
     ap.add_argument("--init", choices=["hf", "random"], default="hf")
+    ap.add_argument("--solver", choices=["plain", "thick", "mixed"], default="mixed",
+                    help="timed solver: plain = the reference algorithm in FP64; thick = FP64 "
+                         "thick restart; mixed = thick restart with FP32 correction products")
+    ap.add_argument("--no-compare", dest="compare", action="store_false",
+                    help="skip the plain FP64 reference solve")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -124,53 +127,74 @@ def main():
         op.sigma(W0)
     log("warmup done")
 
-    op.profile(True)
-    times, energies, nsteps, conv = [], [], [], []
-    for k in range(args.steps):
-        op.set_vector(W0)
-        # The solver restarts from W every nb steps (davidson.h:918-922), so
-        # running it one restart cycle per call is the same computation; each
-        # cycle's result is printed as it completes.
-        t0 = time.perf_counter()
-        n_tot, ok, cyc = 0, False, 0
-        while cyc < args.max_it:
-            E, n, ok = op.solve(max_it=1, nb=args.nb, eps=args.eps,
-                                max_time=max(args.max_time - (time.perf_counter() - t0), 0.0))
-            n_tot += n
-            cyc += 1
-            el = time.perf_counter() - t0
+    def run(solver, nsolve):
+        op.configure(solver)
+        op.profile(True)
+        times, energies, nsteps, conv = [], [], [], []
+        for k in range(nsolve):
+            op.set_vector(W0)
+            t0 = time.perf_counter()
+            E, n, ok = op.solve(max_it=args.max_it, nb=args.nb, eps=args.eps,
+                                max_time=args.max_time)
+            dt = time.perf_counter() - t0
+            times.append(dt)
+            energies.append(E)
+            nsteps.append(n)
+            conv.append(ok)
             tr = op.trace()
-            log(f"solve {k} cycle {cyc}: steps={n_tot} E0={E:.12f} |r|={tr[-1, 1]:.3e} t={el:.2f} s")
-            if ok or n < args.nb or el > args.max_time:
-                break
-        dt = time.perf_counter() - t0
-        times.append(dt)
-        energies.append(E)
-        nsteps.append(n_tot)
-        conv.append(ok)
-        log(f"solve {k}: E0={E:.12f} steps={n_tot} converged={ok} time={dt:.3f} s")
-    stats = op.profile_stats()
-    op.profile(False)
+            log(f"{solver} solve {k}: E0={E:.12f} |r|={tr[-1, 1]:.2e} sigma-products={n} "
+                f"converged={ok} time={dt:.3f} s")
+        stats = op.profile_stats()
+        op.profile(False)
+        return dict(time=float(np.median(times)), E=energies[-1], steps=int(np.median(nsteps)),
+                    converged=all(conv), stats=stats)
 
-    kb = kernel_bytes(wc, na, nb)
-    kernels = {}
-    for name, (cnt, ms) in stats.items():
-        avg = ms / max(cnt, 1)
-        ent = dict(launches=cnt, total_ms=round(ms, 3), avg_ms=round(avg, 4))
-        if name in kb and avg > 0:
-            ent["alg_bytes"] = kb[name]
-            ent["GBps"] = round(kb[name] / (avg * 1e-3) / 1e9, 1)
-        kernels[name] = ent
-    sig_names = ["same_alpha", "transpose", "same_beta", "transpose_add", "mixed"]
-    sig_ms = sum(kernels[n]["avg_ms"] for n in sig_names if n in kernels)
-    sig_bytes = sum(kb[n] for n in sig_names)
-    dom = max((n for n in sig_names if n in kernels), key=lambda n: kernels[n]["total_ms"])
+    best = args.solver
+    plain = run("plain", 1 if best != "plain" else args.steps) if args.compare or best == "plain" else None
+    res = run(best, args.steps) if best != "plain" else plain
+    stats = res["stats"]
+    times = [res["time"]]
+    energies = [res["E"]]
+    nsteps = [res["steps"]]
+    conv = [res["converged"]]
+
+    def kernel_table(stats):
+        kb64 = kernel_bytes(wc, na, nb, 8.0)
+        kb32 = kernel_bytes(wc, na, nb, 4.0)
+        out = {}
+        for name, (cnt, ms) in stats.items():
+            avg = ms / max(cnt, 1)
+            ent = dict(launches=cnt, total_ms=round(ms, 3), avg_ms=round(avg, 4))
+            base = name[:-4] if name.endswith("_f32") else name
+            kb = kb32 if name.endswith("_f32") else kb64
+            if base in kb and avg > 0:
+                ent["alg_bytes"] = kb[base]
+                ent["GBps"] = round(kb[base] / (avg * 1e-3) / 1e9, 1)
+            out[name] = ent
+        return out
+
+    def sigma_rate(kernels, suffix):
+        names = [n + suffix for n in ("same_alpha", "transpose", "same_beta", "transpose_add", "mixed")]
+        if not all(n in kernels for n in names):
+            return None, None
+        ms = sum(kernels[n]["avg_ms"] for n in names)
+        by = sum(kernels[n]["alg_bytes"] for n in names)
+        return round(ms, 3), round(by / (ms * 1e-3) / 1e9, 1)
+
+    kernels = kernel_table(stats)
+    sig_kernels = [n for n in kernels if "alg_bytes" in kernels[n]]
+    dom = max(sig_kernels, key=lambda n: kernels[n]["total_ms"])
     dk = kernels[dom]
     roofline = dict(kernel=dom, bound="hbm", achieved=dk["GBps"], peak=HBM_PEAK_GBPS, unit="GB/s",
                     frac=round(dk["GBps"] / HBM_PEAK_GBPS, 4), traffic=None)
+    s64_ms, s64_gbps = sigma_rate(kernels, "")
+    s32_ms, s32_gbps = sigma_rate(kernels, "_f32")
+    if plain is not None:
+        pk = kernel_table(plain["stats"])
+        s64_ms, s64_gbps = sigma_rate(pk, "")
 
-    T = float(np.median(times))
-    steps_med = int(np.median(nsteps))
+    T = res["time"]
+    dtypes = {"plain": "f64", "thick": "f64", "mixed": "f32+f64"}
     out = {
         "metric": METRIC,
         "value": round(T, 4),
@@ -182,27 +206,35 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtypes[best],
         "data": "synthetic",
         "config": {
             "workload": f"{args.workload}: norb={spec['norb']}, n_alpha=n_beta={spec['nelec_half']}, "
                         f"{spec['nstr']}x{spec['nstr']} strings, {op.ndet} determinants, "
-                        f"{spec['nchol']} Cholesky vectors, seed {spec['seed']}; plain FP64 Davidson "
-                        f"(nb={args.nb}, eps={args.eps:g}, start={args.init})",
-            "solver": "plain-fp64",
-            "davidson_steps": steps_med,
-            "converged": all(conv),
-            "E0": energies[-1],
+                        f"{spec['nchol']} Cholesky vectors, seed {spec['seed']}; Davidson to "
+                        f"|E0 W - H W| < {args.eps:g} (nb={args.nb}, start={args.init})",
+            "solver": best,
+            "sigma_products": res["steps"],
+            "converged": res["converged"],
+            "E0": res["E"],
         },
-        "sigma_GBps": round(sig_bytes / (sig_ms * 1e-3) / 1e9, 1) if sig_ms > 0 else None,
-        "sigma_ms": round(sig_ms, 3),
+        "plain_fp64": None if plain is None else {
+            "time_s": round(plain["time"], 4), "sigma_products": plain["steps"], "E0": plain["E"],
+            "converged": plain["converged"]},
+        "speedup_vs_plain": None if plain is None else round(plain["time"] / T, 3),
+        "dE_vs_plain": None if plain is None else abs(res["E"] - plain["E"]),
+        "sigma_fp64_ms": s64_ms,
+        "sigma_GBps": s64_gbps,
+        "sigma_fp32_ms": s32_ms,
+        "sigma_fp32_GBps": s32_gbps,
         "roofline": roofline,
         "kernels": kernels,
         "build_s": round(t_build, 3),
     }
     if not args.no_cpu_baseline:
+        ref_steps = plain["steps"] if plain is not None else res["steps"]
         log(f"cpu baseline: ~{args.cpu_seconds:.0f} s oracle sample on {args.cpu_threads} threads")
-        cb = cpu_baseline(inst, steps_med, args.cpu_seconds, args.cpu_threads)
+        cb = cpu_baseline(inst, ref_steps, args.cpu_seconds, args.cpu_threads)
         out["cpu_baseline"] = {k: (round(v, 3) if isinstance(v, float) else v)
                                for k, v in cb.items() if k != "sigma_s"}
         out["cpu_baseline"]["sigma_s"] = round(cb["sigma_s"], 3)

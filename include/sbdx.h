@@ -98,6 +98,29 @@ int sbdx_solve(sbdx_handle_t h, int max_it, int nb, double eps, double max_time,
                double *energy, int *n_steps);
 int sbdx_get_vector(sbdx_handle_t h, double *W);
 
+/* Arithmetic of the Davidson solves of this handle (no counterpart in the
+ * reference, which is FP64 throughout):
+ *   SBDX_PREC_FP64   every sigma in FP64 -- the reference's algorithm;
+ *   SBDX_PREC_MIXED  the restart vector's sigma in FP64 (exact FP64
+ *                    residual every cycle, convergence judged only on it),
+ *                    the correction vectors' sigma in FP32; Gram-Schmidt,
+ *                    Rayleigh-Ritz, residual and energy stay FP64. */
+#define SBDX_PREC_FP64 0
+#define SBDX_PREC_MIXED 1
+int sbdx_set_precision(sbdx_handle_t h, int mode);
+
+/* Restart policy of later solves (no counterpart in the reference):
+ *   keep == 0   restart from W alone every nb steps (davidson.h:918-922);
+ *   keep 1..8   thick restart: at each restart W gets an exact FP64 product
+ *               and the exact residual decides convergence; the basis keeps
+ *               W and the next keep-1 Ritz vectors (products by linear
+ *               combination).  max_it then bounds max_it*nb products. */
+int sbdx_set_restart(sbdx_handle_t h, int keep);
+
+/* Wb = H*Wk evaluated in FP32 (inputs rounded to FP32, result widened):
+ * the product the mixed solver applies to correction vectors. */
+int sbdx_sigma_fp32(sbdx_handle_t h, const double *Wk, double *Wb);
+
 /* History of the last solve: trace[2*k] = E0, trace[2*k+1] = |r| of step k,
  * the values the reference prints per Davidson step (davidson.h:839-851). */
 int sbdx_trace(sbdx_handle_t h, double *trace, int cap, int *n);

@@ -163,12 +163,14 @@ static Context *create(int L, size_t bl, const uint64_t *aw, size_t na, const ui
     const auto bs = unpack_words(bw, nb, bl, L);
     build_spin_tables(c, c.A, as);
     build_dress(c, c.A);
+    build_singles_ell(c, c.A);
     if (as == bs) {
       c.B = c.A;
       c.B.alias = true;
     } else {
       build_spin_tables(c, c.B, bs);
       build_dress(c, c.B);
+      build_singles_ell(c, c.B);
     }
     c.hii = (double *)dalloc(c, sizeof(double) * c.N);
     compute_diag(c);
@@ -335,7 +337,10 @@ int sbdx_solve(sbdx_handle_t h, int max_it, int nb, double eps, double max_time,
   Context &c = ctx_of(h);
   if (!c.have_vector) throw Error(SBDX_ERR_ARG, "no start vector (sbdx_set_vector)");
   if (max_it < 1) throw Error(SBDX_ERR_ARG, "max_it must be >= 1");
-  const int rc = davidson_solve(c, max_it, nb, eps, max_time, energy, n_steps);
+  const int rc = c.restart_keep == 0
+                     ? davidson_solve(c, max_it, nb, eps, max_time, energy, n_steps)
+                     : davidson_solve_thick(c, max_it * nb, nb, c.restart_keep, eps, max_time,
+                                            c.precision == SBDX_PREC_MIXED, energy, n_steps);
   if (rc != 0) g_err = "Davidson did not reach eps";
   return rc;
   SBDX_GUARD_END
@@ -349,6 +354,38 @@ int sbdx_davidson(sbdx_handle_t h, double *W, int max_it, int nb, double eps, do
   if (rc != SBDX_OK && rc != SBDX_ERR_NOT_CONVERGED) return rc;
   const int rc2 = sbdx_get_vector(h, W);
   return rc2 != SBDX_OK ? rc2 : rc;
+}
+
+int sbdx_set_precision(sbdx_handle_t h, int mode) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (mode != SBDX_PREC_FP64 && mode != SBDX_PREC_MIXED) throw Error(SBDX_ERR_ARG, "unknown precision mode");
+  if (mode == SBDX_PREC_MIXED) ensure_f32(c);
+  c.precision = mode;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_set_restart(sbdx_handle_t h, int keep) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (keep < 0 || keep > 8) throw Error(SBDX_ERR_ARG, "keep must be in [0, 8]");
+  c.restart_keep = keep;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_sigma_fp32(sbdx_handle_t h, const double *Wk, double *Wb) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!Wk || !Wb) throw Error(SBDX_ERR_ARG, "null vector");
+  upload(c, c.vout, Wk);
+  sign_convert(c, c.vout, c.vin);
+  sigma_alpha_first_f32(c, c.vin, c.vout);
+  sign_convert(c, c.vout, c.vin);
+  download(c, Wb, c.vin);
+  return SBDX_OK;
+  SBDX_GUARD_END
 }
 
 int sbdx_trace(sbdx_handle_t h, double *trace, int cap, int *n) {

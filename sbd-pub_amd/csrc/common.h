@@ -55,12 +55,30 @@ struct SpinTables {
   size_t ns = 0, nd = 0;
   size_t max_s = 0;            // longest singles row
   bool alias = false;          // shares arrays with the other spin
+  // Singles again in sliced-ELL form for the mixed kernels: rows sorted by
+  // singles count (descending) into slices of 64 (one wavefront); within a
+  // slice entry k of lane l sits at sl_off[slice] + 64*k + l, packed as
+  // ket | pq<<16 | (sign<0)<<31; padding entries carry pq = K (a zero slot).
+  uint32_t *ell = nullptr;
+  uint32_t *perm = nullptr;    // [nslice*64] row of each slot, 0xffffffff for padding
+  uint64_t *sl_off = nullptr;  // [nslice]
+  uint32_t *sl_len = nullptr;  // [nslice]
+  int nslice = 0;
+  bool has_ell = false;
 };
 
 struct ProfSlot {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
   long launches = 0;
   double total_ms = 0.0;
+};
+
+// FP32 copies of the sigma operands and work vectors (mixed precision).
+struct F32Operands {
+  bool ready = false;
+  float *s_val_a = nullptr, *d_val_a = nullptr, *s_val_b = nullptr, *d_val_b = nullptr;
+  float *dress_a = nullptr, *dress_b = nullptr, *eri = nullptr, *hii = nullptr;
+  float *x = nullptr, *y = nullptr, *T1 = nullptr, *T2 = nullptr;
 };
 
 struct Context {
@@ -73,6 +91,10 @@ struct Context {
   double *h1 = nullptr;    // [K] spatial one-body
   SpinTables A, B;
   double *hii = nullptr;   // [N]
+  F32Operands f32;
+  int precision = 0;       // 0: FP64 throughout; 1: FP32 sigma on corrections, FP64 restarts
+  int restart_keep = 0;    // 0: restart from W alone (reference); k: thick restart keeping k Ritz vectors
+  std::vector<double *> spareY, spareHY;  // thick-restart staging vectors
   // work vectors
   double *vin = nullptr, *vout = nullptr;   // API staging (alpha-first)
   double *T1 = nullptr, *T2 = nullptr;      // transpose scratch [N]
@@ -128,6 +150,7 @@ void check_launch(const char *what);
 
 // tables.hip
 void build_spin_tables(Context &c, SpinTables &t, const std::vector<uint64_t> &strs);
+void build_singles_ell(Context &c, SpinTables &t);
 void build_dress(Context &c, SpinTables &t);
 void compute_diag(Context &c);
 void sign_convert(Context &c, const double *in, double *out);      // out = S*in
@@ -135,11 +158,15 @@ void sign_convert_add(Context &c, const double *in, double *out);  // out += S*i
 
 // sigma.hip
 void sigma_alpha_first(Context &c, const double *x, double *y);  // y = H' x (overwrite)
+void sigma_alpha_first_f32(Context &c, const double *x, double *y);  // same, FP32 arithmetic
+void ensure_f32(Context &c);
 void transpose_into(Context &c, const double *in, double *out, int R, int Cc);  // out = in^T
 
 // davidson.hip
 int davidson_solve(Context &c, int max_it, int nb, double eps, double max_time, double *energy,
                    int *n_steps);
+int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps, double max_time,
+                         bool mixed, double *energy, int *n_steps);
 double device_dot(Context &c, const double *x, const double *y, size_t n);
 
 // rdm.hip

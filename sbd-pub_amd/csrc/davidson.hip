@@ -19,6 +19,10 @@ namespace sbdx {
 constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
 
+struct OutList {
+  double *p[8];
+};
+
 struct VecList {
   const double *p[kNbMax];
 };
@@ -252,6 +256,53 @@ double device_dot(Context &c, const double *x, const double *y, size_t n) {
   return r;
 }
 
+// out_j = sum_k U[k + ld*(j0+j)] in_k for j < nout (Ritz vectors and their
+// H-products from the subspace basis)
+__global__ void __launch_bounds__(kRedThreads) k_combine(VecList in, int nin, const double *__restrict__ U,
+                                                         int ld, int j0, int nout, OutList out, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc[j] = 0.0;
+    for (int k = 0; k < nin; k++) {
+      const double v = in.p[k][i];
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (j < nout) acc[j] += U[k + (size_t)ld * (j0 + j)] * v;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (j < nout) out.p[j][i] = acc[j];
+  }
+}
+
+// R = HW - E*W with E = <W|HW> (from partials); partial |R|^2 per block.
+__global__ void __launch_bounds__(kRedThreads) k_exact_resid(const double *__restrict__ W,
+                                                             const double *__restrict__ HW, size_t n,
+                                                             const double *__restrict__ epart, int nblk,
+                                                             double *__restrict__ R,
+                                                             double *__restrict__ part,
+                                                             double *__restrict__ Eout) {
+  __shared__ double sh[kRedThreads];
+  __shared__ double Es;
+  if (threadIdx.x == 0) {
+    Es = sum_part(epart, nblk, 1, 0);
+    if (blockIdx.x == 0) *Eout = Es;
+  }
+  __syncthreads();
+  const double E = Es;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double r = HW[i] - E * W[i];
+    R[i] = r;
+    s += r * r;
+  }
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 static void ensure_subspace(Context &c, int nb) {
   while ((int)c.C.size() < nb) {
     c.C.push_back((double *)dalloc(c, sizeof(double) * c.N));
@@ -280,7 +331,15 @@ int davidson_solve(Context &c, int max_it, int nb, double eps, double max_time, 
     SBDX_HIP(hipMemcpyAsync(c.C[0], c.W, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
     for (int ib = 0; ib < nb; ib++) {
       auto s0 = std::chrono::steady_clock::now();
-      sigma_alpha_first(c, c.C[ib], c.HC[ib]);
+      // Mixed precision: the restart vector (ib == 0) gets the exact FP64
+      // product, so every cycle starts from the exact FP64 residual of W;
+      // the correction vectors use the FP32 product, whose error enters W
+      // only through coefficients that shrink with the residual.
+      const bool approx = c.precision == 1 && ib > 0;
+      if (approx)
+        sigma_alpha_first_f32(c, c.C[ib], c.HC[ib]);
+      else
+        sigma_alpha_first(c, c.C[ib], c.HC[ib]);
       {
         ProfScope ps(c, "dav_subspace");
         dots(c, c.C.data(), ib + 1, c.HC[ib], c.part);
@@ -313,6 +372,7 @@ int davidson_solve(Context &c, int max_it, int nb, double eps, double max_time, 
       c.trace.push_back(nr);
       steps++;
       if (nr < eps) {
+        if (approx) break;  // confirm on the exact residual: restart from W
         done = true;
         break;
       }
@@ -343,6 +403,187 @@ int davidson_solve(Context &c, int max_it, int nb, double eps, double max_time, 
         break;
       }
     }
+  }
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  if (energy) *energy = E0;
+  if (n_steps) *n_steps = steps;
+  return done ? 0 : 5;
+}
+
+// Thick-restart Davidson (no counterpart in the reference, which restarts from
+// W alone).  Per step: one new basis vector from the preconditioned residual
+// (same preconditioner, MGS and normalization as above), its product, the new
+// subspace column, Rayleigh-Ritz.  When the basis is full, or when the
+// in-cycle residual estimate falls below eps, the solver restarts:
+//   1. exact FP64 product of the Ritz vector W, E = <W|HW>, R = HW - E W:
+//      convergence is decided only on this exact |R|;
+//   2. the basis keeps W (with its exact product) and the next keep-1 Ritz
+//      vectors, whose products are linear combinations of the old ones.
+// With mixed != 0 the in-cycle products are FP32 (sigma_alpha_first_f32);
+// their error enters W only through coefficients that shrink with the
+// residual, and every restart re-anchors the basis on an exact FP64 product.
+int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps, double max_time,
+                         bool mixed, double *energy, int *n_steps) {
+  if (nb < 3 || nb > kNbMax) throw Error(1, "davidson: nb must be in [3, 64]");
+  const int m = std::max(1, std::min(keep, std::min(8, nb - 2)));
+  ensure_subspace(c, nb);
+  while ((int)c.spareY.size() < m) {
+    c.spareY.push_back((double *)dalloc(c, sizeof(double) * c.N));
+    c.spareHY.push_back((double *)dalloc(c, sizeof(double) * c.N));
+  }
+  const double eps_reg = 1.0e-12;
+  const int ld = kNbMax;
+  double *H = c.dsmall, *U = c.dsmall + ld * ld, *E = c.dsmall + 2 * ld * ld;
+  double *scal = c.dsmall + kScalarOffset;
+  const size_t N = c.N;
+  const dim3 g(kRedBlocks), b(kRedThreads);
+  c.trace.clear();
+  auto t_start = std::chrono::steady_clock::now();
+  auto elapsed = [&]() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  };
+  auto host2 = [&](const double *d0, const double *d1, double &a, double &bb) {
+    SBDX_HIP(hipMemcpyAsync(&a, d0, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipMemcpyAsync(&bb, d1, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+  };
+  auto add_column = [&](int col) {
+    dots(c, c.C.data(), col + 1, c.HC[col], c.part);
+    hipLaunchKernelGGL(k_finish_hcol, dim3(1), dim3(kNbMax), 0, c.stream, col + 1, kRedBlocks, c.part,
+                       H, ld, col);
+    check_launch("k_finish_hcol");
+  };
+  auto apply = [&](const double *x, double *y, bool approx) {
+    if (approx)
+      sigma_alpha_first_f32(c, x, y);
+    else
+      sigma_alpha_first(c, x, y);
+  };
+  // start: C[0] = W / |W|, exact product
+  SBDX_HIP(hipMemcpyAsync(c.C[0], c.W, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+  {
+    const double *X[1] = {c.C[0]};
+    dots(c, X, 1, c.C[0], c.part);
+    hipLaunchKernelGGL(k_scale_part, g, b, 0, c.stream, c.C[0], N, c.part, kRedBlocks);
+    check_launch("k_scale_part");
+  }
+  apply(c.C[0], c.HC[0], false);
+  add_column(0);
+  int d = 0, steps = 1;
+  bool done = false, fresh = true;
+  double E0 = 0.0, nr = 0.0;
+  while (true) {
+    {
+      ProfScope ps(c, "dav_subspace");
+      const size_t shm = sizeof(double) * 2 * (d + 1) * (d + 1);
+      hipLaunchKernelGGL(k_syev, dim3(1), dim3(64), shm, c.stream, d + 1, ld, H, U, E);
+      check_launch("k_syev");
+    }
+    {
+      ProfScope ps(c, "dav_ritz");
+      VecList Cv{}, HCv{};
+      for (int k = 0; k <= d; k++) {
+        Cv.p[k] = c.C[k];
+        HCv.p[k] = c.HC[k];
+      }
+      hipLaunchKernelGGL(k_ritz, g, b, 0, c.stream, Cv, HCv, d + 1, U, E, N, c.W, c.R, c.part);
+      check_launch("k_ritz");
+      hipLaunchKernelGGL(k_scale2, g, b, 0, c.stream, c.W, c.R, N, c.part, kRedBlocks, scal);
+      check_launch("k_scale2");
+    }
+    host2(E, scal, E0, nr);
+    c.trace.push_back(E0);
+    c.trace.push_back(nr);
+    const bool exact_now = !mixed;  // every product in the basis is FP64
+    if (nr < eps && exact_now) {
+      done = true;
+      break;
+    }
+    const bool timeout = elapsed() > max_time || steps >= max_steps;
+    if (d == nb - 1 || (nr < eps && !fresh) || timeout) {
+      // ---- restart with an exact residual check
+      ProfScope ps(c, "dav_restart");
+      double *HW = c.spareHY[0];
+      apply(c.W, HW, false);
+      steps++;
+      const double *X[1] = {c.W};
+      dots(c, X, 1, HW, c.part);
+      double *Eex = scal + 2;
+      hipLaunchKernelGGL(k_exact_resid, g, b, 0, c.stream, c.W, HW, N, c.part, kRedBlocks, c.R,
+                         c.part + kRedBlocks, Eex);
+      check_launch("k_exact_resid");
+      hipLaunchKernelGGL(k_finish_scalar, dim3(1), dim3(1), 0, c.stream, kRedBlocks,
+                         c.part + kRedBlocks, scal + 3);
+      check_launch("k_finish_scalar");
+      double e_ex, r2;
+      host2(Eex, scal + 3, e_ex, r2);
+      E0 = e_ex;
+      nr = std::sqrt(r2);
+      c.trace.push_back(E0);
+      c.trace.push_back(nr);
+      if (nr < eps) {
+        done = true;
+        break;
+      }
+      if (timeout) break;
+      // R /= |R| for the preconditioner (the reference normalizes R too)
+      {
+        const double *Y[1] = {c.R};
+        dots(c, Y, 1, c.R, c.part);
+        hipLaunchKernelGGL(k_scale_part, g, b, 0, c.stream, c.R, N, c.part, kRedBlocks);
+        check_launch("k_scale_part");
+      }
+      // thick restart basis: W (exact product) + Ritz vectors 1..m-1
+      if (m > 1) {
+        VecList Cv{}, HCv{};
+        for (int k = 0; k <= d; k++) {
+          Cv.p[k] = c.C[k];
+          HCv.p[k] = c.HC[k];
+        }
+        OutList Yo{}, HYo{};
+        for (int j = 1; j < m; j++) {
+          Yo.p[j - 1] = c.spareY[j];
+          HYo.p[j - 1] = c.spareHY[j];
+        }
+        hipLaunchKernelGGL(k_combine, g, b, 0, c.stream, Cv, d + 1, U, ld, 1, m - 1, Yo, N);
+        check_launch("k_combine");
+        hipLaunchKernelGGL(k_combine, g, b, 0, c.stream, HCv, d + 1, U, ld, 1, m - 1, HYo, N);
+        check_launch("k_combine");
+      }
+      SBDX_HIP(hipMemcpyAsync(c.C[0], c.W, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+      std::swap(c.HC[0], c.spareHY[0]);
+      for (int j = 1; j < m; j++) {
+        std::swap(c.C[j], c.spareY[j]);
+        std::swap(c.HC[j], c.spareHY[j]);
+      }
+      d = m - 1;
+      for (int j = 0; j <= d; j++) add_column(j);
+      fresh = true;
+      // expand from the exact residual (E0 = exact energy for the preconditioner)
+      SBDX_HIP(hipMemcpyAsync(E, Eex, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    }
+    // ---- expand: preconditioned residual, MGS against the basis, normalize
+    {
+      ProfScope ps(c, "dav_orth");
+      double *cn = c.C[d + 1];
+      hipLaunchKernelGGL(k_precond, g, b, 0, c.stream, c.R, c.hii, E, N, eps_reg, cn);
+      check_launch("k_precond");
+      for (int kb = 0; kb <= d; kb++) {
+        const double *X[1] = {c.C[kb]};
+        dots(c, X, 1, cn, c.part);
+        hipLaunchKernelGGL(k_axpy_part, g, b, 0, c.stream, cn, c.C[kb], N, c.part, kRedBlocks);
+        check_launch("k_axpy_part");
+      }
+      const double *X[1] = {cn};
+      dots(c, X, 1, cn, c.part);
+      hipLaunchKernelGGL(k_scale_part, g, b, 0, c.stream, cn, N, c.part, kRedBlocks);
+      check_launch("k_scale_part");
+    }
+    apply(c.C[d + 1], c.HC[d + 1], mixed);
+    steps++;
+    d++;
+    add_column(d);
+    fresh = false;
   }
   SBDX_HIP(hipStreamSynchronize(c.stream));
   if (energy) *energy = E0;

@@ -7,6 +7,8 @@
 // a time; popcount of the XOR gives the distance and the wave ballot + a
 // masked popcount give each match its slot, so the order is ascending
 // without sorting.
+#include <algorithm>
+
 #include "common.h"
 
 namespace sbdx {
@@ -232,6 +234,62 @@ void build_spin_tables(Context &c, SpinTables &t, const std::vector<uint64_t> &s
   hipLaunchKernelGGL(k_string_energy, dim3(cdiv(n, 256)), dim3(256), 0, c.stream, t.str, n, c.L,
                      c.eri, c.h1, t.e1, t.jsum);
   check_launch("k_string_energy");
+}
+
+// Sliced-ELL copy of the singles (see SpinTables::ell).  Needs kets < 2^16
+// and pq < 2^15; otherwise has_ell stays false and the CSR kernels run.
+void build_singles_ell(Context &c, SpinTables &t) {
+  t.has_ell = false;
+  if (t.n > 65536 || c.K >= 32768) return;
+  const int n = t.n;
+  std::vector<uint64_t> off(n + 1), ent(t.ns);
+  SBDX_HIP(hipMemcpyAsync(off.data(), t.s_off, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipMemcpyAsync(ent.data(), t.s_ent, sizeof(uint64_t) * t.ns, hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  std::vector<uint32_t> order(n);
+  for (int i = 0; i < n; i++) order[i] = (uint32_t)i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return (off[a + 1] - off[a]) > (off[b + 1] - off[b]);
+  });
+  const int nsl = (n + 63) / 64;
+  std::vector<uint32_t> perm((size_t)nsl * 64, 0xffffffffu), sl_len(nsl);
+  std::vector<uint64_t> sl_off(nsl);
+  uint64_t total = 0;
+  for (int s = 0; s < nsl; s++) {
+    uint32_t mx = 0;
+    for (int l = 0; l < 64 && s * 64 + l < n; l++) {
+      const uint32_t r = order[s * 64 + l];
+      perm[(size_t)s * 64 + l] = r;
+      mx = std::max<uint32_t>(mx, (uint32_t)(off[r + 1] - off[r]));
+    }
+    sl_len[s] = mx;
+    sl_off[s] = total;
+    total += (uint64_t)mx * 64;
+  }
+  const uint32_t pad = (uint32_t)c.K << 16;
+  std::vector<uint32_t> ell(total + 1, pad);
+  for (int s = 0; s < nsl; s++)
+    for (int l = 0; l < 64; l++) {
+      const uint32_t r = perm[(size_t)s * 64 + l];
+      if (r == 0xffffffffu) continue;
+      for (uint64_t k = 0; k < off[r + 1] - off[r]; k++) {
+        const uint64_t e = ent[off[r] + k];
+        const uint32_t ket = (uint32_t)(e & 0xffffffffull);
+        const uint32_t pq = (uint32_t)((e >> 32) & 0x7fffffffull);
+        ell[sl_off[s] + 64 * k + l] = ket | (pq << 16) | ((e >> 63) ? 0x80000000u : 0u);
+      }
+    }
+  t.nslice = nsl;
+  t.ell = (uint32_t *)dalloc(c, sizeof(uint32_t) * ell.size());
+  t.perm = (uint32_t *)dalloc(c, sizeof(uint32_t) * perm.size());
+  t.sl_off = (uint64_t *)dalloc(c, sizeof(uint64_t) * nsl);
+  t.sl_len = (uint32_t *)dalloc(c, sizeof(uint32_t) * nsl);
+  SBDX_HIP(hipMemcpyAsync(t.ell, ell.data(), sizeof(uint32_t) * ell.size(), hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(t.perm, perm.data(), sizeof(uint32_t) * perm.size(), hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(t.sl_off, sl_off.data(), sizeof(uint64_t) * nsl, hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(t.sl_len, sl_len.data(), sizeof(uint32_t) * nsl, hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  t.has_ell = true;
 }
 
 void build_dress(Context &c, SpinTables &t) {

@@ -39,8 +39,9 @@ ABI_SYMBOLS = [
     "sbdx_sigma", "sbdx_sigma_device", "sbdx_davidson", "sbdx_set_vector", "sbdx_solve",
     "sbdx_get_vector", "sbdx_trace", "sbdx_energy", "sbdx_rdm", "sbdx_table_count",
     "sbdx_table_export", "sbdx_profile", "sbdx_profile_count", "sbdx_profile_get",
-    "sbdx_work_counts",
+    "sbdx_work_counts", "sbdx_set_precision", "sbdx_sigma_fp32", "sbdx_set_restart",
 ]
+PREC_FP64, PREC_MIXED = 0, 1
 
 
 class SbdxError(RuntimeError):
@@ -83,6 +84,9 @@ def lib():
     L.sbdx_profile_get.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_size_t,
                                    C.POINTER(C.c_long), C.POINTER(C.c_double)]
     L.sbdx_work_counts.argtypes = [C.c_void_p, _dp]
+    L.sbdx_set_precision.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_sigma_fp32.argtypes = [C.c_void_p, _dp, _dp]
+    L.sbdx_set_restart.argtypes = [C.c_void_p, C.c_int]
     _LIB = L
     return L
 
@@ -145,6 +149,29 @@ class TPBOperator:
 
     def sigma(self, Wk):
         return self.run(Wk, None, accumulate=False)
+
+    def sigma_fp32(self, Wk):
+        """H*Wk in FP32 arithmetic (the mixed solver's correction product)."""
+        out = np.zeros(self.ndet)
+        _check(lib().sbdx_sigma_fp32(self.h, np.ascontiguousarray(Wk, dtype=np.float64), out))
+        return out
+
+    def set_precision(self, mode):
+        """PREC_FP64 (reference algorithm) or PREC_MIXED for later solves."""
+        _check(lib().sbdx_set_precision(self.h, int(mode)))
+
+    def set_restart(self, keep):
+        """0: restart from W alone (reference); k >= 1: thick restart keeping
+        k Ritz vectors, exact FP64 residual at every restart."""
+        _check(lib().sbdx_set_restart(self.h, int(keep)))
+
+    def configure(self, solver):
+        """'plain' (reference algorithm, FP64), 'thick' (FP64 thick restart)
+        or 'mixed' (thick restart, FP32 correction products)."""
+        keep, prec = {"plain": (0, PREC_FP64), "thick": (4, PREC_FP64),
+                      "mixed": (4, PREC_MIXED)}[solver]
+        self.set_restart(keep)
+        self.set_precision(prec)
 
     def energy(self, W):
         e = C.c_double()

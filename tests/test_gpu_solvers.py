@@ -1,0 +1,58 @@
+"""GPU tests of the solver variants beyond the reference algorithm: the FP32
+sigma used by the mixed solver, and the thick-restart FP64 / mixed solvers,
+which must reach the plain FP64 solver's energy (the FP64 oracle's) to
+1e-9 Ha with an exact FP64 residual below eps."""
+import numpy as np
+import pytest
+
+from conftest import molecule, random_vector
+
+pytestmark = pytest.mark.gpu
+
+
+def _synthetic():
+    from sbdx import instances as I
+    return I.make_instance(norb=12, nelec_half=4, nstr=150, nchol=12, seed=21)
+
+
+CASES = {"h2o_1em3": lambda: molecule("h2o", "1em3"), "syn12": _synthetic}
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def case(request):
+    import sbdx
+    inst = CASES[request.param]()
+    print(f"[solvers {request.param}] na={len(inst['astr'])}", flush=True)
+    op = sbdx.TPBOperator.from_instance(inst)
+    yield request.param, inst, op
+    op.close()
+
+
+def test_sigma_fp32_accuracy(case):
+    name, inst, op = case
+    v = random_vector(op.ndet, 5)
+    s64 = op.sigma(v)
+    s32 = op.sigma_fp32(v)
+    err = np.linalg.norm(s32 - s64) / np.linalg.norm(s64)
+    assert err < 2e-6, err
+
+
+@pytest.mark.parametrize("solver", ["thick", "mixed"])
+def test_solver_matches_plain(case, solver, oracle_mod):
+    import sbdx
+    name, inst, op = case
+    W0 = sbdx.basis_init_vector(op.ndet) if name.startswith("h2o") else random_vector(op.ndet, 99)
+    orc = oracle_mod.Oracle.from_instance(inst)
+    ref = orc.davidson(W=W0, nb=12, eps=1e-9, max_it=300)
+    assert ref["converged"]
+    op.configure(solver)
+    op.set_vector(W0)
+    E, steps, conv = op.solve(max_it=300, nb=12, eps=1e-9)
+    op.configure("plain")
+    assert conv
+    W = op.get_vector()
+    assert abs(E - ref["E"]) < 1e-9, (solver, E, ref["E"])
+    # exact FP64 residual of the returned vector
+    r = op.sigma(W) - E * W
+    assert np.linalg.norm(r) < 5e-9
+    assert abs(op.energy(W) - ref["E"]) < 1e-9

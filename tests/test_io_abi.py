@@ -100,7 +100,7 @@ def test_workload_sizes():
 
 def _header_functions():
     txt = open(os.path.join(ROOT, "include", "sbdx.h")).read()
-    return sorted(set(re.findall(r"\b(sbdx_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(sbdx_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_abi_exports_every_declared_symbol():
