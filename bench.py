@@ -99,6 +99,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--quiet", action="store_true", help="no per-step solver lines on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no "
@@ -129,6 +130,7 @@ def main():
 
     def run(solver, nsolve):
         op.configure(solver)
+        op.set_verbose(0 if args.quiet else 1)
         op.profile(True)
         times, energies, nsteps, conv = [], [], [], []
         for k in range(nsolve):

@@ -117,9 +117,49 @@ int sbdx_set_precision(sbdx_handle_t h, int mode);
  *               combination).  max_it then bounds max_it*nb products. */
 int sbdx_set_restart(sbdx_handle_t h, int keep);
 
+/* verbose != 0: the solves print one line per step to stderr, step index,
+ * E0 and |r| (as davidson.h:839-851 prints per step); thick-restart solves
+ * mark the exact FP64 residual of each restart.  0 (default) prints nothing. */
+int sbdx_set_verbose(sbdx_handle_t h, int verbose);
+
 /* Wb = H*Wk evaluated in FP32 (inputs rounded to FP32, result widened):
  * the product the mixed solver applies to correction vectors. */
 int sbdx_sigma_fp32(sbdx_handle_t h, const double *Wk, double *Wb);
+
+/*
+ * Learned preconditioner (no counterpart in the reference, whose
+ * preconditioner is R/(E0-d), davidson.h:861-868): a per-determinant MLP
+ * 160 -> 128 -> 128 -> 1 with GELU, BF16 weights on MFMA with FP32
+ * accumulation, FP32 master weights trained with Adam by the library's own
+ * forward/backward kernels.  Features are BF16, sbdx_mlp_nfeat() per row;
+ * parameters are FP32, sbdx_mlp_nparam() in the order W1[160][128], b1[128],
+ * W2[128][128], b2[128], w3[128], b3 (+3 pad).
+ */
+int sbdx_mlp_nparam(void);
+int sbdx_mlp_nfeat(void);
+int sbdx_mlp_set_weights(sbdx_handle_t h, const float *params, size_t n);
+int sbdx_mlp_get_weights(sbdx_handle_t h, float *params, size_t n);
+/* out[i] = MLP(feat[i]) for n rows (host buffers). */
+int sbdx_mlp_forward(sbdx_handle_t h, const uint16_t *feat, size_t n, float *out);
+/* Gradient of mean((MLP(feat) - target)^2) over the n rows, one batch. */
+int sbdx_mlp_gradient(sbdx_handle_t h, const uint16_t *feat, const float *target, size_t n,
+                      float *grad, double *sum_sq_err);
+/* Adam (lr, mini-batches of `batch`, shuffled with seed) for `epochs`
+ * passes; loss[epoch] receives each epoch's mean squared error. */
+int sbdx_mlp_train(sbdx_handle_t h, const uint16_t *feat, const float *target, size_t n, int epochs,
+                   int batch, float lr, uint64_t seed, double *loss);
+/* Record up to cap (residual, Ritz vector, Ritz value) steps during the next
+ * thick-restart solves (cap = 0 stops recording). */
+int sbdx_mlp_record(sbdx_handle_t h, int cap);
+/* Training pairs from the recorded steps of the last solve, against its
+ * converged vector: up to max_pairs rows of features and targets. */
+int sbdx_mlp_collect(sbdx_handle_t h, size_t max_pairs, uint64_t seed, uint16_t *feat, float *target,
+                     size_t *n_pairs);
+/* Preconditioner of thick-restart solves: 0 diagonal, 1 MLP (falls back to
+ * the diagonal step for 3 expansions whenever |r| has not dropped by 10%
+ * over the last 3 steps).  Counts of each kind in the last solve: */
+int sbdx_set_preconditioner(sbdx_handle_t h, int mode);
+int sbdx_precond_stats(sbdx_handle_t h, long *mlp_steps, long *diag_steps);
 
 /* History of the last solve: trace[2*k] = E0, trace[2*k+1] = |r| of step k,
  * the values the reference prints per Davidson step (davidson.h:839-851). */

@@ -3,6 +3,7 @@
 // and the device's alpha-first basis, profiling, and error reporting.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -136,6 +137,12 @@ static Context *create(int L, size_t bl, const uint64_t *aw, size_t na, const ui
     SBDX_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device));
     SBDX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
     c.lds_limit = (size_t)lds;
+    // SBDX_LDS_LIMIT=<bytes> lowers the per-block LDS budget the kernels plan
+    // for, which selects the row-parts and CSR fallbacks on small instances.
+    if (const char *ev = std::getenv("SBDX_LDS_LIMIT")) {
+      const long v = std::atol(ev);
+      if (v > 0 && (size_t)v < c.lds_limit) c.lds_limit = (size_t)v;
+    }
     c.n_cu = ncu;
     SBDX_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     c.L = L;
@@ -375,6 +382,13 @@ int sbdx_set_restart(sbdx_handle_t h, int keep) {
   SBDX_GUARD_END
 }
 
+int sbdx_set_verbose(sbdx_handle_t h, int level) {
+  SBDX_GUARD_BEGIN
+  ctx_of(h).verbose = level;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
 int sbdx_sigma_fp32(sbdx_handle_t h, const double *Wk, double *Wb) {
   SBDX_GUARD_BEGIN
   Context &c = ctx_of(h);
@@ -384,6 +398,117 @@ int sbdx_sigma_fp32(sbdx_handle_t h, const double *Wk, double *Wb) {
   sigma_alpha_first_f32(c, c.vin, c.vout);
   sign_convert(c, c.vout, c.vin);
   download(c, Wb, c.vin);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_mlp_nparam(void) { return mlp_nparam(); }
+int sbdx_mlp_nfeat(void) { return mlp_nfeat(); }
+
+int sbdx_mlp_set_weights(sbdx_handle_t h, const float *params, size_t n) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!params || n != (size_t)mlp_nparam()) throw Error(SBDX_ERR_ARG, "mlp: wrong parameter count");
+  mlp_set(c, params);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_mlp_get_weights(sbdx_handle_t h, float *params, size_t n) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!params || n != (size_t)mlp_nparam()) throw Error(SBDX_ERR_ARG, "mlp: wrong parameter count");
+  mlp_get(c, params);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_mlp_forward(sbdx_handle_t h, const uint16_t *feat, size_t n, float *out) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!feat || !out || n == 0) throw Error(SBDX_ERR_ARG, "mlp: empty input");
+  mlp_ensure(c);
+  const size_t nf = (size_t)mlp_nfeat();
+  TmpBuf df(sizeof(uint16_t) * n * nf), dy(sizeof(float) * n);
+  SBDX_HIP(hipMemcpyAsync(df.p, feat, sizeof(uint16_t) * n * nf, hipMemcpyHostToDevice, c.stream));
+  mlp_forward(c, (const unsigned short *)df.p, n, (float *)dy.p);
+  SBDX_HIP(hipMemcpyAsync(out, dy.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_mlp_gradient(sbdx_handle_t h, const uint16_t *feat, const float *target, size_t n, float *grad,
+                      double *sse) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!feat || !target || !grad || n == 0) throw Error(SBDX_ERR_ARG, "mlp: empty input");
+  mlp_ensure(c);
+  const size_t nf = (size_t)mlp_nfeat();
+  TmpBuf df(sizeof(uint16_t) * n * nf), dt(sizeof(float) * n), di(sizeof(uint32_t) * n);
+  std::vector<uint32_t> idx(n);
+  for (size_t i = 0; i < n; i++) idx[i] = (uint32_t)i;
+  SBDX_HIP(hipMemcpyAsync(df.p, feat, sizeof(uint16_t) * n * nf, hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(dt.p, target, sizeof(float) * n, hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(di.p, idx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
+  const double s = mlp_gradient(c, (const unsigned short *)df.p, (const float *)dt.p,
+                                (const uint32_t *)di.p, n);
+  if (sse) *sse = s;
+  SBDX_HIP(hipMemcpyAsync(grad, c.mlp.grad, sizeof(float) * mlp_nparam(), hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_mlp_train(sbdx_handle_t h, const uint16_t *feat, const float *target, size_t n, int epochs,
+                   int batch, float lr, uint64_t seed, double *loss) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!feat || !target || n == 0 || epochs < 1 || batch < 1) throw Error(SBDX_ERR_ARG, "mlp: bad training arguments");
+  mlp_ensure(c);
+  const size_t nf = (size_t)mlp_nfeat();
+  TmpBuf df(sizeof(uint16_t) * n * nf), dt(sizeof(float) * n);
+  SBDX_HIP(hipMemcpyAsync(df.p, feat, sizeof(uint16_t) * n * nf, hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(dt.p, target, sizeof(float) * n, hipMemcpyHostToDevice, c.stream));
+  mlp_train(c, (const unsigned short *)df.p, (const float *)dt.p, n, epochs, batch, lr, seed, loss);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_mlp_record(sbdx_handle_t h, int cap) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (cap < 0 || cap > 4096) throw Error(SBDX_ERR_ARG, "mlp: record cap must be in [0, 4096]");
+  c.mlp.record_cap = cap;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_mlp_collect(sbdx_handle_t h, size_t max_pairs, uint64_t seed, uint16_t *feat, float *target,
+                     size_t *n_pairs) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!feat || !target || !n_pairs) throw Error(SBDX_ERR_ARG, "mlp: null output");
+  *n_pairs = mlp_collect(c, max_pairs, seed, (unsigned short *)feat, target);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_set_preconditioner(sbdx_handle_t h, int mode) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (mode != 0 && mode != 1) throw Error(SBDX_ERR_ARG, "preconditioner mode must be 0 or 1");
+  if (mode == 1 && !c.mlp.valid) throw Error(SBDX_ERR_ARG, "mlp preconditioner has no weights");
+  c.precond = mode;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_precond_stats(sbdx_handle_t h, long *mlp_steps, long *diag_steps) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (mlp_steps) *mlp_steps = c.mlp.used_mlp;
+  if (diag_steps) *diag_steps = c.mlp.used_diag;
   return SBDX_OK;
   SBDX_GUARD_END
 }

@@ -65,6 +65,13 @@ struct SpinTables {
   uint32_t *sl_len = nullptr;  // [nslice]
   int nslice = 0;
   bool has_ell = false;
+  // The same singles split by ket range into `nparts` parts of `part_h`
+  // kets (ket stored relative to its part), same perm and slices; built on
+  // demand when one ket row does not fit in LDS.
+  int nparts = 0, part_h = 0;
+  std::vector<uint32_t *> pell;
+  std::vector<uint64_t *> psl_off;
+  std::vector<uint32_t *> psl_len;
 };
 
 struct ProfSlot {
@@ -81,6 +88,20 @@ struct F32Operands {
   float *x = nullptr, *y = nullptr, *T1 = nullptr, *T2 = nullptr;
 };
 
+// Learned preconditioner (mlp.hip): FP32 master weights, gradient, Adam
+// moments, and the per-step history a solve records for training pairs.
+struct MlpState {
+  float *prm = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+  int adam_t = 0;
+  bool valid = false;
+  int record_cap = 0;              // >0: the thick solver records (R, W, E) per expansion
+  std::vector<double *> rec_R, rec_W;
+  std::vector<double> rec_E;
+  int rec_n = 0;
+  // preconditioner use statistics of the last solve
+  long used_mlp = 0, used_diag = 0;
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -93,8 +114,11 @@ struct Context {
   double *hii = nullptr;   // [N]
   F32Operands f32;
   int precision = 0;       // 0: FP64 throughout; 1: FP32 sigma on corrections, FP64 restarts
+  int verbose = 0;         // 1: one stderr line per Davidson step (E0, |r|)
   int restart_keep = 0;    // 0: restart from W alone (reference); k: thick restart keeping k Ritz vectors
   std::vector<double *> spareY, spareHY;  // thick-restart staging vectors
+  MlpState mlp;
+  int precond = 0;         // 0: diagonal R/(E0-d); 1: MLP with diagonal fallback on stall
   // work vectors
   double *vin = nullptr, *vout = nullptr;   // API staging (alpha-first)
   double *T1 = nullptr, *T2 = nullptr;      // transpose scratch [N]
@@ -151,6 +175,7 @@ void check_launch(const char *what);
 // tables.hip
 void build_spin_tables(Context &c, SpinTables &t, const std::vector<uint64_t> &strs);
 void build_singles_ell(Context &c, SpinTables &t);
+void build_singles_ell_parts(Context &c, SpinTables &t, int nparts);
 void build_dress(Context &c, SpinTables &t);
 void compute_diag(Context &c);
 void sign_convert(Context &c, const double *in, double *out);      // out = S*in
@@ -171,5 +196,21 @@ double device_dot(Context &c, const double *x, const double *y, size_t n);
 
 // rdm.hip
 void compute_rdm(Context &c, const double *Wp, double *one, double *two);
+
+// mlp.hip
+int mlp_nparam();
+int mlp_nfeat();
+void mlp_ensure(Context &c);
+void mlp_set(Context &c, const float *host);
+void mlp_get(Context &c, float *host);
+void mlp_forward(Context &c, const unsigned short *dfeat, size_t rows, float *dout);
+double mlp_gradient(Context &c, const unsigned short *dfeat, const float *dtarget, const uint32_t *didx,
+                    size_t nbatch);
+void mlp_adam_step(Context &c, float lr);
+void mlp_precond(Context &c, const double *R, const double *E, double *Cout);
+void mlp_features(Context &c, const double *R, double e, unsigned short *dout);
+size_t mlp_collect(Context &c, size_t max_pairs, uint64_t seed, unsigned short *feat_out, float *tgt_out);
+void mlp_train(Context &c, const unsigned short *dfeat, const float *dtgt, size_t n, int epochs, int batch,
+               float lr, uint64_t seed, double *loss);
 
 }  // namespace sbdx

@@ -277,7 +277,7 @@ __global__ void __launch_bounds__(kRedThreads) k_combine(VecList in, int nin, co
   }
 }
 
-// R = HW - E*W with E = <W|HW> (from partials); partial |R|^2 per block.
+// R = E*W - HW with E = <W|HW> (from partials); partial |R|^2 per block.
 __global__ void __launch_bounds__(kRedThreads) k_exact_resid(const double *__restrict__ W,
                                                              const double *__restrict__ HW, size_t n,
                                                              const double *__restrict__ epart, int nblk,
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(kRedThreads) k_exact_resid(const double *__res
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const double r = HW[i] - E * W[i];
+    const double r = E * W[i] - HW[i];  // same sign as k_ritz's residual
     R[i] = r;
     s += r * r;
   }
@@ -342,7 +342,14 @@ int davidson_solve(Context &c, int max_it, int nb, double eps, double max_time, 
         sigma_alpha_first(c, c.C[ib], c.HC[ib]);
       {
         ProfScope ps(c, "dav_subspace");
-        dots(c, c.C.data(), ib + 1, c.HC[ib], c.part);
+        // H[k][ib] = <C_k|H|C_ib>.  With an FP32 product in HC[ib] the
+        // coupling to the restart vector is taken as C_ib . HC[k] instead:
+        // HC[0] is exact, and <W|H|t> = t . (HW) is of the size of the
+        // residual, which the FP32 error of H t (~1e-7 |H|) would swamp.
+        if (approx)
+          dots(c, c.HC.data(), ib + 1, c.C[ib], c.part);
+        else
+          dots(c, c.C.data(), ib + 1, c.HC[ib], c.part);
         hipLaunchKernelGGL(k_finish_hcol, dim3(1), dim3(kNbMax), 0, c.stream, ib + 1, kRedBlocks,
                            c.part, H, ld, ib);
         check_launch("k_finish_hcol");
@@ -371,6 +378,10 @@ int davidson_solve(Context &c, int max_it, int nb, double eps, double max_time, 
       c.trace.push_back(E0);
       c.trace.push_back(nr);
       steps++;
+      if (c.verbose) {
+        fprintf(stderr, "[sbdx] davidson %d.%d E0 = %.15f |r| = %.3e\n", it, ib, E0, nr);
+        fflush(stderr);
+      }
       if (nr < eps) {
         if (approx) break;  // confirm on the exact residual: restart from W
         done = true;
@@ -447,8 +458,13 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
     SBDX_HIP(hipMemcpyAsync(&bb, d1, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SBDX_HIP(hipStreamSynchronize(c.stream));
   };
+  // Column col of the subspace matrix as C[col] . HC[k], k <= col: the
+  // older vector's product is used, so the coupling of every later vector to
+  // the restart vector (whose product is exact FP64) stays exact in mixed
+  // mode; an FP32 error of ~1e-7 |H| there would exceed the residual-sized
+  // coupling <W|H|t> = t . r near convergence.
   auto add_column = [&](int col) {
-    dots(c, c.C.data(), col + 1, c.HC[col], c.part);
+    dots(c, c.HC.data(), col + 1, c.C[col], c.part);
     hipLaunchKernelGGL(k_finish_hcol, dim3(1), dim3(kNbMax), 0, c.stream, col + 1, kRedBlocks, c.part,
                        H, ld, col);
     check_launch("k_finish_hcol");
@@ -472,6 +488,11 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
   int d = 0, steps = 1;
   bool done = false, fresh = true;
   double E0 = 0.0, nr = 0.0;
+  std::vector<double> rhist;
+  int diag_left = 0;
+  c.mlp.used_mlp = c.mlp.used_diag = 0;
+  c.mlp.rec_n = 0;
+  c.mlp.rec_E.clear();
   while (true) {
     {
       ProfScope ps(c, "dav_subspace");
@@ -494,6 +515,10 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
     host2(E, scal, E0, nr);
     c.trace.push_back(E0);
     c.trace.push_back(nr);
+    if (c.verbose) {
+      fprintf(stderr, "[sbdx] thick step %d dim %d E0 = %.15f |r| ~ %.3e\n", steps, d + 1, E0, nr);
+      fflush(stderr);
+    }
     const bool exact_now = !mixed;  // every product in the basis is FP64
     if (nr < eps && exact_now) {
       done = true;
@@ -521,6 +546,10 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       nr = std::sqrt(r2);
       c.trace.push_back(E0);
       c.trace.push_back(nr);
+      if (c.verbose) {
+        fprintf(stderr, "[sbdx] restart step %d E0 = %.15f exact |r| = %.3e\n", steps, E0, nr);
+        fflush(stderr);
+      }
       if (nr < eps) {
         done = true;
         break;
@@ -563,11 +592,39 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       SBDX_HIP(hipMemcpyAsync(E, Eex, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     }
     // ---- expand: preconditioned residual, MGS against the basis, normalize
+    rhist.push_back(nr);
+    if (c.mlp.record_cap > 0 && c.mlp.rec_n < c.mlp.record_cap) {
+      // training pairs: residual, Ritz vector and Ritz value of this step
+      const int k = c.mlp.rec_n++;
+      if ((int)c.mlp.rec_R.size() <= k) {
+        c.mlp.rec_R.push_back((double *)dalloc(c, sizeof(double) * N));
+        c.mlp.rec_W.push_back((double *)dalloc(c, sizeof(double) * N));
+      }
+      SBDX_HIP(hipMemcpyAsync(c.mlp.rec_R[k], c.R, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+      SBDX_HIP(hipMemcpyAsync(c.mlp.rec_W[k], c.W, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+      c.mlp.rec_E.push_back(E0);
+    }
+    // learned step unless the residual stalls (no 10% drop over 3 steps):
+    // then the diagonal step for the next 3 expansions
+    const size_t nh = rhist.size();
+    if (nh >= 4 && rhist[nh - 1] > 0.9 * rhist[nh - 4] && diag_left == 0) diag_left = 3;
+    const bool use_mlp = c.precond == 1 && c.mlp.valid && diag_left == 0;
+    if (diag_left > 0) diag_left--;
+    {
+      ProfScope ps(c, use_mlp ? "precond_mlp" : "dav_orth");
+      double *cn = c.C[d + 1];
+      if (use_mlp) {
+        mlp_precond(c, c.R, E, cn);
+        c.mlp.used_mlp++;
+      } else {
+        hipLaunchKernelGGL(k_precond, g, b, 0, c.stream, c.R, c.hii, E, N, eps_reg, cn);
+        check_launch("k_precond");
+        c.mlp.used_diag++;
+      }
+    }
     {
       ProfScope ps(c, "dav_orth");
       double *cn = c.C[d + 1];
-      hipLaunchKernelGGL(k_precond, g, b, 0, c.stream, c.R, c.hii, E, N, eps_reg, cn);
-      check_launch("k_precond");
       for (int kb = 0; kb <= d; kb++) {
         const double *X[1] = {c.C[kb]};
         dots(c, X, 1, cn, c.part);

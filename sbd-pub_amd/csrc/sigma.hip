@@ -169,6 +169,77 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int nslots,
   }
 }
 
+// Mixed term when one ket row does not fit in LDS: the row is staged in
+// nparts pieces of h kets and each piece is walked with its own sliced-ELL
+// table (SpinTables::pell, kets relative to the piece); the sign-folded ERI
+// row stays staged for all pieces.
+struct PartTables {
+  const uint32_t *ell[8];
+  const uint64_t *off[8];
+  const uint32_t *len[8];
+};
+
+template <typename T, int NT>
+__global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int nslots, int nparts, int h,
+                                                      const uint64_t *__restrict__ a_off,
+                                                      const uint64_t *__restrict__ a_ent,
+                                                      PartTables pt, const uint32_t *__restrict__ perm,
+                                                      const T *__restrict__ eri,
+                                                      const T *__restrict__ x, T *__restrict__ y) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *crow = reinterpret_cast<T *>(lds_raw);
+  T *vrow = crow + h;
+  const int I = blockIdx.x;
+  const int tid = threadIdx.x;
+  T acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = T(0);
+  for (uint64_t e = a_off[I]; e < a_off[I + 1]; e++) {
+    const uint64_t ent = a_ent[e];
+    const size_t J = ent_ket(ent);
+    const T sa = ent_neg(ent) ? T(-1) : T(1);
+    const T *vsrc = eri + (size_t)ent_pq(ent) * K;
+    __syncthreads();
+    for (int i = tid; i < K; i += 1024) vrow[i] = sa * vsrc[i];
+    if (tid == 0) vrow[K] = T(0);
+    for (int p = 0; p < nparts; p++) {
+      const int lo = p * h;
+      const int len_p = (nb - lo) < h ? (nb - lo) : h;
+      __syncthreads();
+      for (int i = tid; i < len_p; i += 1024) crow[i] = x[J * nb + lo + i];
+      __syncthreads();
+      const uint32_t *ellp = pt.ell[p];
+      const uint64_t *offp = pt.off[p];
+      const uint32_t *lenp = pt.len[p];
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const int slot = tid + t * 1024;
+        if (slot < nslots) {
+          const int sl = slot >> 6;
+          const uint32_t len = lenp[sl];
+          const uint32_t *q = ellp + offp[sl] + (slot & 63);
+          T a = T(0);
+#pragma unroll 4
+          for (uint32_t k = 0; k < len; k++) {
+            const uint32_t w = q[64 * (size_t)k];
+            const T v = vrow[(w >> 16) & 0x7fffu] * crow[w & 0xffffu];
+            a += (w >> 31) ? -v : v;
+          }
+          acc[t] += a;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int slot = tid + t * 1024;
+    if (slot < nslots) {
+      const uint32_t r = perm[slot];
+      if (r != 0xffffffffu) y[(size_t)I * nb + r] += acc[t];
+    }
+  }
+}
+
 // Mixed term with the beta singles in CSR form (used when the packed
 // sliced-ELL table cannot hold the indices): x[J,:] and V row in LDS.
 template <typename T, int NT>
@@ -328,13 +399,52 @@ static void launch_mixed_lds(Context &c, const T *eri, const T *x, T *y, size_t 
                      c.A.s_ent, c.B.s_off, c.B.s_ent, eri, x, y);
 }
 
+template <typename T, int NT>
+static void launch_mixed_parts(Context &c, const T *eri, const T *x, T *y, size_t lds) {
+  auto fn = k_mixed_parts<T, NT>;
+  SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  PartTables pt{};
+  for (int p = 0; p < c.B.nparts; p++) {
+    pt.ell[p] = c.B.pell[p];
+    pt.off[p] = c.B.psl_off[p];
+    pt.len[p] = c.B.psl_len[p];
+  }
+  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K,
+                     c.B.nslice * 64, c.B.nparts, c.B.part_h, c.A.s_off, c.A.s_ent, pt, c.B.perm, eri,
+                     x, y);
+}
+
+// Ket row staged in parts: smallest part count (<= 8) whose piece and ERI row fit.
+template <typename T>
+static bool mixed_parts_dispatch(Context &c, const T *eri, const T *x, T *y) {
+  const int nt = (c.B.nslice * 64 + 1023) / 1024;
+  if (nt > 32) return false;
+  // Parts already built (e.g. for the other precision) are kept when they fit.
+  const int first = c.B.nparts >= 2 ? c.B.nparts : 2;
+  for (int P = first; P <= 8; P++) {
+    const size_t h = (c.nb + P - 1) / P;
+    const size_t lds = sizeof(T) * (h + c.K + 1);
+    if (lds > c.lds_limit) continue;
+    build_singles_ell_parts(c, c.B, P);
+    if (nt <= 1) launch_mixed_parts<T, 1>(c, eri, x, y, lds);
+    else if (nt <= 2) launch_mixed_parts<T, 2>(c, eri, x, y, lds);
+    else if (nt <= 4) launch_mixed_parts<T, 4>(c, eri, x, y, lds);
+    else if (nt <= 8) launch_mixed_parts<T, 8>(c, eri, x, y, lds);
+    else if (nt <= 16) launch_mixed_parts<T, 16>(c, eri, x, y, lds);
+    else launch_mixed_parts<T, 32>(c, eri, x, y, lds);
+    return true;
+  }
+  return false;
+}
+
 template <typename T>
 static void sigma_mixed(Context &c, const T *eri, const T *x, T *y) {
   if (c.B.has_ell) {
-    // widest staging that fits: 4 pairs (FP32), 2 pairs, then 1
+    // widest staging that fits: 4 pairs (FP32), 2 pairs, then 1, then row parts
     if (sizeof(T) == 4 && mixed_ell_dispatch<T, 4>(c, eri, x, y)) return check_launch("k_mixed_ell");
     if (mixed_ell_dispatch<T, 2>(c, eri, x, y)) return check_launch("k_mixed_ell");
     if (mixed_ell_dispatch<T, 1>(c, eri, x, y)) return check_launch("k_mixed_ell");
+    if (mixed_parts_dispatch<T>(c, eri, x, y)) return check_launch("k_mixed_parts");
   }
   const int nb = (int)c.nb;
   const size_t lds = sizeof(T) * ((size_t)nb + c.K);

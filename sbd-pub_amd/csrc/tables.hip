@@ -292,6 +292,71 @@ void build_singles_ell(Context &c, SpinTables &t) {
   t.has_ell = true;
 }
 
+// Ket-range parts of the sliced-ELL singles (SpinTables::pell): part p holds
+// the entries with ket in [p*h, (p+1)*h), h = ceil(n/nparts), ket stored as
+// ket - p*h, with the slices and row order of t.perm.
+void build_singles_ell_parts(Context &c, SpinTables &t, int nparts) {
+  if (!t.has_ell || nparts < 2 || t.nparts == nparts) return;
+  const int n = t.n;
+  const int h = (n + nparts - 1) / nparts;
+  std::vector<uint64_t> off(n + 1), ent(t.ns);
+  std::vector<uint32_t> perm((size_t)t.nslice * 64);
+  SBDX_HIP(hipMemcpyAsync(off.data(), t.s_off, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipMemcpyAsync(ent.data(), t.s_ent, sizeof(uint64_t) * t.ns, hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipMemcpyAsync(perm.data(), t.perm, sizeof(uint32_t) * perm.size(), hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  const uint32_t pad = (uint32_t)c.K << 16;
+  t.pell.assign(nparts, nullptr);
+  t.psl_off.assign(nparts, nullptr);
+  t.psl_len.assign(nparts, nullptr);
+  for (int p = 0; p < nparts; p++) {
+    const uint64_t lo = (uint64_t)p * h, hi = std::min<uint64_t>((uint64_t)(p + 1) * h, (uint64_t)n);
+    std::vector<uint32_t> sl_len(t.nslice);
+    std::vector<uint64_t> sl_off(t.nslice);
+    uint64_t total = 0;
+    for (int s = 0; s < t.nslice; s++) {
+      uint32_t mx = 0;
+      for (int l = 0; l < 64; l++) {
+        const uint32_t r = perm[(size_t)s * 64 + l];
+        if (r == 0xffffffffu) continue;
+        uint32_t cnt = 0;
+        for (uint64_t e = off[r]; e < off[r + 1]; e++) {
+          const uint64_t k = ent[e] & 0xffffffffull;
+          cnt += (k >= lo && k < hi);
+        }
+        mx = std::max(mx, cnt);
+      }
+      sl_len[s] = mx;
+      sl_off[s] = total;
+      total += (uint64_t)mx * 64;
+    }
+    std::vector<uint32_t> ell(total + 1, pad);
+    for (int s = 0; s < t.nslice; s++)
+      for (int l = 0; l < 64; l++) {
+        const uint32_t r = perm[(size_t)s * 64 + l];
+        if (r == 0xffffffffu) continue;
+        uint64_t k = 0;
+        for (uint64_t e = off[r]; e < off[r + 1]; e++) {
+          const uint64_t ket = ent[e] & 0xffffffffull;
+          if (ket < lo || ket >= hi) continue;
+          const uint32_t pq = (uint32_t)((ent[e] >> 32) & 0x7fffffffull);
+          ell[sl_off[s] + 64 * k + l] =
+              (uint32_t)(ket - lo) | (pq << 16) | ((ent[e] >> 63) ? 0x80000000u : 0u);
+          k++;
+        }
+      }
+    t.pell[p] = (uint32_t *)dalloc(c, sizeof(uint32_t) * ell.size());
+    t.psl_off[p] = (uint64_t *)dalloc(c, sizeof(uint64_t) * t.nslice);
+    t.psl_len[p] = (uint32_t *)dalloc(c, sizeof(uint32_t) * t.nslice);
+    SBDX_HIP(hipMemcpyAsync(t.pell[p], ell.data(), sizeof(uint32_t) * ell.size(), hipMemcpyHostToDevice, c.stream));
+    SBDX_HIP(hipMemcpyAsync(t.psl_off[p], sl_off.data(), sizeof(uint64_t) * t.nslice, hipMemcpyHostToDevice, c.stream));
+    SBDX_HIP(hipMemcpyAsync(t.psl_len[p], sl_len.data(), sizeof(uint32_t) * t.nslice, hipMemcpyHostToDevice, c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+  }
+  t.nparts = nparts;
+  t.part_h = h;
+}
+
 void build_dress(Context &c, SpinTables &t) {
   t.dress = (double *)dalloc(c, sizeof(double) * (size_t)c.K * t.n);
   ProfScope ps(c, "dress");

@@ -40,6 +40,9 @@ ABI_SYMBOLS = [
     "sbdx_get_vector", "sbdx_trace", "sbdx_energy", "sbdx_rdm", "sbdx_table_count",
     "sbdx_table_export", "sbdx_profile", "sbdx_profile_count", "sbdx_profile_get",
     "sbdx_work_counts", "sbdx_set_precision", "sbdx_sigma_fp32", "sbdx_set_restart",
+    "sbdx_mlp_nparam", "sbdx_mlp_nfeat", "sbdx_mlp_set_weights", "sbdx_mlp_get_weights",
+    "sbdx_mlp_forward", "sbdx_mlp_gradient", "sbdx_mlp_train", "sbdx_mlp_record", "sbdx_mlp_collect",
+    "sbdx_set_preconditioner", "sbdx_precond_stats", "sbdx_set_verbose",
 ]
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -87,6 +90,20 @@ def lib():
     L.sbdx_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_sigma_fp32.argtypes = [C.c_void_p, _dp, _dp]
     L.sbdx_set_restart.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_set_verbose.argtypes = [C.c_void_p, C.c_int]
+    _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    _u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
+    L.sbdx_mlp_set_weights.argtypes = [C.c_void_p, _fp, C.c_size_t]
+    L.sbdx_mlp_get_weights.argtypes = [C.c_void_p, _fp, C.c_size_t]
+    L.sbdx_mlp_forward.argtypes = [C.c_void_p, _u16p, C.c_size_t, _fp]
+    L.sbdx_mlp_gradient.argtypes = [C.c_void_p, _u16p, _fp, C.c_size_t, _fp, C.POINTER(C.c_double)]
+    L.sbdx_mlp_train.argtypes = [C.c_void_p, _u16p, _fp, C.c_size_t, C.c_int, C.c_int, C.c_float,
+                                 C.c_uint64, _dp]
+    L.sbdx_mlp_record.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_mlp_collect.argtypes = [C.c_void_p, C.c_size_t, C.c_uint64, _u16p, _fp,
+                                   C.POINTER(C.c_size_t)]
+    L.sbdx_set_preconditioner.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_precond_stats.argtypes = [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]
     _LIB = L
     return L
 
@@ -160,10 +177,67 @@ class TPBOperator:
         """PREC_FP64 (reference algorithm) or PREC_MIXED for later solves."""
         _check(lib().sbdx_set_precision(self.h, int(mode)))
 
+    # ---------------------------------------------------------- learned preconditioner
+    def mlp_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(lib().sbdx_mlp_set_weights(self.h, w, w.size))
+
+    def mlp_get_weights(self):
+        w = np.zeros(lib().sbdx_mlp_nparam(), dtype=np.float32)
+        _check(lib().sbdx_mlp_get_weights(self.h, w, w.size))
+        return w
+
+    def mlp_forward(self, feat):
+        feat = np.ascontiguousarray(feat, dtype=np.uint16)
+        n = feat.shape[0]
+        out = np.zeros(n, dtype=np.float32)
+        _check(lib().sbdx_mlp_forward(self.h, feat.reshape(-1), n, out))
+        return out
+
+    def mlp_gradient(self, feat, target):
+        feat = np.ascontiguousarray(feat, dtype=np.uint16)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.zeros(lib().sbdx_mlp_nparam(), dtype=np.float32)
+        sse = C.c_double()
+        _check(lib().sbdx_mlp_gradient(self.h, feat.reshape(-1), target, target.size, g, C.byref(sse)))
+        return g, sse.value
+
+    def mlp_train(self, feat, target, epochs=10, batch=4096, lr=1e-3, seed=0):
+        feat = np.ascontiguousarray(feat, dtype=np.uint16)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        loss = np.zeros(epochs)
+        _check(lib().sbdx_mlp_train(self.h, feat.reshape(-1), target, target.size, epochs, batch,
+                                    lr, seed, loss))
+        return loss
+
+    def mlp_record(self, cap):
+        _check(lib().sbdx_mlp_record(self.h, int(cap)))
+
+    def mlp_collect(self, max_pairs, seed=0):
+        nf = lib().sbdx_mlp_nfeat()
+        feat = np.zeros(max_pairs * nf, dtype=np.uint16)
+        tgt = np.zeros(max_pairs, dtype=np.float32)
+        n = C.c_size_t()
+        _check(lib().sbdx_mlp_collect(self.h, max_pairs, seed, feat, tgt, C.byref(n)))
+        return feat[: n.value * nf].reshape(n.value, nf), tgt[: n.value]
+
+    def set_preconditioner(self, mode):
+        """0: diagonal R/(E0-d); 1: learned MLP with diagonal fallback on stall."""
+        _check(lib().sbdx_set_preconditioner(self.h, int(mode)))
+
+    def precond_stats(self):
+        a, b = C.c_long(), C.c_long()
+        _check(lib().sbdx_precond_stats(self.h, C.byref(a), C.byref(b)))
+        return dict(mlp=a.value, diag=b.value)
+
     def set_restart(self, keep):
         """0: restart from W alone (reference); k >= 1: thick restart keeping
         k Ritz vectors, exact FP64 residual at every restart."""
         _check(lib().sbdx_set_restart(self.h, int(keep)))
+
+    def set_verbose(self, level):
+        """Nonzero: one stderr line per Davidson step (E0, |r|)."""
+        _check(lib().sbdx_set_verbose(self.h, int(level)))
 
     def configure(self, solver):
         """'plain' (reference algorithm, FP64), 'thick' (FP64 thick restart)

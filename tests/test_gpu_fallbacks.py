@@ -1,0 +1,37 @@
+"""The mixed-term kernel variants for rows that do not fit in LDS (row
+parts, then the global-memory gather), forced on a small instance by
+lowering the LDS budget with SBDX_LDS_LIMIT, against the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import molecule, random_vector
+
+pytestmark = pytest.mark.gpu
+
+
+# h2o-1em3: 275 strings, K = 576.  FP64 one-pair staging needs 6816 bytes:
+# 6000 forces the two-part kernel, 4000 the global gather.
+@pytest.mark.parametrize("limit", [6000, 4000])
+def test_mixed_fallbacks_match_oracle(limit, oracle_mod):
+    import sbdx
+    m = molecule("h2o", "1em3")
+    old = os.environ.get("SBDX_LDS_LIMIT")
+    os.environ["SBDX_LDS_LIMIT"] = str(limit)
+    print(f"[fallback] LDS limit {limit}", flush=True)
+    try:
+        op = sbdx.TPBOperator.from_instance(m)
+    finally:
+        if old is None:
+            del os.environ["SBDX_LDS_LIMIT"]
+        else:
+            os.environ["SBDX_LDS_LIMIT"] = old
+    orc = oracle_mod.Oracle.from_instance(m)
+    v = random_vector(op.ndet, 31)
+    s = op.sigma(v)
+    r = orc.sigma(v)
+    assert np.max(np.abs(s - r)) < 1e-12 * np.max(np.abs(r))
+    s32 = op.sigma_fp32(v)
+    assert np.linalg.norm(s32 - r) / np.linalg.norm(r) < 2e-6
+    op.close()

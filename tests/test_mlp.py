@@ -1,0 +1,110 @@
+"""Learned preconditioner: the numpy reference MLP is pinned by finite
+differences and the GELU derivative by numerical differentiation (CPU);
+the GPU forward, gradient, training and preconditioned solve are checked
+against it and against the plain solver (GPU)."""
+import numpy as np
+import pytest
+
+import mlp_reference as M
+from conftest import molecule, random_vector
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((n, M.DIN), dtype=np.float32)
+    X[:, :128] = rng.integers(0, 2, size=(n, 128))
+    X[:, 128:133] = rng.normal(0, 1, size=(n, 5))
+    X[:, 133] = 1.0
+    t = (np.tanh(X[:, 130]) + 0.3 * X[:, 128] - 0.2 * X[:, 5]).astype(np.float32)
+    return X, t
+
+
+def test_gelu_derivative():
+    z = np.linspace(-6, 6, 97)
+    h = 1e-6
+    num = (M.gelu(z + h) - M.gelu(z - h)) / (2 * h)
+    assert np.max(np.abs(num - M.gelu_d(z))) < 1e-8
+    assert abs(M.gelu(0.0)) == 0.0 and abs(M.gelu(10.0) - 10.0) < 1e-12
+
+
+def test_reference_gradient_finite_differences():
+    X, t = _data(16, 1)
+    p = M.init_params(3).astype(np.float64)
+    _, g = M.loss_and_grad(p, X, t, rb=False)
+    rng = np.random.default_rng(0)
+    for idx in list(rng.integers(0, M.NPARAM - 4, size=40)) + [M.B3, M.W3 + 5, M.B1 + 7, M.B2 + 1]:
+        e = np.zeros_like(p)
+        e[idx] = 1e-6
+        lp, _ = M.loss_and_grad(p + e, X, t, rb=False)
+        lm, _ = M.loss_and_grad(p - e, X, t, rb=False)
+        num = (lp - lm) / 2e-6
+        assert abs(num - g[idx]) < 1e-6 * max(1.0, abs(g[idx])), (idx, num, g[idx])
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.00390625, 1.005859375, -3.14159, 0.0], dtype=np.float32)
+    r = M.bf16(x)
+    assert r[0] == 1.0 and r[1] == 1.0  # tie to even
+    assert r[2] == 1.0078125
+    assert np.all(np.abs(r - x) <= np.abs(x) * 2 ** -8)
+
+
+@pytest.mark.gpu
+class TestGPU:
+    @pytest.fixture(scope="class")
+    def op(self):
+        import sbdx
+        m = molecule("h2o", "1em3")
+        print("[mlp] h2o-1em3 operator", flush=True)
+        o = sbdx.TPBOperator.from_instance(m)
+        yield o
+        o.close()
+
+    def test_forward_matches_reference(self, op):
+        X, _ = _data(200, 2)
+        p = M.init_params(5)
+        op.mlp_set_weights(p)
+        y = op.mlp_forward(M.bf16_bits(X))
+        yr, _ = M.forward(p, X)
+        assert np.max(np.abs(y - yr)) < 1e-3 * max(1.0, np.max(np.abs(yr)))
+
+    def test_gradient_matches_reference(self, op):
+        X, t = _data(300, 4)
+        p = M.init_params(7)
+        op.mlp_set_weights(p)
+        g, sse = op.mlp_gradient(M.bf16_bits(X), t)
+        loss, gr = M.loss_and_grad(p, X, t)
+        assert abs(sse / 300 - loss) < 1e-3 * max(loss, 1e-6)
+        for name, a, b in [("W1", M.W1, M.B1), ("b1", M.B1, M.W2), ("W2", M.W2, M.B2),
+                           ("b2", M.B2, M.W3), ("w3", M.W3, M.B3), ("b3", M.B3, M.B3 + 1)]:
+            err = np.linalg.norm(g[a:b] - gr[a:b]) / max(np.linalg.norm(gr[a:b]), 1e-12)
+            assert err < 2e-3, (name, err)
+
+    def test_training_reduces_loss(self, op):
+        X, t = _data(8192, 8)
+        op.mlp_set_weights(M.init_params(9))
+        loss = op.mlp_train(M.bf16_bits(X), t, epochs=12, batch=512, lr=2e-3, seed=1)
+        assert loss[-1] < 0.2 * loss[0], loss
+
+    def test_collect_and_preconditioned_solve(self, op):
+        import sbdx
+        W0 = sbdx.basis_init_vector(op.ndet)
+        op.configure("thick")
+        op.mlp_record(64)
+        op.set_vector(W0)
+        E_ref, n_ref, ok = op.solve(max_it=100, nb=12, eps=1e-8)
+        assert ok
+        feat, tgt = op.mlp_collect(60000, seed=3)
+        op.mlp_record(0)
+        assert feat.shape[0] > 1000 and np.all(np.isfinite(tgt))
+        op.mlp_set_weights(M.init_params(11))
+        loss = op.mlp_train(feat, tgt, epochs=4, batch=1024, lr=1e-3, seed=2)
+        assert np.all(np.isfinite(loss))
+        op.set_preconditioner(1)
+        op.set_vector(W0)
+        E, n, ok = op.solve(max_it=100, nb=12, eps=1e-8)
+        st = op.precond_stats()
+        op.set_preconditioner(0)
+        op.configure("plain")
+        assert ok and abs(E - E_ref) < 1e-9, (E, E_ref)
+        assert st["mlp"] > 0
