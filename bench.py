@@ -26,6 +26,7 @@ for _p in (ROOT, os.path.join(ROOT, "sbd-pub_amd")):
 
 METRIC = "time-to-E0 (|ΔE|<1e-8 Ha) & sigma GB/s: mixed+learned vs plain FP64, 1 GPU"
 HBM_PEAK_GBPS = 8000.0
+MFMA_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}  # MI355X dense matrix peaks
 
 
 def log(msg):
@@ -34,19 +35,23 @@ def log(msg):
 
 def kernel_bytes(wc, na, nb, e=8.0):
     """Algorithmic bytes per launch of each sigma kernel group, element size
-    e (8 FP64, 4 FP32).  same-spin: every (entry x other-spin column) reads
-    x[J,m] (+ the dressing dress[pq,m] for singles), plus diagonal, hii and
-    output streams, plus the table entries once; mixed (direct form): one x
-    gather per (alpha single x beta single) term plus reading and writing
-    sigma, plus the tables once; transposes: one read + one write."""
+    e (8 FP64, 4 FP32).  same-spin (sparse form): every (entry x other-spin
+    column) reads x[J,m], plus diagonal, hii and output streams, plus the
+    table entries once; mixed (direct form): one x gather per (alpha single
+    x beta single) term, per (alpha single x beta string) and (alpha string
+    x beta single) dressing term, plus reading and writing sigma, plus the
+    tables once; transposes: one read + one write; the dense same-spin form
+    streams x, hii and y for the diagonal and reads x and y once per GEMM."""
     N = na * nb
     ab = wc["alpha_singles"] * wc["beta_singles"]
+    dressing = wc["alpha_singles"] * nb + wc["beta_singles"] * na
     return {
-        "same_alpha": e * (nb * (2 * wc["alpha_singles"] + wc["alpha_doubles"]) + 3 * N)
+        "same_alpha": e * (nb * (wc["alpha_singles"] + wc["alpha_doubles"]) + 3 * N)
         + (8.0 + e) * wc["alpha_singles"] + (4.0 + e) * wc["alpha_doubles"],
-        "same_beta": e * (na * (2 * wc["beta_singles"] + wc["beta_doubles"]) + 2 * N)
+        "same_beta": e * (na * (wc["beta_singles"] + wc["beta_doubles"]) + 2 * N)
         + (8.0 + e) * wc["beta_singles"] + (4.0 + e) * wc["beta_doubles"],
-        "mixed": e * (ab + 2 * N) + 8.0 * wc["alpha_singles"] + 4.0 * wc["beta_singles"],
+        "mixed": e * (ab + dressing + 2 * N) + 8.0 * wc["alpha_singles"] + 4.0 * wc["beta_singles"],
+        "same_gemm": e * (3 * N + 2 * 2 * N + na * na + nb * nb),
         "transpose": 2 * e * N,
         "transpose_add": 3 * e * N,
     }
@@ -124,9 +129,13 @@ def main():
     else:
         W0 = np.random.default_rng(1).standard_normal(op.ndet)
         W0 /= np.linalg.norm(W0)
+    # warmup: also settles the same-spin form (dense GEMM or sparse rows)
+    # for each precision before any timed solve
     for _ in range(args.warmup):
         op.sigma(W0)
-    log("warmup done")
+        if args.solver == "mixed":
+            op.sigma_fp32(W0)
+    log(f"warmup done; same-spin form {op.same_spin_forms()}")
 
     def run(solver, nsolve):
         op.configure(solver)
@@ -160,6 +169,8 @@ def main():
     nsteps = [res["steps"]]
     conv = [res["converged"]]
 
+    gemm_flops = 2.0 * na * na * nb + 2.0 * nb * nb * na  # dense same-spin, per sigma
+
     def kernel_table(stats):
         kb64 = kernel_bytes(wc, na, nb, 8.0)
         kb32 = kernel_bytes(wc, na, nb, 4.0)
@@ -172,12 +183,17 @@ def main():
             if base in kb and avg > 0:
                 ent["alg_bytes"] = kb[base]
                 ent["GBps"] = round(kb[base] / (avg * 1e-3) / 1e9, 1)
+            if base == "same_gemm" and avg > 0:
+                ent["flops"] = gemm_flops
+                ent["TFLOPs"] = round(gemm_flops / (avg * 1e-3) / 1e12, 2)
             out[name] = ent
         return out
 
+    SIGMA_PARTS = ("same_gemm", "same_alpha", "transpose", "same_beta", "transpose_add", "mixed")
+
     def sigma_rate(kernels, suffix):
-        names = [n + suffix for n in ("same_alpha", "transpose", "same_beta", "transpose_add", "mixed")]
-        if not all(n in kernels for n in names):
+        names = [n + suffix for n in SIGMA_PARTS if n + suffix in kernels]
+        if not any(n.startswith("mixed") for n in names):
             return None, None
         ms = sum(kernels[n]["avg_ms"] for n in names)
         by = sum(kernels[n]["alg_bytes"] for n in names)
@@ -187,8 +203,13 @@ def main():
     sig_kernels = [n for n in kernels if "alg_bytes" in kernels[n]]
     dom = max(sig_kernels, key=lambda n: kernels[n]["total_ms"])
     dk = kernels[dom]
-    roofline = dict(kernel=dom, bound="hbm", achieved=dk["GBps"], peak=HBM_PEAK_GBPS, unit="GB/s",
-                    frac=round(dk["GBps"] / HBM_PEAK_GBPS, 4), traffic=None)
+    if dom.startswith("same_gemm"):
+        peak = MFMA_PEAK_TFLOPS["f32" if dom.endswith("_f32") else "f64"]
+        roofline = dict(kernel=dom, bound="mfma", achieved=dk["TFLOPs"], peak=peak, unit="TFLOP/s",
+                        frac=round(dk["TFLOPs"] / peak, 4), traffic=None)
+    else:
+        roofline = dict(kernel=dom, bound="hbm", achieved=dk["GBps"], peak=HBM_PEAK_GBPS,
+                        unit="GB/s", frac=round(dk["GBps"] / HBM_PEAK_GBPS, 4), traffic=None)
     s64_ms, s64_gbps = sigma_rate(kernels, "")
     s32_ms, s32_gbps = sigma_rate(kernels, "_f32")
     if plain is not None:
@@ -231,6 +252,7 @@ def main():
         "sigma_fp32_GBps": s32_gbps,
         "roofline": roofline,
         "kernels": kernels,
+        "same_spin_form": op.same_spin_forms(),
         "build_s": round(t_build, 3),
     }
     if not args.no_cpu_baseline:

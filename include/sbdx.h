@@ -117,6 +117,15 @@ int sbdx_set_precision(sbdx_handle_t h, int mode);
  *               combination).  max_it then bounds max_it*nb products. */
 int sbdx_set_restart(sbdx_handle_t h, int keep);
 
+/* Form of the same-spin terms chosen for precision 0 (FP64) or 1 (FP32) (no
+ * counterpart in the reference): -1 not chosen yet (no sigma of that
+ * precision so far), 0 sparse row kernels, 1 dense GEMMs over the same-spin
+ * blocks.  The choice is made on the first sigma of each precision by timing
+ * both forms; ms_* receive those timings (0 when a form was not timed, e.g.
+ * when SBDX_SAME_SPIN=sparse|dense in the environment fixed it). */
+int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_sparse,
+                        float *ms_dense);
+
 /* verbose != 0: the solves print one line per step to stderr, step index,
  * E0 and |r| (as davidson.h:839-851 prints per step); thick-restart solves
  * mark the exact FP64 residual of each restart.  0 (default) prints nothing. */

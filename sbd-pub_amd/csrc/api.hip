@@ -169,14 +169,12 @@ static Context *create(int L, size_t bl, const uint64_t *aw, size_t na, const ui
     const auto as = unpack_words(aw, na, bl, L);
     const auto bs = unpack_words(bw, nb, bl, L);
     build_spin_tables(c, c.A, as);
-    build_dress(c, c.A);
     build_singles_ell(c, c.A);
     if (as == bs) {
       c.B = c.A;
       c.B.alias = true;
     } else {
       build_spin_tables(c, c.B, bs);
-      build_dress(c, c.B);
       build_singles_ell(c, c.B);
     }
     c.hii = (double *)dalloc(c, sizeof(double) * c.N);
@@ -203,6 +201,7 @@ static void destroy(Context *c) {
   if (!c) return;
   (void)hipStreamSynchronize(c->stream);
   prof_clear(*c);
+  same_dense_release(*c);
   dfree_all(*c);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -378,6 +377,18 @@ int sbdx_set_restart(sbdx_handle_t h, int keep) {
   Context &c = ctx_of(h);
   if (keep < 0 || keep > 8) throw Error(SBDX_ERR_ARG, "keep must be in [0, 8]");
   c.restart_keep = keep;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_sparse,
+                        float *ms_dense) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (precision != 0 && precision != 1) throw Error(SBDX_ERR_ARG, "precision must be 0 or 1");
+  if (form) *form = c.same_mode[precision];
+  if (ms_sparse) *ms_sparse = c.same_ms[precision][0];
+  if (ms_dense) *ms_dense = c.same_ms[precision][1];
   return SBDX_OK;
   SBDX_GUARD_END
 }

@@ -49,7 +49,6 @@ struct SpinTables {
   uint32_t *d_ket = nullptr;   // [nd]
   uint32_t *d_orb = nullptr;   // [nd]
   double *d_val = nullptr;     // [nd]
-  double *dress = nullptr;     // [K][n]: dress[pq*n + x] = sum_{k in str[x]} (pq|kk)
   double *e1 = nullptr;        // [n]: sum_p h_pp + sum_{p<q}((pp|qq)-(pq|qp)) of the string
   double *jsum = nullptr;      // [n][L]: sum_{p in str} (pp|rr)
   size_t ns = 0, nd = 0;
@@ -84,7 +83,7 @@ struct ProfSlot {
 struct F32Operands {
   bool ready = false;
   float *s_val_a = nullptr, *d_val_a = nullptr, *s_val_b = nullptr, *d_val_b = nullptr;
-  float *dress_a = nullptr, *dress_b = nullptr, *eri = nullptr, *hii = nullptr;
+  float *eri = nullptr, *hii = nullptr;
   float *x = nullptr, *y = nullptr, *T1 = nullptr, *T2 = nullptr;
 };
 
@@ -113,6 +112,16 @@ struct Context {
   SpinTables A, B;
   double *hii = nullptr;   // [N]
   F32Operands f32;
+  // Same-spin blocks as dense symmetric matrices for the GEMM form of the
+  // same-spin terms (row-major n x n; the beta block aliases the alpha one
+  // when the string sets are equal), FP64 and FP32, built on first use.
+  double *Hd_a = nullptr, *Hd_b = nullptr;
+  float *Hf_a = nullptr, *Hf_b = nullptr;
+  void *blas = nullptr;  // rocblas_handle bound to `stream`
+  // Same-spin form per precision [0 FP64, 1 FP32]: -1 not chosen yet,
+  // 0 sparse row kernels, 1 dense GEMM; the timings that decided it (ms).
+  int same_mode[2] = {-1, -1};
+  float same_ms[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
   int precision = 0;       // 0: FP64 throughout; 1: FP32 sigma on corrections, FP64 restarts
   int verbose = 0;         // 1: one stderr line per Davidson step (E0, |r|)
   int restart_keep = 0;    // 0: restart from W alone (reference); k: thick restart keeping k Ritz vectors
@@ -176,7 +185,6 @@ void check_launch(const char *what);
 void build_spin_tables(Context &c, SpinTables &t, const std::vector<uint64_t> &strs);
 void build_singles_ell(Context &c, SpinTables &t);
 void build_singles_ell_parts(Context &c, SpinTables &t, int nparts);
-void build_dress(Context &c, SpinTables &t);
 void compute_diag(Context &c);
 void sign_convert(Context &c, const double *in, double *out);      // out = S*in
 void sign_convert_add(Context &c, const double *in, double *out);  // out += S*in
@@ -185,6 +193,7 @@ void sign_convert_add(Context &c, const double *in, double *out);  // out += S*i
 void sigma_alpha_first(Context &c, const double *x, double *y);  // y = H' x (overwrite)
 void sigma_alpha_first_f32(Context &c, const double *x, double *y);  // same, FP32 arithmetic
 void ensure_f32(Context &c);
+void same_dense_release(Context &c);  // frees the rocBLAS handle
 void transpose_into(Context &c, const double *in, double *out, int R, int Cc);  // out = in^T
 
 // davidson.hip

@@ -6,14 +6,24 @@
 // other-spin index) with double atomics.  Here the three terms are split by
 // spin so every product is a coalesced row operation and no atomics are
 // needed:
-//   same-spin alpha   y[I,:]  = hii*x + sum_J a_IJ(:) x[J,:]       rows of x
-//   same-spin beta    on the transposed vector, same kernel          rows of x^T
+//   same-spin         y = hii*x + H_a X + X H_b with X the na x nb vector:
+//                     either two dense GEMMs (the same-spin blocks H_a, H_b
+//                     are symmetric and, at selected-CI sizes, ~5-15% full
+//                     with no empty tiles) or sparse row kernels (x and its
+//                     transpose); the faster is picked on first use
 //   mixed alpha-beta  y[I,:] += sum_{J in S_a(I)} s_a V_pq * x[J,:] gathered
 //                     through the beta-singles table, with x[J,:] and the
 //                     ERI row V_pq staged in LDS.
-// An alpha single's coefficient depends on the beta string only through
-// sum_{k in beta} (pq|kk); that "dressing" is a precomputed [pq][ib] table,
-// read coalesced.
+// A single's matrix element depends on the other spin's string through
+// sum_{k in other} (pq|kk) (the "dressing"); that part is evaluated in the
+// mixed kernels from the staged ERI row's diagonal (alpha singles) and from
+// one extra staged row per bra, sum_{k in alpha(I)} (kk|..), with x[I,:]
+// (beta singles), so the same-spin blocks are plain matrices.
+#include <rocblas/rocblas.h>
+
+#include <cstdlib>
+#include <string>
+
 #include "common.h"
 
 namespace sbdx {
@@ -25,9 +35,36 @@ __device__ __forceinline__ uint32_t ent_ket(uint64_t e) { return (uint32_t)(e & 
 __device__ __forceinline__ uint32_t ent_pq(uint64_t e) { return (uint32_t)((e >> 32) & 0x7fffffffull); }
 __device__ __forceinline__ bool ent_neg(uint64_t e) { return (e >> 63) != 0; }
 
+// sum over the occupied orbitals k of `occ` of v[k*(L+1)], the (kk) entries
+// of an ERI row; every string of a spin has the same count, so the trip
+// count is uniform across a wavefront.
+template <typename T>
+__device__ __forceinline__ T diag_sum(const T *v, uint64_t occ, int L1) {
+  T s = T(0);
+  while (occ) {
+    s += v[__builtin_ctzll(occ) * L1];
+    occ &= occ - 1;
+  }
+  return s;
+}
+
+// The spectator row of bra I: w[i] = sum_{k in alpha(I)} (kk|i) for the
+// off-diagonal ERI indices i (zero on the (rr) diagonal, which no beta single
+// addresses), so the dressing kernels pick up beta singles' alpha dressing.
+template <typename T>
+__device__ __forceinline__ T spectator_val(const T *__restrict__ eri, uint64_t occ, int L1, int K,
+                                           int i) {
+  if (i % L1 == 0) return T(0);
+  T s = T(0);
+  while (occ) {
+    s += eri[(size_t)(__builtin_ctzll(occ) * L1) * K + i];
+    occ &= occ - 1;
+  }
+  return s;
+}
+
 // Same-spin rows: for bra row r of this spin and every other-spin column m,
-//   y[r*M+m] = (DIAG ? hii*x : 0) + sum_singles (v + s*dress[pq][m]) x[J*M+m]
-//                                   + sum_doubles v x[J*M+m]
+//   y[r*M+m] = (DIAG ? hii*x : 0) + sum_singles v x[J*M+m] + sum_doubles v x[J*M+m]
 // One wavefront per (row, 64-column chunk), four rows per block.  Workgroups
 // are dealt to the 8 XCDs round-robin, so block b runs on XCD b%8; the
 // mapping gives XCD g the chunks g, g+8, ... and sweeps every row of one chunk
@@ -41,7 +78,6 @@ __global__ void __launch_bounds__(256) k_same_xcd(int nrows, int M, int nch, int
                                                   const uint64_t *__restrict__ d_off,
                                                   const uint32_t *__restrict__ d_ket,
                                                   const T *__restrict__ d_val,
-                                                  const T *__restrict__ dress,
                                                   const T *__restrict__ hii,
                                                   const T *__restrict__ x, T *__restrict__ y) {
   const int g = blockIdx.x & 7;
@@ -57,12 +93,7 @@ __global__ void __launch_bounds__(256) k_same_xcd(int nrows, int M, int nch, int
   if (DIAG) acc = hii[r * Ms + mm] * x[r * Ms + mm];
   const uint64_t s0 = s_off[r], s1 = s_off[r + 1];
 #pragma unroll 4
-  for (uint64_t e = s0; e < s1; e++) {
-    const uint64_t ent = s_ent[e];
-    const T dr = dress[(size_t)ent_pq(ent) * Ms + mm];
-    const T coef = s_val[e] + (ent_neg(ent) ? -dr : dr);
-    acc += coef * x[(size_t)ent_ket(ent) * Ms + mm];
-  }
+  for (uint64_t e = s0; e < s1; e++) acc += s_val[e] * x[(size_t)ent_ket(s_ent[e]) * Ms + mm];
   const uint64_t d0 = d_off[r], d1 = d_off[r + 1];
 #pragma unroll 8
   for (uint64_t e = d0; e < d1; e++) acc += d_val[e] * x[(size_t)d_ket[e] * Ms + mm];
@@ -92,16 +123,22 @@ __global__ void __launch_bounds__(256) k_transpose(const T *__restrict__ in, T *
 }
 
 // Mixed alpha-beta term, beta singles in sliced-ELL form (SpinTables::ell).
-// One block per alpha bra row I.  Each LDS round stages EB alpha singles of
-// I: their ket rows x[J,:] and sign-folded ERI rows s_a*(pq|..), with a zero
-// at index K for padding entries.  Thread slot = tid + t*1024 walks beta row
+// One block per alpha bra row I.  Each LDS round stages EB "pairs": for an
+// alpha single of I its ket row x[J,:] and sign-folded ERI row s_a*(pq|..);
+// after the singles, one spectator pair x[I,:] with the row
+// sum_{k in alpha(I)} (kk|..) (spectator_val).  Index K of each ERI row is a
+// zero for padding entries.  Thread slot = tid + t*1024 walks beta row
 // perm[slot]; slices are 64 rows of similar length, so the trip count is
 // uniform per wavefront and each entry load is a coalesced 256-byte access
-// shared by the EB staged pairs.
+// shared by the EB staged pairs.  Per pair the slot also adds the alpha
+// single's dressing x[J,r] * sum_{k in beta(r)} s_a (pq|kk) from the staged
+// ERI row's diagonal (zero in the spectator row).
 template <typename T, int NT, int EB>
-__global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int nslots,
+__global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int nslots,
                                                     const uint64_t *__restrict__ a_off,
                                                     const uint64_t *__restrict__ a_ent,
+                                                    const uint64_t *__restrict__ a_str,
+                                                    const uint64_t *__restrict__ b_str,
                                                     const uint32_t *__restrict__ ell,
                                                     const uint32_t *__restrict__ perm,
                                                     const uint64_t *__restrict__ sl_off,
@@ -111,12 +148,22 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int nslots,
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
   const int ldr = nb + K + 1;
+  const int L1 = L + 1;
   const int I = blockIdx.x;
   const int tid = threadIdx.x;
   T acc[NT];
+  uint64_t occ[NT];
+  int rr[NT];
 #pragma unroll
-  for (int t = 0; t < NT; t++) acc[t] = T(0);
-  const uint64_t e0 = a_off[I], e1 = a_off[I + 1];
+  for (int t = 0; t < NT; t++) {
+    acc[t] = T(0);
+    const int slot = tid + t * 1024;
+    const uint32_t r = slot < nslots ? perm[slot] : 0xffffffffu;
+    rr[t] = r != 0xffffffffu ? (int)r : 0;
+    occ[t] = r != 0xffffffffu ? b_str[r] : 0ull;  // padding slots: no dressing
+  }
+  const uint64_t occ_a = a_str[I];
+  const uint64_t e0 = a_off[I], e1 = a_off[I + 1] + 1;  // + the spectator pair
   for (uint64_t e = e0; e < e1; e += EB) {
     const int ne = (int)((e1 - e) < (uint64_t)EB ? (e1 - e) : (uint64_t)EB);
     __syncthreads();
@@ -124,13 +171,17 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int nslots,
     for (int k = 0; k < EB; k++) {
       T *crow = lds + k * ldr;
       T *vrow = crow + nb;
-      if (k < ne) {
+      if (k < ne && e + k + 1 < e1) {
         const uint64_t ent = a_ent[e + k];
         const size_t J = ent_ket(ent);
         const T sa = ent_neg(ent) ? T(-1) : T(1);
         const T *vsrc = eri + (size_t)ent_pq(ent) * K;
         for (int i = tid; i < nb; i += 1024) crow[i] = x[J * nb + i];
         for (int i = tid; i < K; i += 1024) vrow[i] = sa * vsrc[i];
+        if (tid == 0) vrow[K] = T(0);
+      } else if (k < ne) {
+        for (int i = tid; i < nb; i += 1024) crow[i] = x[(size_t)I * nb + i];
+        for (int i = tid; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
         if (tid == 0) vrow[K] = T(0);
       } else {
         for (int i = tid; i < ldr; i += 1024) crow[i] = T(0);
@@ -155,6 +206,9 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int nslots,
           for (int q = 1; q < EB; q++) v += lds[q * ldr + nb + rs] * lds[q * ldr + jb];
           a += (w >> 31) ? -v : v;
         }
+#pragma unroll
+        for (int q = 0; q < EB; q++)
+          a += lds[q * ldr + rr[t]] * diag_sum(lds + q * ldr + nb, occ[t], L1);
         acc[t] += a;
       }
     }
@@ -172,7 +226,8 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int nslots,
 // Mixed term when one ket row does not fit in LDS: the row is staged in
 // nparts pieces of h kets and each piece is walked with its own sliced-ELL
 // table (SpinTables::pell, kets relative to the piece); the sign-folded ERI
-// row stays staged for all pieces.
+// row (or the spectator row, last) stays staged for all pieces, and the
+// dressing of beta row r is added with the piece that holds ket r.
 struct PartTables {
   const uint32_t *ell[8];
   const uint64_t *off[8];
@@ -180,27 +235,46 @@ struct PartTables {
 };
 
 template <typename T, int NT>
-__global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int nslots, int nparts, int h,
-                                                      const uint64_t *__restrict__ a_off,
+__global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int L, int nslots, int nparts,
+                                                      int h, const uint64_t *__restrict__ a_off,
                                                       const uint64_t *__restrict__ a_ent,
+                                                      const uint64_t *__restrict__ a_str,
+                                                      const uint64_t *__restrict__ b_str,
                                                       PartTables pt, const uint32_t *__restrict__ perm,
                                                       const T *__restrict__ eri,
                                                       const T *__restrict__ x, T *__restrict__ y) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *crow = reinterpret_cast<T *>(lds_raw);
   T *vrow = crow + h;
+  const int L1 = L + 1;
   const int I = blockIdx.x;
   const int tid = threadIdx.x;
   T acc[NT];
+  uint64_t occ[NT];
+  int rr[NT];
 #pragma unroll
-  for (int t = 0; t < NT; t++) acc[t] = T(0);
-  for (uint64_t e = a_off[I]; e < a_off[I + 1]; e++) {
-    const uint64_t ent = a_ent[e];
-    const size_t J = ent_ket(ent);
-    const T sa = ent_neg(ent) ? T(-1) : T(1);
-    const T *vsrc = eri + (size_t)ent_pq(ent) * K;
+  for (int t = 0; t < NT; t++) {
+    acc[t] = T(0);
+    const int slot = tid + t * 1024;
+    const uint32_t r = slot < nslots ? perm[slot] : 0xffffffffu;
+    rr[t] = r != 0xffffffffu ? (int)r : 0;
+    occ[t] = r != 0xffffffffu ? b_str[r] : 0ull;
+  }
+  const uint64_t occ_a = a_str[I];
+  const uint64_t e1 = a_off[I + 1];
+  for (uint64_t e = a_off[I]; e <= e1; e++) {
+    const bool spect = e == e1;
+    size_t J = I;
     __syncthreads();
-    for (int i = tid; i < K; i += 1024) vrow[i] = sa * vsrc[i];
+    if (!spect) {
+      const uint64_t ent = a_ent[e];
+      J = ent_ket(ent);
+      const T sa = ent_neg(ent) ? T(-1) : T(1);
+      const T *vsrc = eri + (size_t)ent_pq(ent) * K;
+      for (int i = tid; i < K; i += 1024) vrow[i] = sa * vsrc[i];
+    } else {
+      for (int i = tid; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
+    }
     if (tid == 0) vrow[K] = T(0);
     for (int p = 0; p < nparts; p++) {
       const int lo = p * h;
@@ -225,6 +299,8 @@ __global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int nslots,
             const T v = vrow[(w >> 16) & 0x7fffu] * crow[w & 0xffffu];
             a += (w >> 31) ? -v : v;
           }
+          if ((unsigned)(rr[t] - lo) < (unsigned)len_p)
+            a += crow[rr[t] - lo] * diag_sum(vrow, occ[t], L1);
           acc[t] += a;
         }
       }
@@ -241,10 +317,14 @@ __global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int nslots,
 }
 
 // Mixed term with the beta singles in CSR form (used when the packed
-// sliced-ELL table cannot hold the indices): x[J,:] and V row in LDS.
+// sliced-ELL table cannot hold the indices): x[J,:] and V row in LDS, then
+// the spectator pair; dressing as in k_mixed_ell.
 template <typename T, int NT>
-__global__ void __launch_bounds__(1024) k_mixed_lds(int nb, int K, const uint64_t *__restrict__ a_off,
+__global__ void __launch_bounds__(1024) k_mixed_lds(int nb, int K, int L,
+                                                    const uint64_t *__restrict__ a_off,
                                                     const uint64_t *__restrict__ a_ent,
+                                                    const uint64_t *__restrict__ a_str,
+                                                    const uint64_t *__restrict__ b_str,
                                                     const uint64_t *__restrict__ b_off,
                                                     const uint64_t *__restrict__ b_ent,
                                                     const T *__restrict__ eri,
@@ -252,25 +332,33 @@ __global__ void __launch_bounds__(1024) k_mixed_lds(int nb, int K, const uint64_
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *crow = reinterpret_cast<T *>(lds_raw);
   T *vrow = crow + nb;
+  const int L1 = L + 1;
   const int I = blockIdx.x;
   const int tid = threadIdx.x;
   T acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) acc[t] = T(0);
-  for (uint64_t e = a_off[I]; e < a_off[I + 1]; e++) {
-    const uint64_t ent = a_ent[e];
-    const size_t J = ent_ket(ent);
-    const T sa = ent_neg(ent) ? T(-1) : T(1);
-    const T *vsrc = eri + (size_t)ent_pq(ent) * K;
+  const uint64_t occ_a = a_str[I];
+  const uint64_t e1 = a_off[I + 1];
+  for (uint64_t e = a_off[I]; e <= e1; e++) {
+    size_t J = I;
     __syncthreads();
+    if (e < e1) {
+      const uint64_t ent = a_ent[e];
+      J = ent_ket(ent);
+      const T sa = ent_neg(ent) ? T(-1) : T(1);
+      const T *vsrc = eri + (size_t)ent_pq(ent) * K;
+      for (int i = tid; i < K; i += 1024) vrow[i] = sa * vsrc[i];
+    } else {
+      for (int i = tid; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
+    }
     for (int i = tid; i < nb; i += 1024) crow[i] = x[J * nb + i];
-    for (int i = tid; i < K; i += 1024) vrow[i] = sa * vsrc[i];
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < NT; t++) {
       const int ib = tid + t * 1024;
       if (ib < nb) {
-        T a = T(0);
+        T a = crow[ib] * diag_sum(vrow, b_str[ib], L1);
         for (uint64_t f = b_off[ib]; f < b_off[ib + 1]; f++) {
           const uint64_t be = b_ent[f];
           const T v = vrow[ent_pq(be)] * crow[ent_ket(be)];
@@ -289,9 +377,11 @@ __global__ void __launch_bounds__(1024) k_mixed_lds(int nb, int K, const uint64_
 
 // Same term without LDS staging (rows too long for LDS): gathers from HBM.
 template <typename T>
-__global__ void __launch_bounds__(256) k_mixed_global(int nb, int nchunk, int K,
+__global__ void __launch_bounds__(256) k_mixed_global(int nb, int nchunk, int K, int L,
                                                       const uint64_t *__restrict__ a_off,
                                                       const uint64_t *__restrict__ a_ent,
+                                                      const uint64_t *__restrict__ a_str,
+                                                      const uint64_t *__restrict__ b_str,
                                                       const uint64_t *__restrict__ b_off,
                                                       const uint64_t *__restrict__ b_ent,
                                                       const T *__restrict__ eri,
@@ -299,19 +389,29 @@ __global__ void __launch_bounds__(256) k_mixed_global(int nb, int nchunk, int K,
   const int I = blockIdx.x / nchunk;
   const int ib = (blockIdx.x % nchunk) * blockDim.x + threadIdx.x;
   if (ib >= nb) return;
+  const int L1 = L + 1;
   const uint64_t f0 = b_off[ib], f1 = b_off[ib + 1];
+  const uint64_t occ_b = b_str[ib];
   T acc = T(0);
   for (uint64_t e = a_off[I]; e < a_off[I + 1]; e++) {
     const uint64_t ent = a_ent[e];
     const T *xr = x + (size_t)ent_ket(ent) * nb;
     const T *vr = eri + (size_t)ent_pq(ent) * K;
-    T a = T(0);
+    T a = xr[ib] * diag_sum(vr, occ_b, L1);
     for (uint64_t f = f0; f < f1; f++) {
       const uint64_t be = b_ent[f];
       const T v = vr[ent_pq(be)] * xr[ent_ket(be)];
       a += ent_neg(be) ? -v : v;
     }
     acc += ent_neg(ent) ? -a : a;
+  }
+  // spectator: beta singles dressed by the alpha string of I
+  const uint64_t occ_a = a_str[I];
+  const T *xr = x + (size_t)I * nb;
+  for (uint64_t f = f0; f < f1; f++) {
+    const uint64_t be = b_ent[f];
+    const T v = spectator_val(eri, occ_a, L1, K, (int)ent_pq(be)) * xr[ent_ket(be)];
+    acc += ent_neg(be) ? -v : v;
   }
   y[(size_t)I * nb + ib] += acc;
 }
@@ -324,33 +424,59 @@ __global__ void __launch_bounds__(256) k_cast(size_t n, const From *__restrict__
     out[i] = (To)in[i];
 }
 
+// Dense same-spin block from the tables: H[r*n + ket] = value (zeroed first).
+template <typename T>
+__global__ void __launch_bounds__(256) k_densify(int n, const uint64_t *__restrict__ s_off,
+                                                 const uint64_t *__restrict__ s_ent,
+                                                 const double *__restrict__ s_val,
+                                                 const uint64_t *__restrict__ d_off,
+                                                 const uint32_t *__restrict__ d_ket,
+                                                 const double *__restrict__ d_val, T *__restrict__ H) {
+  for (int r = blockIdx.x; r < n; r += gridDim.x) {
+    T *row = H + (size_t)r * n;
+    for (uint64_t e = s_off[r] + threadIdx.x; e < s_off[r + 1]; e += blockDim.x)
+      row[ent_ket(s_ent[e])] = (T)s_val[e];
+    for (uint64_t e = d_off[r] + threadIdx.x; e < d_off[r + 1]; e += blockDim.x)
+      row[d_ket[e]] = (T)d_val[e];
+  }
+}
+
+// y = hii * x (the diagonal term of the GEMM form).
+template <typename T>
+__global__ void __launch_bounds__(256) k_diag_mul(size_t n, const T *__restrict__ hii,
+                                                  const T *__restrict__ x, T *__restrict__ y) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    y[i] = hii[i] * x[i];
+}
+
 // ------------------------------------------------------------------ host side
 template <typename T>
 struct SigmaOperands {
-  const T *s_val_a, *d_val_a, *s_val_b, *d_val_b, *dress_a, *dress_b, *eri, *hii;
+  const T *s_val_a, *d_val_a, *s_val_b, *d_val_b, *eri, *hii;
   T *T1, *T2;
 };
 
 static SigmaOperands<double> operands(Context &c, double *) {
-  return {c.A.s_val, c.A.d_val, c.B.s_val, c.B.d_val, c.A.dress, c.B.dress, c.eri, c.hii, c.T1, c.T2};
+  return {c.A.s_val, c.A.d_val, c.B.s_val, c.B.d_val, c.eri, c.hii, c.T1, c.T2};
 }
 static SigmaOperands<float> operands(Context &c, float *) {
-  return {c.f32.s_val_a, c.f32.d_val_a, c.f32.s_val_b, c.f32.d_val_b, c.f32.dress_a,
-          c.f32.dress_b, c.f32.eri,     c.f32.hii,     c.f32.T1,      c.f32.T2};
+  return {c.f32.s_val_a, c.f32.d_val_a, c.f32.s_val_b, c.f32.d_val_b,
+          c.f32.eri,     c.f32.hii,     c.f32.T1,      c.f32.T2};
 }
 
 template <typename T>
 static void sigma_same(Context &c, const SpinTables &t, const T *s_val, const T *d_val, int nrows,
-                       int M, const T *dress, const T *hii, const T *x, T *y, bool diag) {
+                       int M, const T *hii, const T *x, T *y, bool diag) {
   const int nch = (int)cdiv(M, 64), nrb = (int)cdiv(nrows, 4);
   const size_t nblk = (size_t)8 * cdiv(nch, 8) * nrb;
   if (nblk >= (1ull << 31)) throw Error(3, "sigma: grid too large for one launch");
   if (diag)
     hipLaunchKernelGGL((k_same_xcd<T, true>), dim3((unsigned)nblk), dim3(256), 0, c.stream, nrows, M,
-                       nch, nrb, t.s_off, t.s_ent, s_val, t.d_off, t.d_ket, d_val, dress, hii, x, y);
+                       nch, nrb, t.s_off, t.s_ent, s_val, t.d_off, t.d_ket, d_val, hii, x, y);
   else
     hipLaunchKernelGGL((k_same_xcd<T, false>), dim3((unsigned)nblk), dim3(256), 0, c.stream, nrows, M,
-                       nch, nrb, t.s_off, t.s_ent, s_val, t.d_off, t.d_ket, d_val, dress, hii, x, y);
+                       nch, nrb, t.s_off, t.s_ent, s_val, t.d_off, t.d_ket, d_val, hii, x, y);
   check_launch("k_same_xcd");
 }
 
@@ -372,9 +498,9 @@ template <typename T, int NT, int EB>
 static void launch_mixed_ell(Context &c, const T *eri, const T *x, T *y, size_t lds) {
   auto fn = k_mixed_ell<T, NT, EB>;
   SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K,
-                     c.B.nslice * 64, c.A.s_off, c.A.s_ent, c.B.ell, c.B.perm, c.B.sl_off,
-                     c.B.sl_len, eri, x, y);
+  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
+                     c.B.nslice * 64, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell, c.B.perm,
+                     c.B.sl_off, c.B.sl_len, eri, x, y);
 }
 
 template <typename T, int EB>
@@ -395,8 +521,8 @@ template <typename T, int NT>
 static void launch_mixed_lds(Context &c, const T *eri, const T *x, T *y, size_t lds) {
   auto fn = k_mixed_lds<T, NT>;
   SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.A.s_off,
-                     c.A.s_ent, c.B.s_off, c.B.s_ent, eri, x, y);
+  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
+                     c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.s_off, c.B.s_ent, eri, x, y);
 }
 
 template <typename T, int NT>
@@ -409,9 +535,9 @@ static void launch_mixed_parts(Context &c, const T *eri, const T *x, T *y, size_
     pt.off[p] = c.B.psl_off[p];
     pt.len[p] = c.B.psl_len[p];
   }
-  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K,
-                     c.B.nslice * 64, c.B.nparts, c.B.part_h, c.A.s_off, c.A.s_ent, pt, c.B.perm, eri,
-                     x, y);
+  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
+                     c.B.nslice * 64, c.B.nparts, c.B.part_h, c.A.s_off, c.A.s_ent, c.A.str, c.B.str,
+                     pt, c.B.perm, eri, x, y);
 }
 
 // Ket row staged in parts: smallest part count (<= 8) whose piece and ERI row fit.
@@ -459,20 +585,97 @@ static void sigma_mixed(Context &c, const T *eri, const T *x, T *y) {
   } else {
     const unsigned nchunk = cdiv(nb, 256);
     hipLaunchKernelGGL((k_mixed_global<T>), dim3((unsigned)((size_t)c.na * nchunk)), dim3(256), 0,
-                       c.stream, nb, (int)nchunk, c.K, c.A.s_off, c.A.s_ent, c.B.s_off, c.B.s_ent, eri,
-                       x, y);
+                       c.stream, nb, (int)nchunk, c.K, c.L, c.A.s_off, c.A.s_ent, c.A.str, c.B.str,
+                       c.B.s_off, c.B.s_ent, eri, x, y);
   }
   check_launch("k_mixed");
 }
 
+// ---- same-spin terms as GEMMs
+static rocblas_handle blas(Context &c) {
+  if (!c.blas) {
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success)
+      throw Error(3, "rocblas_create_handle failed");
+    c.blas = h;
+    if (rocblas_set_stream(h, c.stream) != rocblas_status_success)
+      throw Error(3, "rocblas_set_stream failed");
+  }
+  return (rocblas_handle)c.blas;
+}
+
+void same_dense_release(Context &c) {
+  if (c.blas) (void)rocblas_destroy_handle((rocblas_handle)c.blas);
+  c.blas = nullptr;
+}
+
 template <typename T>
-static void sigma_impl(Context &c, const T *x, T *y) {
-  const SigmaOperands<T> o = operands(c, (T *)nullptr);
+static T *densify(Context &c, const SpinTables &t) {
+  const size_t n = (size_t)t.n;
+  T *H = (T *)dalloc(c, sizeof(T) * n * n);
+  SBDX_HIP(hipMemsetAsync(H, 0, sizeof(T) * n * n, c.stream));
+  hipLaunchKernelGGL((k_densify<T>), dim3((unsigned)std::min<size_t>(n, 65535)), dim3(256), 0, c.stream,
+                     (int)n, t.s_off, t.s_ent, t.s_val, t.d_off, t.d_ket, t.d_val, H);
+  check_launch("k_densify");
+  return H;
+}
+
+static void dense_blocks(Context &c, double *&Ha, double *&Hb) {
+  if (!c.Hd_a) {
+    c.Hd_a = densify<double>(c, c.A);
+    c.Hd_b = c.B.alias ? c.Hd_a : densify<double>(c, c.B);
+  }
+  Ha = c.Hd_a;
+  Hb = c.Hd_b;
+}
+static void dense_blocks(Context &c, float *&Ha, float *&Hb) {
+  if (!c.Hf_a) {
+    c.Hf_a = densify<float>(c, c.A);
+    c.Hf_b = c.B.alias ? c.Hf_a : densify<float>(c, c.B);
+  }
+  Ha = c.Hf_a;
+  Hb = c.Hf_b;
+}
+
+static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const double *A, int lda,
+                           const double *B, int ldb, double *C, int ldc) {
+  const double one = 1.0;
+  return rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B,
+                       ldb, &one, C, ldc);
+}
+static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const float *A, int lda,
+                           const float *B, int ldb, float *C, int ldc) {
+  const float one = 1.0f;
+  return rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B,
+                       ldb, &one, C, ldc);
+}
+
+// y = hii*x + H_a X + X H_b.  Column-major, the na x nb row-major X is the
+// nb x na matrix X^T, so with symmetric H_a, H_b:
+//   (H_a X)^T = X^T H_a   -> gemm(nb, na, na, X^T, H_a)
+//   (X H_b)^T = H_b X^T   -> gemm(nb, na, nb, H_b, X^T)
+template <typename T>
+static void same_dense(Context &c, const T *hii, const T *x, T *y) {
+  T *Ha, *Hb;
+  dense_blocks(c, Ha, Hb);
+  const int na = (int)c.na, nb = (int)c.nb;
+  const unsigned g = (unsigned)std::min<size_t>(cdiv(c.N, 256), 65536);
+  hipLaunchKernelGGL((k_diag_mul<T>), dim3(g), dim3(256), 0, c.stream, c.N, hii, x, y);
+  check_launch("k_diag_mul");
+  rocblas_handle h = blas(c);
+  if (gemm(h, nb, na, na, x, nb, Ha, na, y, nb) != rocblas_status_success)
+    throw Error(3, "same-spin alpha GEMM failed");
+  if (gemm(h, nb, na, nb, Hb, nb, x, nb, y, nb) != rocblas_status_success)
+    throw Error(3, "same-spin beta GEMM failed");
+}
+
+template <typename T>
+static void same_sparse(Context &c, const SigmaOperands<T> &o, const T *x, T *y) {
   const int na = (int)c.na, nb = (int)c.nb;
   const bool f32 = sizeof(T) == 4;
   {
     ProfScope ps(c, f32 ? "same_alpha_f32" : "same_alpha");
-    sigma_same<T>(c, c.A, o.s_val_a, o.d_val_a, na, nb, o.dress_b, o.hii, x, y, true);
+    sigma_same<T>(c, c.A, o.s_val_a, o.d_val_a, na, nb, o.hii, x, y, true);
   }
   {
     ProfScope ps(c, f32 ? "transpose_f32" : "transpose");
@@ -480,11 +683,67 @@ static void sigma_impl(Context &c, const T *x, T *y) {
   }
   {
     ProfScope ps(c, f32 ? "same_beta_f32" : "same_beta");
-    sigma_same<T>(c, c.B, o.s_val_b, o.d_val_b, nb, na, o.dress_a, nullptr, o.T1, o.T2, false);
+    sigma_same<T>(c, c.B, o.s_val_b, o.d_val_b, nb, na, nullptr, o.T1, o.T2, false);
   }
   {
     ProfScope ps(c, f32 ? "transpose_add_f32" : "transpose_add");
     transpose<T>(c, o.T2, y, nb, na, true);
+  }
+}
+
+// Autotuner for the same-spin form: when the dense blocks fit (below a
+// quarter of free device memory, 32-bit GEMM dimensions), time both forms
+// on this x (second of two runs each, HIP events on the handle's stream;
+// both overwrite y, which the caller's real pass overwrites again) and keep
+// the faster for this precision.  SBDX_SAME_SPIN=sparse|dense overrides.
+template <typename T>
+static int same_mode(Context &c, const SigmaOperands<T> &o, const T *x, T *y) {
+  const int pi = sizeof(T) == 4 ? 1 : 0;
+  if (c.same_mode[pi] >= 0) return c.same_mode[pi];
+  const char *env = std::getenv("SBDX_SAME_SPIN");
+  const size_t na = c.na, nb = c.nb;
+  const size_t bytes = sizeof(T) * (na * na + (c.B.alias ? 0 : nb * nb));
+  size_t free_b = 0, total_b = 0;
+  SBDX_HIP(hipMemGetInfo(&free_b, &total_b));
+  const bool fits = na < (1u << 31) && nb < (1u << 31) && na * nb < (1ull << 31) &&
+                    bytes < free_b / 4;
+  if (env && std::string(env) == "sparse") return c.same_mode[pi] = 0;
+  if (env && std::string(env) == "dense") {
+    if (!fits) throw Error(3, "SBDX_SAME_SPIN=dense: same-spin blocks do not fit");
+    return c.same_mode[pi] = 1;
+  }
+  if (!fits) return c.same_mode[pi] = 0;
+  hipEvent_t e0, e1;
+  SBDX_HIP(hipEventCreate(&e0));
+  SBDX_HIP(hipEventCreate(&e1));
+  float ms[2] = {0.f, 0.f};
+  for (int mode = 0; mode < 2; mode++)
+    for (int rep = 0; rep < 2; rep++) {
+      SBDX_HIP(hipEventRecord(e0, c.stream));
+      if (mode == 0)
+        same_sparse<T>(c, o, x, y);
+      else
+        same_dense<T>(c, o.hii, x, y);
+      SBDX_HIP(hipEventRecord(e1, c.stream));
+      SBDX_HIP(hipEventSynchronize(e1));
+      SBDX_HIP(hipEventElapsedTime(&ms[mode], e0, e1));
+    }
+  SBDX_HIP(hipEventDestroy(e0));
+  SBDX_HIP(hipEventDestroy(e1));
+  c.same_ms[pi][0] = ms[0];
+  c.same_ms[pi][1] = ms[1];
+  return c.same_mode[pi] = ms[1] < ms[0] ? 1 : 0;
+}
+
+template <typename T>
+static void sigma_impl(Context &c, const T *x, T *y) {
+  const SigmaOperands<T> o = operands(c, (T *)nullptr);
+  const bool f32 = sizeof(T) == 4;
+  if (same_mode<T>(c, o, x, y) == 1) {
+    ProfScope ps(c, f32 ? "same_gemm_f32" : "same_gemm");
+    same_dense<T>(c, o.hii, x, y);
+  } else {
+    same_sparse<T>(c, o, x, y);
   }
   {
     ProfScope ps(c, f32 ? "mixed_f32" : "mixed");
@@ -519,15 +778,12 @@ void ensure_f32(Context &c) {
   };
   c.f32.s_val_a = cast(c.A.s_val, c.A.ns);
   c.f32.d_val_a = cast(c.A.d_val, c.A.nd);
-  c.f32.dress_a = cast(c.A.dress, K * c.A.n);
   if (c.B.alias) {
     c.f32.s_val_b = c.f32.s_val_a;
     c.f32.d_val_b = c.f32.d_val_a;
-    c.f32.dress_b = c.f32.dress_a;
   } else {
     c.f32.s_val_b = cast(c.B.s_val, c.B.ns);
     c.f32.d_val_b = cast(c.B.d_val, c.B.nd);
-    c.f32.dress_b = cast(c.B.dress, K * c.B.n);
   }
   c.f32.eri = cast(c.eri, K * K);
   c.f32.hii = cast(c.hii, c.N);

@@ -1,4 +1,4 @@
-// Excitation tables, dressing tables, the diagonal and the basis sign.
+// Excitation tables, the diagonal and the basis sign.
 //
 // Table construction follows the semantics of GenerateExcitation
 // (chemistry/tpb/helper.h:185-323): for every bra string, every ket string of
@@ -90,26 +90,6 @@ __global__ void __launch_bounds__(256) k_table_fill(
     ps += (uint64_t)__popcll(m2);
     pd += (uint64_t)__popcll(m4);
   }
-}
-
-// dress[pq*n + x] = sum_{k in str[x]} (pq|kk): the part of an other-spin
-// single excitation p->q that depends on this spin's string x.
-__global__ void __launch_bounds__(256) k_dress(const uint64_t *__restrict__ s, int n, int L,
-                                               const double *__restrict__ eri,
-                                               double *__restrict__ dress) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int pq = blockIdx.y;
-  if (x >= n) return;
-  const int K = L * L;
-  const double *row = eri + (size_t)pq * K;
-  uint64_t occ = s[x];
-  double acc = 0.0;
-  while (occ) {
-    const int k = __ffsll((unsigned long long)occ) - 1;
-    occ &= occ - 1;
-    acc += row[k * L + k];
-  }
-  dress[(size_t)pq * n + x] = acc;
 }
 
 // Per-string pieces of the diagonal: e1[x] = sum_p h_pp + sum_{p<q}
@@ -355,14 +335,6 @@ void build_singles_ell_parts(Context &c, SpinTables &t, int nparts) {
   }
   t.nparts = nparts;
   t.part_h = h;
-}
-
-void build_dress(Context &c, SpinTables &t) {
-  t.dress = (double *)dalloc(c, sizeof(double) * (size_t)c.K * t.n);
-  ProfScope ps(c, "dress");
-  hipLaunchKernelGGL(k_dress, dim3(cdiv(t.n, 256), c.K), dim3(256), 0, c.stream, t.str, t.n, c.L,
-                     c.eri, t.dress);
-  check_launch("k_dress");
 }
 
 void compute_diag(Context &c) {

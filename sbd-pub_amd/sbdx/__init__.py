@@ -43,6 +43,7 @@ ABI_SYMBOLS = [
     "sbdx_mlp_nparam", "sbdx_mlp_nfeat", "sbdx_mlp_set_weights", "sbdx_mlp_get_weights",
     "sbdx_mlp_forward", "sbdx_mlp_gradient", "sbdx_mlp_train", "sbdx_mlp_record", "sbdx_mlp_collect",
     "sbdx_set_preconditioner", "sbdx_precond_stats", "sbdx_set_verbose",
+    "sbdx_same_spin_form",
 ]
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -91,6 +92,8 @@ def lib():
     L.sbdx_sigma_fp32.argtypes = [C.c_void_p, _dp, _dp]
     L.sbdx_set_restart.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_set_verbose.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_same_spin_form.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_float), C.POINTER(C.c_float)]
     _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     _u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
     L.sbdx_mlp_set_weights.argtypes = [C.c_void_p, _fp, C.c_size_t]
@@ -299,6 +302,17 @@ class TPBOperator:
         _check(lib().sbdx_work_counts(self.h, out))
         return dict(alpha_singles=out[0], alpha_doubles=out[1], beta_singles=out[2],
                     beta_doubles=out[3], mixed_terms=out[4], ndet=out[5])
+
+    def same_spin_forms(self):
+        """Per precision: the same-spin form in use ('dense' GEMM, 'sparse'
+        rows, None before the first sigma) and the autotuner's timings (ms)."""
+        out = {}
+        for prec, key in ((0, "fp64"), (1, "fp32")):
+            f, a, b = C.c_int(), C.c_float(), C.c_float()
+            _check(lib().sbdx_same_spin_form(self.h, prec, C.byref(f), C.byref(a), C.byref(b)))
+            out[key] = dict(form={-1: None, 0: "sparse", 1: "dense"}[f.value],
+                            ms_sparse=round(a.value, 3), ms_dense=round(b.value, 3))
+        return out
 
     def profile(self, enable=True):
         _check(lib().sbdx_profile(self.h, 1 if enable else 0))

@@ -35,3 +35,33 @@ def test_mixed_fallbacks_match_oracle(limit, oracle_mod):
     s32 = op.sigma_fp32(v)
     assert np.linalg.norm(s32 - r) / np.linalg.norm(r) < 2e-6
     op.close()
+
+
+# The same-spin terms as dense GEMMs or as sparse row kernels (normally the
+# faster form is picked on first use), each against the oracle, on distinct
+# alpha/beta string sets and on a molecule.
+@pytest.mark.parametrize("form", ["sparse", "dense"])
+@pytest.mark.parametrize("case", ["tiny_ab", "h2o_1em3"])
+def test_same_spin_forms_match_oracle(form, case, oracle_mod):
+    import sbdx
+    from test_gpu_parity import CASES
+    inst = CASES[case]()
+    old = os.environ.get("SBDX_SAME_SPIN")
+    os.environ["SBDX_SAME_SPIN"] = form
+    print(f"[same-spin {form}] {case}", flush=True)
+    try:
+        op = sbdx.TPBOperator.from_instance(inst)
+        v = random_vector(op.ndet, 17)
+        s = op.sigma(v)
+        s32 = op.sigma_fp32(v)
+    finally:
+        if old is None:
+            del os.environ["SBDX_SAME_SPIN"]
+        else:
+            os.environ["SBDX_SAME_SPIN"] = old
+    orc = oracle_mod.Oracle.from_instance(inst)
+    r = orc.sigma(v)
+    assert np.max(np.abs(s - r)) < 1e-12 * np.max(np.abs(r))
+    assert np.linalg.norm(s32 - r) / np.linalg.norm(r) < 2e-6
+    op.close()
+    orc.close()
