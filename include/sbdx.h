@@ -74,6 +74,12 @@ int sbdx_diag(sbdx_handle_t h, double *hii);
  * chemistry/tpb/mult.h:625-881); with accumulate == 0 Wb is overwritten.
  * The diagonal term uses the handle's own hii. */
 int sbdx_sigma(sbdx_handle_t h, const double *Wk, double *Wb, int accumulate);
+/* k products at once (no counterpart in the reference, whose operator takes
+ * one vector per run): Wb[v] (+)= H*Wk[v] for v < k, host arrays of k
+ * consecutive n_det vectors, 1 <= k <= 64.  The same-spin terms run as
+ * batched GEMMs and the mixed term as one multi-vector kernel per group of
+ * up to four vectors, so the table and ERI-row traffic is shared. */
+int sbdx_sigma_block(sbdx_handle_t h, int k, const double *Wk, double *Wb, int accumulate);
 /* Same with device pointers on the handle's stream (synchronous on return). */
 int sbdx_sigma_device(sbdx_handle_t h, const double *d_Wk, double *d_Wb, int accumulate);
 
@@ -90,6 +96,25 @@ int sbdx_sigma_device(sbdx_handle_t h, const double *d_Wk, double *d_Wb, int acc
  */
 int sbdx_davidson(sbdx_handle_t h, double *W, int max_it, int nb, double eps,
                   double max_time, double *energy, int *n_steps);
+
+/*
+ * Block Davidson for the lowest nroots eigenpairs (no counterpart in the
+ * reference, whose Davidson is single-root): `block` (nroots <= block <= 16)
+ * correction vectors per iteration go through one block sigma; basis up to
+ * min(64, 4*block) vectors, thick restart on the Ritz block; convergence
+ * when |E_j W_j - H W_j| < eps for every j < nroots, judged on exact FP64
+ * products (with SBDX_PREC_MIXED the corrections' products are FP32 and
+ * every restart recomputes the block's products in FP64).  The
+ * preconditioner is the handle's (sbdx_set_preconditioner).
+ * W: host, nroots consecutive n_det vectors, linearly independent start
+ * vectors on entry (block - nroots more are unit vectors on the lowest
+ * diagonal elements), the Ritz vectors on exit; energies[nroots] ascending;
+ * *n_products counts single-vector products.  max_it bounds iterations
+ * (block expansions).  Returns SBDX_ERR_NOT_CONVERGED with valid outputs
+ * when eps was not reached.
+ */
+int sbdx_davidson_block(sbdx_handle_t h, int nroots, int block, double *W, int max_it, double eps,
+                        double max_time, double *energies, int *n_products);
 
 /* Split form of sbdx_davidson for callers that keep the vector resident:
  * upload the start vector, solve on the device, download the solution. */

@@ -26,6 +26,16 @@ void *dalloc(Context &c, size_t bytes) {
   return p;
 }
 
+void dfree(Context &c, void *p) {
+  if (!p) return;
+  for (size_t i = 0; i < c.allocs.size(); i++)
+    if (c.allocs[i] == p) {
+      (void)hipFree(p);
+      c.allocs.erase(c.allocs.begin() + (long)i);
+      return;
+    }
+}
+
 void dfree_all(Context &c) {
   for (void *p : c.allocs) (void)hipFree(p);
   c.allocs.clear();
@@ -310,6 +320,57 @@ int sbdx_sigma(sbdx_handle_t h, const double *Wk, double *Wb, int accumulate) {
     sign_convert(c, c.vout, c.vin);
     download(c, Wb, c.vin);
   }
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_sigma_block(sbdx_handle_t h, int k, const double *Wk, double *Wb, int accumulate) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!Wk || !Wb) throw Error(SBDX_ERR_ARG, "null vector");
+  if (k < 1 || k > 64) throw Error(SBDX_ERR_ARG, "k must be in [1, 64]");
+  const size_t N = c.N;
+  TmpBuf xb(sizeof(double) * N * k), yb(sizeof(double) * N * k);
+  double *x = (double *)xb.p, *y = (double *)yb.p;
+  for (int v = 0; v < k; v++) {
+    upload(c, c.vout, Wk + (size_t)v * N);
+    sign_convert(c, c.vout, x + (size_t)v * N);
+  }
+  sigma_block(c, k, x, y);
+  for (int v = 0; v < k; v++) {
+    if (accumulate) {
+      upload(c, c.vin, Wb + (size_t)v * N);
+      sign_convert_add(c, y + (size_t)v * N, c.vin);
+    } else {
+      sign_convert(c, y + (size_t)v * N, c.vin);
+    }
+    download(c, Wb + (size_t)v * N, c.vin);
+  }
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_davidson_block(sbdx_handle_t h, int nroots, int block, double *W, int max_it, double eps,
+                        double max_time, double *energies, int *n_products) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!W || !energies) throw Error(SBDX_ERR_ARG, "null argument");
+  if (nroots < 1 || block < nroots || block > 16)
+    throw Error(SBDX_ERR_ARG, "need 1 <= nroots <= block <= 16");
+  const size_t N = c.N;
+  TmpBuf w0(sizeof(double) * N * nroots), w1(sizeof(double) * N * nroots);
+  double *a = (double *)w0.p, *b = (double *)w1.p;
+  for (int j = 0; j < nroots; j++) {
+    upload(c, c.vin, W + (size_t)j * N);
+    sign_convert(c, c.vin, a + (size_t)j * N);
+  }
+  const int rc = davidson_block(c, nroots, block, max_it, eps, max_time, c.precision == 1, a, b, energies,
+                                n_products);
+  for (int j = 0; j < nroots; j++) {
+    sign_convert(c, b + (size_t)j * N, c.vin);
+    download(c, W + (size_t)j * N, c.vin);
+  }
+  if (rc != 0) throw Error(SBDX_ERR_NOT_CONVERGED, "davidson_block: not converged");
   return SBDX_OK;
   SBDX_GUARD_END
 }

@@ -85,6 +85,8 @@ struct F32Operands {
   float *s_val_a = nullptr, *d_val_a = nullptr, *s_val_b = nullptr, *d_val_b = nullptr;
   float *eri = nullptr, *hii = nullptr;
   float *x = nullptr, *y = nullptr, *T1 = nullptr, *T2 = nullptr;
+  float *xb = nullptr, *yb = nullptr;  // block products, kcap vectors each
+  int kcap = 0;
 };
 
 // Learned preconditioner (mlp.hip): FP32 master weights, gradient, Adam
@@ -126,6 +128,12 @@ struct Context {
   int verbose = 0;         // 1: one stderr line per Davidson step (E0, |r|)
   int restart_keep = 0;    // 0: restart from W alone (reference); k: thick restart keeping k Ritz vectors
   std::vector<double *> spareY, spareHY;  // thick-restart staging vectors
+  // Block Davidson: contiguous basis and products (blk_m vectors each), the
+  // Ritz block W, HW and residuals R (blk_b vectors each), small matrices.
+  double *blk_C = nullptr, *blk_HC = nullptr, *blk_W = nullptr, *blk_HW = nullptr,
+         *blk_R = nullptr, *blk_small = nullptr;
+  int blk_m = 0, blk_b = 0;
+  std::vector<double> blk_E;  // Ritz values of the last block solve
   MlpState mlp;
   int precond = 0;         // 0: diagonal R/(E0-d); 1: MLP with diagonal fallback on stall
   // work vectors
@@ -161,6 +169,8 @@ constexpr int kScalarOffset = 2 * kNbMax * kNbMax + kNbMax;
 // ---------------------------------------------------------------- helpers
 void *dalloc(Context &c, size_t bytes);
 void dfree_all(Context &c);
+void dfree(Context &c, void *p);  // one dalloc'ed buffer
+void *blas_handle(Context &c);     // rocblas_handle bound to the stream
 
 // Device scratch released at scope exit (callers synchronize first).
 struct TmpBuf {
@@ -193,12 +203,17 @@ void sign_convert_add(Context &c, const double *in, double *out);  // out += S*i
 void sigma_alpha_first(Context &c, const double *x, double *y);  // y = H' x (overwrite)
 void sigma_alpha_first_f32(Context &c, const double *x, double *y);  // same, FP32 arithmetic
 void ensure_f32(Context &c);
+// y_v = H' x_v for k contiguous vectors (x + v*N), FP64 / FP32 arithmetic
+void sigma_block(Context &c, int k, const double *x, double *y);
+void sigma_block_f32(Context &c, int k, const double *x, double *y);
 void same_dense_release(Context &c);  // frees the rocBLAS handle
 void transpose_into(Context &c, const double *in, double *out, int R, int Cc);  // out = in^T
 
 // davidson.hip
 int davidson_solve(Context &c, int max_it, int nb, double eps, double max_time, double *energy,
                    int *n_steps);
+int davidson_block(Context &c, int nroots, int block, int max_it, double eps, double max_time,
+                   bool mixed, const double *W0, double *Wout, double *Eout, int *n_products);
 int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps, double max_time,
                          bool mixed, double *energy, int *n_steps);
 double device_dot(Context &c, const double *x, const double *y, size_t n);

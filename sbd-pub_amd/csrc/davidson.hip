@@ -9,8 +9,12 @@
 // fixed grid, then a fixed-order sum), so results are reproducible run to
 // run.  The subspace eigensolve runs on the device (cyclic Jacobi, ascending
 // order), so only |R| and E0 travel to the host once per step.
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <string>
 
 #include "common.h"
 
@@ -645,6 +649,301 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
   SBDX_HIP(hipStreamSynchronize(c.stream));
   if (energy) *energy = E0;
   if (n_steps) *n_steps = steps;
+  return done ? 0 : 5;
+}
+
+// ------------------------------------------------------------ block Davidson
+// R_j = HW_j - E_j W_j for j < b (vectors contiguous, stride n); partial
+// |R_j|^2 of this block into part[blk*b + j].
+__global__ void __launch_bounds__(kRedThreads) k_block_resid(int b, size_t n, const double *__restrict__ W,
+                                                             const double *__restrict__ HW,
+                                                             const double *__restrict__ E,
+                                                             double *__restrict__ R,
+                                                             double *__restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (int j = 0; j < b; j++) {
+    const double e = E[j];
+    const double *w = W + (size_t)j * n;
+    const double *hw = HW + (size_t)j * n;
+    double *r = R + (size_t)j * n;
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+      const double v = hw[i] - e * w[i];
+      r[i] = v;
+      s += v * v;
+    }
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * b + j] = t;
+  }
+}
+
+// out[k] = fixed-order sum over blocks of part[blk*nv + k]
+__global__ void k_finish_vec(int nv, int nblk, const double *__restrict__ part, double *__restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= nv) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; b++) s += part[(size_t)b * nv + k];
+  out[k] = s;
+}
+
+// Columns j in [j0, j1) of the subspace matrix hold HC_i . C_j for every row
+// i; keep the entries with i <= j (the older vector's product) and mirror
+// them: H[j][i] = H[i][j] for i < j.
+__global__ void k_symm_cols(double *H, int ld, int j0, int j1) {
+  const int j = j0 + blockIdx.x;
+  const int i = threadIdx.x;
+  if (j < j1 && i < j) H[j + (size_t)ld * i] = H[i + (size_t)ld * j];
+}
+
+// Block Davidson (no counterpart in the reference, whose solver is
+// single-root): the lowest nroots eigenpairs with `block` >= nroots
+// correction vectors per iteration, all pushed through one block sigma
+// (sigma_block: batched GEMMs and the multi-vector mixed kernel).
+//   basis C (contiguous, up to m vectors) and products HC, subspace matrix
+//   H[i][j] = HC_i . C_j (i <= j) by GEMM, Rayleigh-Ritz on the device;
+//   Ritz block W = C U, HW = HC U, R = HW - E W (GEMMs + one pass);
+//   corrections: preconditioned residuals of the lowest `block` Ritz pairs
+//   (diagonal, or the MLP with the stall fallback of the thick solver),
+//   classical Gram-Schmidt twice against the basis by GEMM, then modified
+//   Gram-Schmidt inside the block; vectors whose norm falls below 1e-10
+//   after projection are dropped;
+//   restart when the next block would not fit: the basis becomes W (its
+//   product HW by combination; in mixed mode an exact FP64 block product),
+//   so convergence (|R_j| < eps for j < nroots) is decided on exact FP64
+//   products in both modes.
+// W0/Wout: device, nroots vectors in the alpha-first basis.
+int davidson_block(Context &c, int nroots, int block, int max_it, double eps, double max_time,
+                   bool mixed, const double *W0, double *Wout, double *Eout, int *n_products) {
+  if (nroots < 1 || block < nroots || block > 16) throw Error(1, "davidson_block: need 1 <= nroots <= block <= 16");
+  const int m = std::min(kNbMax, std::max(2 * block, 4 * block));
+  const size_t N = c.N;
+  if (c.blk_m < m || c.blk_b < block) {
+    dfree(c, c.blk_C);
+    dfree(c, c.blk_HC);
+    dfree(c, c.blk_W);
+    dfree(c, c.blk_HW);
+    dfree(c, c.blk_R);
+    c.blk_C = c.blk_HC = c.blk_W = c.blk_HW = c.blk_R = nullptr;
+    c.blk_m = c.blk_b = 0;
+    c.blk_C = (double *)dalloc(c, sizeof(double) * N * m);
+    c.blk_HC = (double *)dalloc(c, sizeof(double) * N * m);
+    c.blk_W = (double *)dalloc(c, sizeof(double) * N * block);
+    c.blk_HW = (double *)dalloc(c, sizeof(double) * N * block);
+    c.blk_R = (double *)dalloc(c, sizeof(double) * N * block);
+    c.blk_m = m;
+    c.blk_b = block;
+  }
+  if (!c.blk_small) c.blk_small = (double *)dalloc(c, sizeof(double) * kNbMax * kNbMax * 2);
+  rocblas_handle hb = (rocblas_handle)blas_handle(c);
+  const int ld = kNbMax;
+  double *H = c.dsmall, *U = c.dsmall + ld * ld, *E = c.dsmall + 2 * ld * ld;
+  double *S = c.blk_small;             // CGS coefficients (d x block, ld d)
+  double *nrm = c.blk_small + ld * ld;  // block residual norms^2
+  const double eps_reg = 1.0e-12;
+  const dim3 g(kRedBlocks), bthr(kRedThreads);
+  double *C = c.blk_C, *HC = c.blk_HC;
+  auto col = [&](double *base, int j) { return base + (size_t)j * N; };
+  auto chk = [](rocblas_status st, const char *what) {
+    if (st != rocblas_status_success) throw Error(3, std::string("davidson_block: ") + what + " failed");
+  };
+  const double one = 1.0, zero = 0.0, mone = -1.0;
+  auto t_start = std::chrono::steady_clock::now();
+  auto elapsed = [&]() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  };
+  // classical Gram-Schmidt (twice) of the nv vectors at C[d..] against C[0..d)
+  auto cgs = [&](int d, int nv) {
+    if (d == 0) return;
+    for (int pass = 0; pass < 2; pass++) {
+      chk(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, d, nv, (int)N, &one, C,
+                        (int)N, col(C, d), (int)N, &zero, S, d),
+          "CGS coefficients");
+      chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nv, d, &mone, C,
+                        (int)N, S, d, &one, col(C, d), (int)N),
+          "CGS update");
+    }
+  };
+  // MGS of C[d..d+nv) among themselves; keeps those with norm > tol, packed
+  // at C[d..); returns how many were kept
+  auto mgs_pack = [&](int d, int nv) {
+    int kept = 0;
+    for (int j = 0; j < nv; j++) {
+      double *v = col(C, d + j);
+      for (int k = 0; k < kept; k++) {
+        const double *X[1] = {col(C, d + k)};
+        dots(c, X, 1, v, c.part);
+        hipLaunchKernelGGL(k_axpy_part, g, bthr, 0, c.stream, v, col(C, d + k), N, c.part, kRedBlocks);
+        check_launch("k_axpy_part");
+      }
+      const double nv2 = device_dot(c, v, v, N);
+      if (!(nv2 > 1e-20)) continue;
+      double *dst = col(C, d + kept);
+      if (dst != v)
+        SBDX_HIP(hipMemcpyAsync(dst, v, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+      const double *X[1] = {dst};
+      dots(c, X, 1, dst, c.part);
+      hipLaunchKernelGGL(k_scale_part, g, bthr, 0, c.stream, dst, N, c.part, kRedBlocks);
+      check_launch("k_scale_part");
+      kept++;
+    }
+    return kept;
+  };
+  // products of C[d..d+nv) and the new subspace columns
+  auto expand = [&](int d, int nv, bool approx) {
+    {
+      ProfScope ps(c, approx ? "blk_sigma_f32" : "blk_sigma");
+      if (approx)
+        sigma_block_f32(c, nv, col(C, d), col(HC, d));
+      else
+        sigma_block(c, nv, col(C, d), col(HC, d));
+    }
+    ProfScope ps(c, "blk_subspace");
+    chk(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, d + nv, nv, (int)N, &one,
+                      HC, (int)N, col(C, d), (int)N, &zero, H + (size_t)ld * d, ld),
+        "subspace columns");
+    hipLaunchKernelGGL(k_symm_cols, dim3(nv), dim3(kNbMax), 0, c.stream, H, ld, d, d + nv);
+    check_launch("k_symm_cols");
+  };
+
+  // initial block: the given vectors, then unit vectors on the lowest
+  // diagonal elements, orthonormalized
+  SBDX_HIP(hipMemcpyAsync(C, W0, sizeof(double) * N * nroots, hipMemcpyDeviceToDevice, c.stream));
+  if (block > nroots) {
+    std::vector<double> hh(N);
+    SBDX_HIP(hipMemcpyAsync(hh.data(), c.hii, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+    std::vector<size_t> idx(N);
+    for (size_t i = 0; i < N; i++) idx[i] = i;
+    const size_t kk = std::min<size_t>(N, (size_t)(block - nroots));
+    std::partial_sort(idx.begin(), idx.begin() + kk, idx.end(),
+                      [&](size_t a, size_t b) { return hh[a] < hh[b] || (hh[a] == hh[b] && a < b); });
+    SBDX_HIP(hipMemsetAsync(col(C, nroots), 0, sizeof(double) * N * (block - nroots), c.stream));
+    const double v1 = 1.0;
+    for (size_t k = 0; k < kk; k++)
+      SBDX_HIP(hipMemcpyAsync(col(C, nroots + (int)k) + idx[k], &v1, sizeof(double), hipMemcpyHostToDevice,
+                              c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+  }
+  int d = mgs_pack(0, block);
+  if (d < nroots) throw Error(1, "davidson_block: start vectors are linearly dependent");
+  expand(0, d, false);
+  int products = d;
+  bool done = false, exact_now = true;
+  c.trace.clear();
+  c.blk_E.assign(nroots, 0.0);
+  std::vector<double> hE(block), hR(block);
+  std::vector<double> rhist;
+  int diag_left = 0;
+  c.mlp.used_mlp = c.mlp.used_diag = 0;
+  for (int it = 0;; it++) {
+    const int nr = std::min(block, d);
+    {
+      ProfScope ps(c, "blk_ritz");
+      const size_t shm = sizeof(double) * 2 * d * d;
+      hipLaunchKernelGGL(k_syev, dim3(1), dim3(64), shm, c.stream, d, ld, H, U, E);
+      check_launch("k_syev");
+      chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nr, d, &one, C, (int)N, U,
+                        ld, &zero, c.blk_W, (int)N),
+          "Ritz vectors");
+      chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nr, d, &one, HC, (int)N,
+                        U, ld, &zero, c.blk_HW, (int)N),
+          "Ritz products");
+      hipLaunchKernelGGL(k_block_resid, g, bthr, 0, c.stream, nr, N, c.blk_W, c.blk_HW, E, c.blk_R, c.part);
+      check_launch("k_block_resid");
+      hipLaunchKernelGGL(k_finish_vec, dim3(1), dim3(kNbMax), 0, c.stream, nr, kRedBlocks, c.part, nrm);
+      check_launch("k_finish_vec");
+    }
+    SBDX_HIP(hipMemcpyAsync(hE.data(), E, sizeof(double) * nr, hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipMemcpyAsync(hR.data(), nrm, sizeof(double) * nr, hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+    double rmax = 0.0;
+    for (int j = 0; j < nroots; j++) {
+      hR[j] = std::sqrt(hR[j]);
+      rmax = std::max(rmax, hR[j]);
+      c.blk_E[j] = hE[j];
+    }
+    c.trace.push_back(hE[0]);
+    c.trace.push_back(rmax);
+    if (c.verbose) {
+      fprintf(stderr, "[sbdx] block it %d dim %d products %d%s E0 = %.15f max|r| = %.3e\n", it, d, products,
+              exact_now ? " (exact)" : "", hE[0], rmax);
+      fflush(stderr);
+    }
+    if (rmax < eps && exact_now) {
+      done = true;
+      break;
+    }
+    const bool out_of_time = elapsed() > max_time || it >= max_it;
+    if (d + block > m || (rmax < eps && !exact_now) || out_of_time) {
+      // restart on the Ritz block (exact products in mixed mode)
+      ProfScope ps(c, "blk_restart");
+      SBDX_HIP(hipMemcpyAsync(C, c.blk_W, sizeof(double) * N * nr, hipMemcpyDeviceToDevice, c.stream));
+      if (mixed) {
+        d = 0;
+        expand(0, nr, false);
+        products += nr;
+      } else {
+        SBDX_HIP(hipMemcpyAsync(HC, c.blk_HW, sizeof(double) * N * nr, hipMemcpyDeviceToDevice, c.stream));
+        chk(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, nr, nr, (int)N, &one, HC,
+                          (int)N, C, (int)N, &zero, H, ld),
+            "restart subspace");
+        hipLaunchKernelGGL(k_symm_cols, dim3(nr), dim3(kNbMax), 0, c.stream, H, ld, 0, nr);
+        check_launch("k_symm_cols");
+      }
+      d = nr;
+      exact_now = true;
+      if (out_of_time) {
+        // one more Rayleigh-Ritz on the exact block for the outputs
+        const size_t shm = sizeof(double) * 2 * d * d;
+        hipLaunchKernelGGL(k_syev, dim3(1), dim3(64), shm, c.stream, d, ld, H, U, E);
+        check_launch("k_syev");
+        chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nr, d, &one, C, (int)N,
+                          U, ld, &zero, c.blk_W, (int)N),
+            "Ritz vectors");
+        SBDX_HIP(hipMemcpyAsync(hE.data(), E, sizeof(double) * nroots, hipMemcpyDeviceToHost, c.stream));
+        SBDX_HIP(hipStreamSynchronize(c.stream));
+        for (int j = 0; j < nroots; j++) c.blk_E[j] = hE[j];
+        break;
+      }
+      continue;  // Rayleigh-Ritz on the restarted basis gives the residuals
+    }
+    // corrections from the lowest nr Ritz pairs
+    rhist.push_back(rmax);
+    const size_t nh = rhist.size();
+    if (nh >= 4 && rhist[nh - 1] > 0.9 * rhist[nh - 4] && diag_left == 0) diag_left = 3;
+    const bool use_mlp = c.precond == 1 && c.mlp.valid && diag_left == 0;
+    if (diag_left > 0) diag_left--;
+    {
+      ProfScope ps(c, use_mlp ? "precond_mlp" : "blk_orth");
+      for (int j = 0; j < nr; j++) {
+        if (use_mlp) {
+          mlp_precond(c, col(c.blk_R, j), E + j, col(C, d + j));
+          c.mlp.used_mlp++;
+        } else {
+          hipLaunchKernelGGL(k_precond, g, bthr, 0, c.stream, col(c.blk_R, j), c.hii, E + j, N, eps_reg,
+                             col(C, d + j));
+          check_launch("k_precond");
+          c.mlp.used_diag++;
+        }
+      }
+    }
+    int nv;
+    {
+      ProfScope ps(c, "blk_orth");
+      cgs(d, nr);
+      nv = mgs_pack(d, nr);
+    }
+    if (nv == 0) throw Error(5, "davidson_block: no new direction (basis exhausted)");
+    expand(d, nv, mixed);
+    products += nv;
+    d += nv;
+    exact_now = !mixed;
+  }
+  SBDX_HIP(hipMemcpyAsync(Wout, c.blk_W, sizeof(double) * N * nroots, hipMemcpyDeviceToDevice, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  for (int j = 0; j < nroots; j++) Eout[j] = c.blk_E[j];
+  if (n_products) *n_products = products;
   return done ? 0 : 5;
 }
 

@@ -223,6 +223,97 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
   }
 }
 
+// Mixed term for KV vectors at once (x_v = x + v*N): one ERI row and the KV
+// ket rows x_v[J,:] staged per alpha single (then the spectator pair), so
+// each sliced-ELL entry costs one table load and 1 + KV LDS reads for KV
+// multiply-adds.  Dressing as in k_mixed_ell.
+template <typename T, int NT, int KV>
+__global__ void __launch_bounds__(1024) k_mixed_multi(int nb, int K, int L, int nslots, size_t N,
+                                                      const uint64_t *__restrict__ a_off,
+                                                      const uint64_t *__restrict__ a_ent,
+                                                      const uint64_t *__restrict__ a_str,
+                                                      const uint64_t *__restrict__ b_str,
+                                                      const uint32_t *__restrict__ ell,
+                                                      const uint32_t *__restrict__ perm,
+                                                      const uint64_t *__restrict__ sl_off,
+                                                      const uint32_t *__restrict__ sl_len,
+                                                      const T *__restrict__ eri,
+                                                      const T *__restrict__ x, T *__restrict__ y) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *vrow = reinterpret_cast<T *>(lds_raw);  // K + 1
+  T *crow = vrow + K + 1;                    // KV rows of nb
+  const int L1 = L + 1;
+  const int I = blockIdx.x;
+  const int tid = threadIdx.x;
+  T acc[KV][NT];
+  uint64_t occ[NT];
+  int rr[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+#pragma unroll
+    for (int v = 0; v < KV; v++) acc[v][t] = T(0);
+    const int slot = tid + t * 1024;
+    const uint32_t r = slot < nslots ? perm[slot] : 0xffffffffu;
+    rr[t] = r != 0xffffffffu ? (int)r : 0;
+    occ[t] = r != 0xffffffffu ? b_str[r] : 0ull;
+  }
+  const uint64_t occ_a = a_str[I];
+  const uint64_t e1 = a_off[I + 1];
+  for (uint64_t e = a_off[I]; e <= e1; e++) {
+    size_t J = I;
+    __syncthreads();
+    if (e < e1) {
+      const uint64_t ent = a_ent[e];
+      J = ent_ket(ent);
+      const T sa = ent_neg(ent) ? T(-1) : T(1);
+      const T *vsrc = eri + (size_t)ent_pq(ent) * K;
+      for (int i = tid; i < K; i += 1024) vrow[i] = sa * vsrc[i];
+    } else {
+      for (int i = tid; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
+    }
+    if (tid == 0) vrow[K] = T(0);
+#pragma unroll
+    for (int v = 0; v < KV; v++)
+      for (int i = tid; i < nb; i += 1024) crow[v * nb + i] = x[v * N + J * nb + i];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int slot = tid + t * 1024;
+      if (slot < nslots) {
+        const int sl = slot >> 6;
+        const uint32_t len = sl_len[sl];
+        const uint32_t *p = ell + sl_off[sl] + (slot & 63);
+        T a[KV];
+#pragma unroll
+        for (int v = 0; v < KV; v++) a[v] = T(0);
+#pragma unroll 4
+        for (uint32_t k = 0; k < len; k++) {
+          const uint32_t w = p[64 * (size_t)k];
+          const int jb = (int)(w & 0xffffu);
+          const T vr = vrow[(w >> 16) & 0x7fffu];
+          const T sv = (w >> 31) ? -vr : vr;
+#pragma unroll
+          for (int v = 0; v < KV; v++) a[v] += sv * crow[v * nb + jb];
+        }
+        const T d = diag_sum(vrow, occ[t], L1);
+#pragma unroll
+        for (int v = 0; v < KV; v++) acc[v][t] += a[v] + crow[v * nb + rr[t]] * d;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int slot = tid + t * 1024;
+    if (slot < nslots) {
+      const uint32_t r = perm[slot];
+      if (r != 0xffffffffu) {
+#pragma unroll
+        for (int v = 0; v < KV; v++) y[v * N + (size_t)I * nb + r] += acc[v][t];
+      }
+    }
+  }
+}
+
 // Mixed term when one ket row does not fit in LDS: the row is staged in
 // nparts pieces of h kets and each piece is walked with its own sliced-ELL
 // table (SpinTables::pell, kets relative to the piece); the sign-folded ERI
@@ -591,6 +682,50 @@ static void sigma_mixed(Context &c, const T *eri, const T *x, T *y) {
   check_launch("k_mixed");
 }
 
+template <typename T, int NT, int KV>
+static void launch_mixed_multi(Context &c, const T *eri, const T *x, T *y, size_t lds) {
+  auto fn = k_mixed_multi<T, NT, KV>;
+  SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
+                     c.B.nslice * 64, c.N, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell, c.B.perm,
+                     c.B.sl_off, c.B.sl_len, eri, x, y);
+}
+
+template <typename T, int KV>
+static bool mixed_multi_dispatch(Context &c, const T *eri, const T *x, T *y) {
+  if (!c.B.has_ell) return false;
+  const size_t lds = sizeof(T) * ((size_t)c.K + 1 + (size_t)KV * c.nb);
+  const int nt = (c.B.nslice * 64 + 1023) / 1024;
+  if (lds > c.lds_limit || nt > 32) return false;
+  if (nt <= 1) launch_mixed_multi<T, 1, KV>(c, eri, x, y, lds);
+  else if (nt <= 2) launch_mixed_multi<T, 2, KV>(c, eri, x, y, lds);
+  else if (nt <= 4) launch_mixed_multi<T, 4, KV>(c, eri, x, y, lds);
+  else if (nt <= 8) launch_mixed_multi<T, 8, KV>(c, eri, x, y, lds);
+  else if (nt <= 16) launch_mixed_multi<T, 16, KV>(c, eri, x, y, lds);
+  else launch_mixed_multi<T, 32, KV>(c, eri, x, y, lds);
+  check_launch("k_mixed_multi");
+  return true;
+}
+
+// Mixed term of k contiguous vectors: groups of 4, then 2 vectors through
+// k_mixed_multi while their rows fit in LDS, single vectors otherwise.
+template <typename T>
+static void sigma_mixed_block(Context &c, const T *eri, int k, const T *x, T *y) {
+  const size_t N = c.N;
+  int v = 0;
+  while (v < k) {
+    const int left = k - v;
+    if (left >= 4 && mixed_multi_dispatch<T, 4>(c, eri, x + v * N, y + v * N)) {
+      v += 4;
+    } else if (left >= 2 && mixed_multi_dispatch<T, 2>(c, eri, x + v * N, y + v * N)) {
+      v += 2;
+    } else {
+      sigma_mixed<T>(c, eri, x + v * N, y + v * N);
+      v += 1;
+    }
+  }
+}
+
 // ---- same-spin terms as GEMMs
 static rocblas_handle blas(Context &c) {
   if (!c.blas) {
@@ -603,6 +738,8 @@ static rocblas_handle blas(Context &c) {
   }
   return (rocblas_handle)c.blas;
 }
+
+void *blas_handle(Context &c) { return (void *)blas(c); }
 
 void same_dense_release(Context &c) {
   if (c.blas) (void)rocblas_destroy_handle((rocblas_handle)c.blas);
@@ -666,6 +803,44 @@ static void same_dense(Context &c, const T *hii, const T *x, T *y) {
   if (gemm(h, nb, na, na, x, nb, Ha, na, y, nb) != rocblas_status_success)
     throw Error(3, "same-spin alpha GEMM failed");
   if (gemm(h, nb, na, nb, Hb, nb, x, nb, y, nb) != rocblas_status_success)
+    throw Error(3, "same-spin beta GEMM failed");
+}
+
+static rocblas_status gemm_batched(rocblas_handle h, int m, int n, int k, const double *A, int lda,
+                                   rocblas_stride sa, const double *B, int ldb, rocblas_stride sb,
+                                   double *C, int ldc, rocblas_stride sc, int batch) {
+  const double one = 1.0;
+  return rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, k,
+                                       &one, A, lda, sa, B, ldb, sb, &one, C, ldc, sc, batch);
+}
+static rocblas_status gemm_batched(rocblas_handle h, int m, int n, int k, const float *A, int lda,
+                                   rocblas_stride sa, const float *B, int ldb, rocblas_stride sb,
+                                   float *C, int ldc, rocblas_stride sc, int batch) {
+  const float one = 1.0f;
+  return rocblas_sgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, k,
+                                       &one, A, lda, sa, B, ldb, sb, &one, C, ldc, sc, batch);
+}
+
+// Dense form for k contiguous vectors: the alpha term as a strided batch
+// over the vectors (shared H_a), the beta term as one GEMM over all k*na
+// columns of the side-by-side X^T blocks.
+template <typename T>
+static void same_dense_block(Context &c, const T *hii, int k, const T *x, T *y) {
+  T *Ha, *Hb;
+  dense_blocks(c, Ha, Hb);
+  const int na = (int)c.na, nb = (int)c.nb;
+  const size_t N = c.N;
+  if ((size_t)k * na >= (1ull << 31)) throw Error(3, "block sigma: too many columns for one GEMM");
+  const unsigned g = (unsigned)std::min<size_t>(cdiv(N, 256), 65536);
+  for (int v = 0; v < k; v++) {
+    hipLaunchKernelGGL((k_diag_mul<T>), dim3(g), dim3(256), 0, c.stream, N, hii, x + v * N, y + v * N);
+    check_launch("k_diag_mul");
+  }
+  rocblas_handle h = blas(c);
+  if (gemm_batched(h, nb, na, na, x, nb, (rocblas_stride)N, Ha, na, 0, y, nb, (rocblas_stride)N, k) !=
+      rocblas_status_success)
+    throw Error(3, "same-spin alpha batched GEMM failed");
+  if (gemm(h, nb, k * na, nb, Hb, nb, x, nb, y, nb) != rocblas_status_success)
     throw Error(3, "same-spin beta GEMM failed");
 }
 
@@ -752,6 +927,46 @@ static void sigma_impl(Context &c, const T *x, T *y) {
 }
 
 void sigma_alpha_first(Context &c, const double *x, double *y) { sigma_impl<double>(c, x, y); }
+
+// k products at once, vectors contiguous (x + v*N -> y + v*N).
+template <typename T>
+static void sigma_block_impl(Context &c, int k, const T *x, T *y) {
+  const SigmaOperands<T> o = operands(c, (T *)nullptr);
+  const bool f32 = sizeof(T) == 4;
+  const size_t N = c.N;
+  if (same_mode<T>(c, o, x, y) == 1) {
+    ProfScope ps(c, f32 ? "blk_same_gemm_f32" : "blk_same_gemm");
+    same_dense_block<T>(c, o.hii, k, x, y);
+  } else {
+    for (int v = 0; v < k; v++) same_sparse<T>(c, o, x + v * N, y + v * N);
+  }
+  {
+    ProfScope ps(c, f32 ? "blk_mixed_f32" : "blk_mixed");
+    sigma_mixed_block<T>(c, o.eri, k, x, y);
+  }
+}
+
+void sigma_block(Context &c, int k, const double *x, double *y) {
+  if (k < 1) return;
+  sigma_block_impl<double>(c, k, x, y);
+}
+
+void sigma_block_f32(Context &c, int k, const double *x, double *y) {
+  if (k < 1) return;
+  ensure_f32(c);
+  const size_t n = (size_t)k * c.N;
+  if (c.f32.kcap < k) {
+    c.f32.xb = (float *)dalloc(c, sizeof(float) * n);
+    c.f32.yb = (float *)dalloc(c, sizeof(float) * n);
+    c.f32.kcap = k;
+  }
+  const unsigned g = (unsigned)std::min<size_t>(cdiv(n, 256), 65536);
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(g), dim3(256), 0, c.stream, n, x, c.f32.xb);
+  check_launch("k_cast");
+  sigma_block_impl<float>(c, k, c.f32.xb, c.f32.yb);
+  hipLaunchKernelGGL((k_cast<float, double>), dim3(g), dim3(256), 0, c.stream, n, c.f32.yb, y);
+  check_launch("k_cast");
+}
 
 void sigma_alpha_first_f32(Context &c, const double *x, double *y) {
   ensure_f32(c);

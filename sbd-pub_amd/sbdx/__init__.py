@@ -43,7 +43,7 @@ ABI_SYMBOLS = [
     "sbdx_mlp_nparam", "sbdx_mlp_nfeat", "sbdx_mlp_set_weights", "sbdx_mlp_get_weights",
     "sbdx_mlp_forward", "sbdx_mlp_gradient", "sbdx_mlp_train", "sbdx_mlp_record", "sbdx_mlp_collect",
     "sbdx_set_preconditioner", "sbdx_precond_stats", "sbdx_set_verbose",
-    "sbdx_same_spin_form",
+    "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
 ]
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -90,6 +90,9 @@ def lib():
     L.sbdx_work_counts.argtypes = [C.c_void_p, _dp]
     L.sbdx_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_sigma_fp32.argtypes = [C.c_void_p, _dp, _dp]
+    L.sbdx_sigma_block.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
+    L.sbdx_davidson_block.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_double,
+                                      C.c_double, _dp, C.POINTER(C.c_int)]
     L.sbdx_set_restart.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_set_verbose.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_same_spin_form.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
@@ -277,6 +280,29 @@ class TPBOperator:
         rc = _check(lib().sbdx_solve(self.h, max_it, nb, eps, max_time, C.byref(e), C.byref(n)),
                     allow=(5,))
         return e.value, n.value, rc == 0
+
+    def sigma_block(self, V):
+        """Products of the k rows of V (k x ndet) in one block call."""
+        V = np.ascontiguousarray(np.atleast_2d(V), dtype=np.float64)
+        out = np.zeros_like(V)
+        _check(lib().sbdx_sigma_block(self.h, V.shape[0], V.reshape(-1), out.reshape(-1), 0))
+        return out
+
+    def davidson_block(self, nroots, block, W0=None, max_it=200, eps=1e-8, max_time=1e30):
+        """Lowest nroots eigenpairs by block Davidson.  W0: nroots x ndet
+        start vectors (default: HF and unit vectors at the next-lowest
+        diagonal elements).  Returns (energies, W, products, converged)."""
+        if W0 is None:
+            W0 = np.zeros((nroots, self.ndet))
+            order = np.argsort(self.make_diag(), kind="stable")[:nroots]
+            for j, i in enumerate(order):
+                W0[j, i] = 1.0
+        W = np.ascontiguousarray(W0, dtype=np.float64).reshape(nroots, self.ndet).copy()
+        E = np.zeros(nroots)
+        n = C.c_int()
+        rc = _check(lib().sbdx_davidson_block(self.h, nroots, block, W.reshape(-1), max_it, eps,
+                                              max_time, E, C.byref(n)), allow=(5,))
+        return E, W, n.value, rc == 0
 
     def trace(self):
         n = C.c_int()
