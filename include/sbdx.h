@@ -151,6 +151,14 @@ int sbdx_set_restart(sbdx_handle_t h, int keep);
 int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_sparse,
                         float *ms_dense);
 
+/* Staging width of the mixed-term kernel chosen for precision 0 (FP64) or 1
+ * (FP32) by timing widths 1, 2 and 4 (alpha singles staged in LDS per round)
+ * on the first sigma (no counterpart in the reference): *width 0 before
+ * that, -1 when no width fits in LDS (row parts or the CSR/global variants
+ * run), else 1, 2 or 4; ms[3] the timings (0 for a width that did not fit
+ * or was not timed).  SBDX_MIXED_EB=1|2|4 in the environment fixes it. */
+int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, float *ms);
+
 /* verbose != 0: the solves print one line per step to stderr, step index,
  * E0 and |r| (as davidson.h:839-851 prints per step); thick-restart solves
  * mark the exact FP64 residual of each restart.  0 (default) prints nothing. */

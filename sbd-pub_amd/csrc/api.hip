@@ -454,6 +454,17 @@ int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_spa
   SBDX_GUARD_END
 }
 
+int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, float *ms) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (precision != 0 && precision != 1) throw Error(SBDX_ERR_ARG, "precision must be 0 or 1");
+  if (width) *width = c.mixed_eb[precision];
+  if (ms)
+    for (int w = 0; w < 3; w++) ms[w] = c.mixed_eb_ms[precision][w];
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
 int sbdx_set_verbose(sbdx_handle_t h, int level) {
   SBDX_GUARD_BEGIN
   ctx_of(h).verbose = level;

@@ -124,6 +124,11 @@ struct Context {
   // 0 sparse row kernels, 1 dense GEMM; the timings that decided it (ms).
   int same_mode[2] = {-1, -1};
   float same_ms[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+  // Staging width (pairs per LDS round) of the sliced-ELL mixed kernel per
+  // precision: 0 not tuned, -1 none fits, else 1, 2 or 4; timings (ms) of
+  // widths 1, 2, 4 (0: did not fit).
+  int mixed_eb[2] = {0, 0};
+  float mixed_eb_ms[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   int precision = 0;       // 0: FP64 throughout; 1: FP32 sigma on corrections, FP64 restarts
   int verbose = 0;         // 1: one stderr line per Davidson step (E0, |r|)
   int restart_keep = 0;    // 0: restart from W alone (reference); k: thick restart keeping k Ritz vectors

@@ -654,13 +654,71 @@ static bool mixed_parts_dispatch(Context &c, const T *eri, const T *x, T *y) {
   return false;
 }
 
+static bool mixed_ell_eb(Context &c, int eb, const double *eri, const double *x, double *y) {
+  return eb == 4 ? mixed_ell_dispatch<double, 4>(c, eri, x, y)
+       : eb == 2 ? mixed_ell_dispatch<double, 2>(c, eri, x, y)
+                 : mixed_ell_dispatch<double, 1>(c, eri, x, y);
+}
+static bool mixed_ell_eb(Context &c, int eb, const float *eri, const float *x, float *y) {
+  return eb == 4 ? mixed_ell_dispatch<float, 4>(c, eri, x, y)
+       : eb == 2 ? mixed_ell_dispatch<float, 2>(c, eri, x, y)
+                 : mixed_ell_dispatch<float, 1>(c, eri, x, y);
+}
+static double *scratch_vec(Context &c, const double *) { return c.T2; }
+static float *scratch_vec(Context &c, const float *) { return c.f32.T2; }
+
+// Autotuner for the sliced-ELL mixed kernel's staging width: every width
+// whose LDS footprint fits (1, 2, 4 pairs per round) runs twice on this x
+// into a scratch vector (the kernel accumulates, so y is untouched), HIP
+// events on the stream time the second run, the fastest is kept for this
+// precision; -1 when none fits.  SBDX_MIXED_EB=1|2|4 overrides.
+template <typename T>
+static int mixed_eb(Context &c, const T *eri, const T *x) {
+  const int pi = sizeof(T) == 4 ? 1 : 0;
+  if (c.mixed_eb[pi] != 0) return c.mixed_eb[pi];
+  const char *env = std::getenv("SBDX_MIXED_EB");
+  if (env) {
+    const int eb = std::atoi(env);
+    if (eb != 1 && eb != 2 && eb != 4) throw Error(1, "SBDX_MIXED_EB must be 1, 2 or 4");
+    return c.mixed_eb[pi] = eb;
+  }
+  T *scratch = scratch_vec(c, x);
+  hipEvent_t e0, e1;
+  SBDX_HIP(hipEventCreate(&e0));
+  SBDX_HIP(hipEventCreate(&e1));
+  const int widths[3] = {1, 2, 4};
+  int best = -1;
+  float best_ms = 0.f;
+  for (int w = 0; w < 3; w++) {
+    c.mixed_eb_ms[pi][w] = 0.f;
+    bool ok = true;
+    float ms = 0.f;
+    for (int rep = 0; rep < 2 && ok; rep++) {
+      SBDX_HIP(hipEventRecord(e0, c.stream));
+      ok = mixed_ell_eb(c, widths[w], eri, x, scratch);
+      SBDX_HIP(hipEventRecord(e1, c.stream));
+      SBDX_HIP(hipEventSynchronize(e1));
+      SBDX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    }
+    if (!ok) continue;
+    check_launch("k_mixed_ell");
+    c.mixed_eb_ms[pi][w] = ms;
+    if (best < 0 || ms < best_ms) {
+      best = widths[w];
+      best_ms = ms;
+    }
+  }
+  SBDX_HIP(hipEventDestroy(e0));
+  SBDX_HIP(hipEventDestroy(e1));
+  return c.mixed_eb[pi] = best;
+}
+
 template <typename T>
 static void sigma_mixed(Context &c, const T *eri, const T *x, T *y) {
   if (c.B.has_ell) {
-    // widest staging that fits: 4 pairs (FP32), 2 pairs, then 1, then row parts
-    if (sizeof(T) == 4 && mixed_ell_dispatch<T, 4>(c, eri, x, y)) return check_launch("k_mixed_ell");
-    if (mixed_ell_dispatch<T, 2>(c, eri, x, y)) return check_launch("k_mixed_ell");
-    if (mixed_ell_dispatch<T, 1>(c, eri, x, y)) return check_launch("k_mixed_ell");
+    // tuned staging width, then row parts
+    const int eb = mixed_eb<T>(c, eri, x);
+    if (eb > 0 && mixed_ell_eb(c, eb, eri, x, y)) return check_launch("k_mixed_ell");
     if (mixed_parts_dispatch<T>(c, eri, x, y)) return check_launch("k_mixed_parts");
   }
   const int nb = (int)c.nb;

@@ -44,6 +44,7 @@ ABI_SYMBOLS = [
     "sbdx_mlp_forward", "sbdx_mlp_gradient", "sbdx_mlp_train", "sbdx_mlp_record", "sbdx_mlp_collect",
     "sbdx_set_preconditioner", "sbdx_precond_stats", "sbdx_set_verbose",
     "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
+    "sbdx_mixed_staging",
 ]
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -98,6 +99,7 @@ def lib():
     L.sbdx_same_spin_form.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]
     _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.sbdx_mixed_staging.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _fp]
     _u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
     L.sbdx_mlp_set_weights.argtypes = [C.c_void_p, _fp, C.c_size_t]
     L.sbdx_mlp_get_weights.argtypes = [C.c_void_p, _fp, C.c_size_t]
@@ -338,6 +340,10 @@ class TPBOperator:
             _check(lib().sbdx_same_spin_form(self.h, prec, C.byref(f), C.byref(a), C.byref(b)))
             out[key] = dict(form={-1: None, 0: "sparse", 1: "dense"}[f.value],
                             ms_sparse=round(a.value, 3), ms_dense=round(b.value, 3))
+            w, ms = C.c_int(), np.zeros(3, dtype=np.float32)
+            _check(lib().sbdx_mixed_staging(self.h, prec, C.byref(w), ms))
+            out[key]["mixed_width"] = w.value
+            out[key]["mixed_ms"] = [round(float(t), 3) for t in ms]
         return out
 
     def profile(self, enable=True):
