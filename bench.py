@@ -52,6 +52,11 @@ def kernel_bytes(wc, na, nb, e=8.0):
         + (8.0 + e) * wc["beta_singles"] + (4.0 + e) * wc["beta_doubles"],
         "mixed": e * (ab + dressing + 2 * N) + 8.0 * wc["alpha_singles"] + 4.0 * wc["beta_singles"],
         "same_gemm": e * (3 * N + 2 * 2 * N + na * na + nb * nb),
+        # spin-flip symmetric form: the mixed term for the upper triangle
+        # (half the gathers); one GEMM plus y = hii*x + G + G^T
+        "sym_mixed": e * ((ab + dressing) / 2 + 2 * N) + 8.0 * wc["alpha_singles"]
+        + 4.0 * wc["beta_singles"],
+        "sym_same": e * (2 * N + na * na + 4 * N),
         "transpose": 2 * e * N,
         "transpose_add": 3 * e * N,
     }
@@ -96,15 +101,27 @@ def main():
     ap.add_argument("--max-it", type=int, default=200)
     ap.add_argument("--max-time", type=float, default=900.0)
     ap.add_argument("--init", choices=["hf", "random"], default="hf")
-    ap.add_argument("--solver", choices=["plain", "thick", "mixed"], default="mixed",
+    ap.add_argument("--solver", choices=["plain", "thick", "mixed", "block"], default="mixed",
                     help="timed solver: plain = the reference algorithm in FP64; thick = FP64 "
-                         "thick restart; mixed = thick restart with FP32 correction products")
+                         "thick restart; mixed = thick restart with FP32 correction products; "
+                         "block = block Davidson (--block corrections per iteration, one root) "
+                         "with FP32 correction products")
+    ap.add_argument("--block", type=int, default=4)
+    ap.add_argument("--no-symmetry", dest="symmetry", action="store_false",
+                    help="general sigma in the optimized solver (default: spin-flip symmetric "
+                         "form when the string sets are equal)")
+    ap.add_argument("--keep", type=int, default=4,
+                    help="Ritz vectors kept at a thick restart (thick / mixed solvers)")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
                     help="skip the plain FP64 reference solve")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quiet", action="store_true", help="no per-step solver lines on stderr")
+    ap.add_argument("--precond", choices=["diag", "mlp"], default="diag",
+                    help="preconditioner of the timed solver (mlp: learned, weights below)")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "sbd-pub_amd", "sbdx", "data",
+                                                      "precond_mlp.npy"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no "
@@ -124,6 +141,7 @@ def main():
     wc = op.work_counts()
     log(f"operator built in {t_build:.2f} s: ndet={op.ndet} counts={wc}")
 
+    same_sets = bool(np.array_equal(inst["astr"], inst["bstr"]))
     if args.init == "hf":
         W0 = sbdx.basis_init_vector(op.ndet)
     else:
@@ -137,39 +155,53 @@ def main():
             op.sigma_fp32(W0)
     log(f"warmup done; same-spin form {op.same_spin_forms()}")
 
-    def run(solver, nsolve):
+    if args.precond == "mlp":
+        op.mlp_set_weights(np.load(args.weights))
+        log(f"learned preconditioner weights from {args.weights}")
+
+    def run(solver, nsolve, precond="diag"):
         op.configure(solver)
+        if solver in ("thick", "mixed"):
+            op.set_restart(args.keep)
+            op.set_symmetry(args.symmetry and same_sets)
+        op.set_preconditioner(1 if precond == "mlp" else 0)
         op.set_verbose(0 if args.quiet else 1)
         op.profile(True)
         times, energies, nsteps, conv = [], [], [], []
         for k in range(nsolve):
             op.set_vector(W0)
             t0 = time.perf_counter()
-            E, n, ok = op.solve(max_it=args.max_it, nb=args.nb, eps=args.eps,
-                                max_time=args.max_time)
+            if solver == "block":
+                Eb, _, n, ok = op.davidson_block(1, args.block, W0=W0[None, :], max_it=args.max_it,
+                                                 eps=args.eps, max_time=args.max_time)
+                E = float(Eb[0])
+            else:
+                E, n, ok = op.solve(max_it=args.max_it, nb=args.nb, eps=args.eps,
+                                    max_time=args.max_time)
             dt = time.perf_counter() - t0
             times.append(dt)
             energies.append(E)
             nsteps.append(n)
             conv.append(ok)
             tr = op.trace()
-            log(f"{solver} solve {k}: E0={E:.12f} |r|={tr[-1, 1]:.2e} sigma-products={n} "
-                f"converged={ok} time={dt:.3f} s")
+            log(f"{solver}/{precond} solve {k}: E0={E:.12f} |r|={tr[-1, 1]:.2e} sigma-products={n} "
+                f"converged={ok} time={dt:.3f} s precond-steps={op.precond_stats()}")
         stats = op.profile_stats()
         op.profile(False)
         return dict(time=float(np.median(times)), E=energies[-1], steps=int(np.median(nsteps)),
-                    converged=all(conv), stats=stats)
+                    converged=all(conv), stats=stats, precond_steps=op.precond_stats())
 
     best = args.solver
     plain = run("plain", 1 if best != "plain" else args.steps) if args.compare or best == "plain" else None
-    res = run(best, args.steps) if best != "plain" else plain
+    res = run(best, args.steps, args.precond) if best != "plain" else plain
     stats = res["stats"]
     times = [res["time"]]
     energies = [res["E"]]
     nsteps = [res["steps"]]
     conv = [res["converged"]]
 
-    gemm_flops = 2.0 * na * na * nb + 2.0 * nb * nb * na  # dense same-spin, per sigma
+    gemm_flops = {"same_gemm": 2.0 * na * na * nb + 2.0 * nb * nb * na,  # dense same-spin
+                  "sym_same": 2.0 * na * na * nb}                        # symmetric form
 
     def kernel_table(stats):
         kb64 = kernel_bytes(wc, na, nb, 8.0)
@@ -183,13 +215,14 @@ def main():
             if base in kb and avg > 0:
                 ent["alg_bytes"] = kb[base]
                 ent["GBps"] = round(kb[base] / (avg * 1e-3) / 1e9, 1)
-            if base == "same_gemm" and avg > 0:
-                ent["flops"] = gemm_flops
-                ent["TFLOPs"] = round(gemm_flops / (avg * 1e-3) / 1e12, 2)
+            if base in gemm_flops and avg > 0:
+                ent["flops"] = gemm_flops[base]
+                ent["TFLOPs"] = round(gemm_flops[base] / (avg * 1e-3) / 1e12, 2)
             out[name] = ent
         return out
 
-    SIGMA_PARTS = ("same_gemm", "same_alpha", "transpose", "same_beta", "transpose_add", "mixed")
+    SIGMA_PARTS = ("same_gemm", "same_alpha", "transpose", "same_beta", "transpose_add", "mixed",
+                   "sym_same", "sym_mixed")
 
     def sigma_rate(kernels, suffix):
         names = [n + suffix for n in SIGMA_PARTS if n + suffix in kernels]
@@ -203,7 +236,7 @@ def main():
     sig_kernels = [n for n in kernels if "alg_bytes" in kernels[n]]
     dom = max(sig_kernels, key=lambda n: kernels[n]["total_ms"])
     dk = kernels[dom]
-    if dom.startswith("same_gemm"):
+    if dom.startswith("same_gemm") or dom.startswith("sym_same"):
         peak = MFMA_PEAK_TFLOPS["f32" if dom.endswith("_f32") else "f64"]
         roofline = dict(kernel=dom, bound="mfma", achieved=dk["TFLOPs"], peak=peak, unit="TFLOP/s",
                         frac=round(dk["TFLOPs"] / peak, 4), traffic=None)
@@ -217,7 +250,7 @@ def main():
         s64_ms, s64_gbps = sigma_rate(pk, "")
 
     T = res["time"]
-    dtypes = {"plain": "f64", "thick": "f64", "mixed": "f32+f64"}
+    dtypes = {"plain": "f64", "thick": "f64", "mixed": "f32+f64", "block": "f32+f64"}
     out = {
         "metric": METRIC,
         "value": round(T, 4),
@@ -237,6 +270,11 @@ def main():
                         f"{spec['nchol']} Cholesky vectors, seed {spec['seed']}; Davidson to "
                         f"|E0 W - H W| < {args.eps:g} (nb={args.nb}, start={args.init})",
             "solver": best,
+            "subspace": args.nb,
+            "restart_keep": args.keep if best in ("thick", "mixed") else None,
+            "spin_flip_symmetric": bool(args.symmetry and same_sets and best in ("thick", "mixed")),
+            "preconditioner": args.precond,
+            "precond_steps": res["precond_steps"],
             "sigma_products": res["steps"],
             "converged": res["converged"],
             "E0": res["E"],

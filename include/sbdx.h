@@ -80,6 +80,9 @@ int sbdx_sigma(sbdx_handle_t h, const double *Wk, double *Wb, int accumulate);
  * batched GEMMs and the mixed term as one multi-vector kernel per group of
  * up to four vectors, so the table and ERI-row traffic is shared. */
 int sbdx_sigma_block(sbdx_handle_t h, int k, const double *Wk, double *Wb, int accumulate);
+/* The same k products in FP32 arithmetic (the block solver's correction
+ * products in mixed precision); Wb is overwritten. */
+int sbdx_sigma_block_fp32(sbdx_handle_t h, int k, const double *Wk, double *Wb);
 /* Same with device pointers on the handle's stream (synchronous on return). */
 int sbdx_sigma_device(sbdx_handle_t h, const double *d_Wk, double *d_Wb, int accumulate);
 
@@ -151,13 +154,44 @@ int sbdx_set_restart(sbdx_handle_t h, int keep);
 int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_sparse,
                         float *ms_dense);
 
-/* Staging width of the mixed-term kernel chosen for precision 0 (FP64) or 1
- * (FP32) by timing widths 1, 2 and 4 (alpha singles staged in LDS per round)
- * on the first sigma (no counterpart in the reference): *width 0 before
- * that, -1 when no width fits in LDS (row parts or the CSR/global variants
- * run), else 1, 2 or 4; ms[3] the timings (0 for a width that did not fit
- * or was not timed).  SBDX_MIXED_EB=1|2|4 in the environment fixes it. */
-int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, float *ms);
+/* Configuration of the mixed-term kernel chosen for precision 0 (FP64) or 1
+ * (FP32) on the first sigma by timing every staging width (alpha singles
+ * staged in LDS per round: 1, 2, 4) and slots per thread (1, 2, 4, 8; a
+ * row's slots are split over blocks accordingly) that fit (no counterpart
+ * in the reference): *width 0 before that, -1 when no width fits in LDS
+ * (row parts or the CSR/global variants run), else 1, 2 or 4;
+ * *slots_per_thread, *ms the winner and its time; *multi whether block
+ * products use the multi-vector kernel (1) or per-vector launches (0),
+ * decided by timing on the first block product (-1 before).
+ * SBDX_MIXED_EB=1|2|4 and SBDX_MIXED_NT=1|2|4|8 fix the coordinates. */
+int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, int *slots_per_thread,
+                       float *ms, int *multi);
+
+/* The subspace eigensolver of the Davidson solves, on the host: eigenvalues
+ * E[n] ascending and orthonormal eigenvectors (columns of V, n x n
+ * column-major) of the symmetric n x n column-major A (not modified), by
+ * Householder tridiagonalization and implicit QR with Wilkinson shifts.
+ * Replaces the reference's LAPACK dsyev call (hp_numeric::MatHeev,
+ * framework/hp_numeric.h:503-522); needs no handle or GPU. */
+int sbdx_symmetric_eig(int n, const double *A, double *V, double *E);
+
+/* Spin-flip symmetry (no counterpart in the reference): with equal alpha and
+ * beta string sets, H commutes with the exchange of the two strings, and in
+ * the alpha-first basis of the device a vector of the symmetric sector is a
+ * symmetric na x na matrix (W[ia*nb+ib] = W[ib*nb+ia] after the basis sign
+ * of the interleaved convention).  With on != 0 the thick-restart solves
+ * (sbdx_set_restart > 0) compute sigma in that form: one same-spin GEMM
+ * G = H X and y = hii*x + G + G^T, and the mixed term for the upper
+ * triangle only, mirrored -- about half the work.  The start vector must be
+ * symmetric (the Hartree-Fock vector of BasisInitVector is); the solve
+ * then stays in the symmetric sector and returns its lowest state, which is
+ * what the reference's Davidson reaches from that start.  Errors with
+ * SBDX_ERR_ARG when the string sets differ or the start vector is not
+ * symmetric. */
+int sbdx_set_symmetry(sbdx_handle_t h, int on);
+/* sigma of a spin-flip symmetric vector (reference convention) in the
+ * symmetric form, FP64 (fp32 = 0) or FP32 arithmetic; Wb is overwritten. */
+int sbdx_sigma_sym(sbdx_handle_t h, const double *Wk, double *Wb, int fp32);
 
 /* verbose != 0: the solves print one line per step to stderr, step index,
  * E0 and |r| (as davidson.h:839-851 prints per step); thick-restart solves

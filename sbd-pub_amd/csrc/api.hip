@@ -212,6 +212,7 @@ static void destroy(Context *c) {
   (void)hipStreamSynchronize(c->stream);
   prof_clear(*c);
   same_dense_release(*c);
+  if (c->hpin) (void)hipHostFree(c->hpin);
   dfree_all(*c);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -324,7 +325,8 @@ int sbdx_sigma(sbdx_handle_t h, const double *Wk, double *Wb, int accumulate) {
   SBDX_GUARD_END
 }
 
-int sbdx_sigma_block(sbdx_handle_t h, int k, const double *Wk, double *Wb, int accumulate) {
+static int sigma_block_host(sbdx_handle_t h, int k, const double *Wk, double *Wb, int accumulate,
+                            bool fp32) {
   SBDX_GUARD_BEGIN
   Context &c = ctx_of(h);
   if (!Wk || !Wb) throw Error(SBDX_ERR_ARG, "null vector");
@@ -336,7 +338,10 @@ int sbdx_sigma_block(sbdx_handle_t h, int k, const double *Wk, double *Wb, int a
     upload(c, c.vout, Wk + (size_t)v * N);
     sign_convert(c, c.vout, x + (size_t)v * N);
   }
-  sigma_block(c, k, x, y);
+  if (fp32)
+    sigma_block_f32(c, k, x, y);
+  else
+    sigma_block(c, k, x, y);
   for (int v = 0; v < k; v++) {
     if (accumulate) {
       upload(c, c.vin, Wb + (size_t)v * N);
@@ -348,6 +353,14 @@ int sbdx_sigma_block(sbdx_handle_t h, int k, const double *Wk, double *Wb, int a
   }
   return SBDX_OK;
   SBDX_GUARD_END
+}
+
+int sbdx_sigma_block(sbdx_handle_t h, int k, const double *Wk, double *Wb, int accumulate) {
+  return sigma_block_host(h, k, Wk, Wb, accumulate, false);
+}
+
+int sbdx_sigma_block_fp32(sbdx_handle_t h, int k, const double *Wk, double *Wb) {
+  return sigma_block_host(h, k, Wk, Wb, 0, true);
 }
 
 int sbdx_davidson_block(sbdx_handle_t h, int nroots, int block, double *W, int max_it, double eps,
@@ -454,13 +467,54 @@ int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_spa
   SBDX_GUARD_END
 }
 
-int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, float *ms) {
+int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, int *slots_per_thread,
+                       float *ms, int *multi) {
   SBDX_GUARD_BEGIN
   Context &c = ctx_of(h);
   if (precision != 0 && precision != 1) throw Error(SBDX_ERR_ARG, "precision must be 0 or 1");
   if (width) *width = c.mixed_eb[precision];
-  if (ms)
-    for (int w = 0; w < 3; w++) ms[w] = c.mixed_eb_ms[precision][w];
+  if (slots_per_thread) *slots_per_thread = c.mixed_nt[precision];
+  if (ms) *ms = c.mixed_best_ms[precision];
+  if (multi) *multi = c.multi_mode[precision];
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_symmetric_eig(int n, const double *A, double *V, double *E) {
+  SBDX_GUARD_BEGIN
+  if (n < 1 || n > 4096) throw Error(SBDX_ERR_ARG, "n must be in [1, 4096]");
+  if (!A || !V || !E) throw Error(SBDX_ERR_ARG, "null argument");
+  std::vector<double> a(A, A + (size_t)n * n);
+  host_syev(n, a.data(), V, E);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_set_symmetry(sbdx_handle_t h, int on) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (on && !sigma_sym_available(c))
+    throw Error(SBDX_ERR_ARG, "spin-flip symmetry needs equal alpha and beta string sets (and memory for the "
+                              "dense same-spin block)");
+  c.symmetric = on ? 1 : 0;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_sigma_sym(sbdx_handle_t h, const double *Wk, double *Wb, int fp32) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!Wk || !Wb) throw Error(SBDX_ERR_ARG, "null vector");
+  if (!sigma_sym_available(c)) throw Error(SBDX_ERR_ARG, "spin-flip symmetry not available for this handle");
+  upload(c, c.vout, Wk);
+  sign_convert(c, c.vout, c.vin);
+  if (sym_defect(c, c.vin) > 1e-12) throw Error(SBDX_ERR_ARG, "vector is not spin-flip symmetric");
+  if (fp32)
+    sigma_sym_f32(c, c.vin, c.vout);
+  else
+    sigma_sym(c, c.vin, c.vout);
+  sign_convert(c, c.vout, c.vin);
+  download(c, Wb, c.vin);
   return SBDX_OK;
   SBDX_GUARD_END
 }

@@ -64,6 +64,12 @@ struct SpinTables {
   uint32_t *sl_len = nullptr;  // [nslice]
   int nslice = 0;
   bool has_ell = false;
+  // the same singles with slices in row order (identity perm), built on
+  // first use by the symmetric mixed kernel
+  uint32_t *nell = nullptr;
+  uint64_t *nsl_off = nullptr;
+  uint32_t *nsl_len = nullptr;
+  bool has_nell = false;
   // The same singles split by ket range into `nparts` parts of `part_h`
   // kets (ket stored relative to its part), same perm and slices; built on
   // demand when one ket row does not fit in LDS.
@@ -124,13 +130,18 @@ struct Context {
   // 0 sparse row kernels, 1 dense GEMM; the timings that decided it (ms).
   int same_mode[2] = {-1, -1};
   float same_ms[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
-  // Staging width (pairs per LDS round) of the sliced-ELL mixed kernel per
-  // precision: 0 not tuned, -1 none fits, else 1, 2 or 4; timings (ms) of
-  // widths 1, 2, 4 (0: did not fit).
+  // Sliced-ELL mixed kernel configuration per precision: staging width
+  // (pairs per LDS round; 0 not tuned, -1 none fits, else 1, 2 or 4), slots
+  // per thread, and the tuned time (ms).
   int mixed_eb[2] = {0, 0};
-  float mixed_eb_ms[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  int mixed_nt[2] = {0, 0};
+  float mixed_best_ms[2] = {0.f, 0.f};
+  // block products: 1 when the multi-vector kernel beat per-vector launches
+  // on first use, 0 when not, -1 not decided
+  int multi_mode[2] = {-1, -1};
   int precision = 0;       // 0: FP64 throughout; 1: FP32 sigma on corrections, FP64 restarts
   int verbose = 0;         // 1: one stderr line per Davidson step (E0, |r|)
+  int symmetric = 0;       // 1: thick-restart solves use the spin-flip symmetric sigma
   int restart_keep = 0;    // 0: restart from W alone (reference); k: thick restart keeping k Ritz vectors
   std::vector<double *> spareY, spareHY;  // thick-restart staging vectors
   // Block Davidson: contiguous basis and products (blk_m vectors each), the
@@ -151,6 +162,7 @@ struct Context {
   std::vector<double *> C, HC;
   double *W = nullptr, *R = nullptr;
   double *dsmall = nullptr;                 // small device matrices/scalars
+  double *hpin = nullptr;                   // pinned host staging for the subspace eigensolver
   std::vector<double> trace;
   bool have_vector = false;
   size_t lds_limit = 65536;
@@ -168,8 +180,9 @@ struct Context {
 // [2 nbmax^2, +nbmax)     E (eigenvalues ascending)
 // then scalars: norms, residual
 constexpr int kNbMax = 64;
-constexpr size_t kSmallDoubles = 2 * kNbMax * kNbMax + kNbMax + 64;
+constexpr size_t kSmallDoubles = 2 * kNbMax * kNbMax + kNbMax + 64 + kNbMax;
 constexpr int kScalarOffset = 2 * kNbMax * kNbMax + kNbMax;
+constexpr int kCoefOffset = kScalarOffset + 64;  // kNbMax projection coefficients
 
 // ---------------------------------------------------------------- helpers
 void *dalloc(Context &c, size_t bytes);
@@ -200,6 +213,7 @@ void check_launch(const char *what);
 void build_spin_tables(Context &c, SpinTables &t, const std::vector<uint64_t> &strs);
 void build_singles_ell(Context &c, SpinTables &t);
 void build_singles_ell_parts(Context &c, SpinTables &t, int nparts);
+void build_singles_ell_natural(Context &c, SpinTables &t);
 void compute_diag(Context &c);
 void sign_convert(Context &c, const double *in, double *out);      // out = S*in
 void sign_convert_add(Context &c, const double *in, double *out);  // out += S*in
@@ -210,8 +224,17 @@ void sigma_alpha_first_f32(Context &c, const double *x, double *y);  // same, FP
 void ensure_f32(Context &c);
 // y_v = H' x_v for k contiguous vectors (x + v*N), FP64 / FP32 arithmetic
 void sigma_block(Context &c, int k, const double *x, double *y);
+// spin-flip symmetric form (equal string sets, x = x^T as na x na): y = H' x
+bool sigma_sym_available(Context &c);
+void sigma_sym(Context &c, const double *x, double *y);
+void sigma_sym_f32(Context &c, const double *x, double *y);
+double sym_defect(Context &c, const double *v);  // max|v - v^T| / max|v|
+void symmetrize(Context &c, double *v);         // v = (v + v^T)/2
 void sigma_block_f32(Context &c, int k, const double *x, double *y);
 void same_dense_release(Context &c);  // frees the rocBLAS handle
+// eigenpairs of a symmetric n x n column-major A (overwritten), ascending:
+// E[n], eigenvectors in the columns of Q (n x n)
+void host_syev(int n, double *A, double *Q, double *E);
 void transpose_into(Context &c, const double *in, double *out, int R, int Cc);  // out = in^T
 
 // davidson.hip

@@ -7,8 +7,9 @@
 //
 // Every reduction is a fixed two-level tree (per-block partial sums in a
 // fixed grid, then a fixed-order sum), so results are reproducible run to
-// run.  The subspace eigensolve runs on the device (cyclic Jacobi, ascending
-// order), so only |R| and E0 travel to the host once per step.
+// run.  The subspace eigenproblem (at most 64 x 64, LAPACK dsyev in the
+// reference, framework/hp_numeric.h:503-522) is solved on the host by
+// Householder tridiagonalization and implicit QR, through pinned buffers.
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
@@ -72,68 +73,124 @@ __global__ void k_finish_hcol(int nv, int nblk, const double *__restrict__ part,
   H[col + (size_t)ld * k] = s;
 }
 
-// Cyclic Jacobi eigensolver for the leading n x n block of H (ld), one
-// thread; eigenvalues ascending into E, eigenvectors (columns) into U (ld).
-__global__ void k_syev(int n, int ld, const double *__restrict__ H, double *__restrict__ U,
-                       double *__restrict__ E) {
-  extern __shared__ double sm[];
-  if (threadIdx.x != 0) return;
-  double *A = sm;          // n x n
-  double *V = sm + n * n;  // n x n
+// ---- subspace eigensolver (host): eigenpairs of the leading n x n block of
+// the symmetric subspace matrix, ascending.  Householder reduction to
+// tridiagonal form with the reflectors accumulated into Q (Golub & Van Loan,
+// Matrix Computations, Alg. 8.3.1), then implicit symmetric QR steps with a
+// Wilkinson shift on the unreduced trailing block (Alg. 8.3.2-8.3.3), Givens
+// rotations applied to T and Q, deflating off-diagonals below
+// 1e-16 (|t_ii| + |t_i+1,i+1|).  O(n^3) for n <= 64: well under a
+// millisecond, where one thread on the device took ~100 ms at n = 40.
+// A, Q: n x n column-major (leading dimension n); E: n values.
+void host_syev(int n, double *A, double *Q, double *E) {
+  auto a = [&](int i, int j) -> double & { return A[i + (size_t)n * j]; };
+  auto q = [&](int i, int j) -> double & { return Q[i + (size_t)n * j]; };
   for (int j = 0; j < n; j++)
-    for (int i = 0; i < n; i++) {
-      A[i + j * n] = H[i + (size_t)j * ld];
-      V[i + j * n] = (i == j) ? 1.0 : 0.0;
+    for (int i = 0; i < n; i++) q(i, j) = i == j ? 1.0 : 0.0;
+  std::vector<double> v(n), p(n), w(n);
+  for (int k = 0; k + 2 < n; k++) {
+    // reflector P = I - beta v v^T with P x = alpha e1, x = A[k+1:, k]
+    double sig = 0.0;
+    for (int i = k + 2; i < n; i++) sig += a(i, k) * a(i, k);
+    const double x0 = a(k + 1, k);
+    if (sig == 0.0) continue;
+    const double mu = std::sqrt(x0 * x0 + sig);
+    const double v0 = x0 <= 0.0 ? x0 - mu : -sig / (x0 + mu);
+    const double beta = 2.0 * v0 * v0 / (sig + v0 * v0);
+    for (int i = 0; i < n; i++) v[i] = 0.0;
+    v[k + 1] = 1.0;
+    for (int i = k + 2; i < n; i++) v[i] = a(i, k) / v0;
+    // A <- P A P on rows/columns k+1.. (and column/row k)
+    for (int i = k; i < n; i++) {
+      double t = 0.0;
+      for (int j = k + 1; j < n; j++) t += a(i, j) * v[j];
+      p[i] = beta * t;
     }
-  for (int sweep = 0; sweep < 100; sweep++) {
-    double off = 0.0, tot = 0.0;
-    for (int j = 0; j < n; j++)
+    double pv = 0.0;
+    for (int i = k + 1; i < n; i++) pv += p[i] * v[i];
+    for (int i = k + 1; i < n; i++) w[i] = p[i] - 0.5 * beta * pv * v[i];
+    for (int j = k + 1; j < n; j++)
+      for (int i = k + 1; i < n; i++) a(i, j) -= v[i] * w[j] + w[i] * v[j];
+    // with this choice of v0 the reflector maps x to |x| e1 (Alg. 5.1.1)
+    a(k + 1, k) = a(k, k + 1) = mu;
+    for (int i = k + 2; i < n; i++) a(i, k) = a(k, i) = 0.0;
+    // Q <- Q P
+    for (int i = 0; i < n; i++) {
+      double t = 0.0;
+      for (int j = k + 1; j < n; j++) t += q(i, j) * v[j];
+      t *= beta;
+      for (int j = k + 1; j < n; j++) q(i, j) -= t * v[j];
+    }
+  }
+  // implicit symmetric QR on the tridiagonal T (held in A)
+  std::vector<double> d(n), e(n > 1 ? n - 1 : 1);
+  for (int i = 0; i < n; i++) d[i] = a(i, i);
+  for (int i = 0; i + 1 < n; i++) e[i] = a(i + 1, i);
+  const double tiny = 1e-16;
+  for (int iter = 0; iter < 60 * n; iter++) {
+    for (int i = 0; i + 1 < n; i++)
+      if (std::fabs(e[i]) <= tiny * (std::fabs(d[i]) + std::fabs(d[i + 1]))) e[i] = 0.0;
+    int hi = n - 1;
+    while (hi > 0 && e[hi - 1] == 0.0) hi--;
+    if (hi == 0) break;
+    int lo = hi - 1;
+    while (lo > 0 && e[lo - 1] != 0.0) lo--;
+    // Wilkinson shift from the trailing 2x2 of the block [lo, hi]
+    const double dd = 0.5 * (d[hi - 1] - d[hi]);
+    const double b2 = e[hi - 1] * e[hi - 1];
+    const double den = dd + (dd >= 0.0 ? 1.0 : -1.0) * std::sqrt(dd * dd + b2);
+    const double mu = den != 0.0 ? d[hi] - b2 / den : d[hi];
+    double x = d[lo] - mu, z = e[lo];
+    for (int k = lo; k < hi; k++) {
+      // Givens G = [c s; -s c] with G^T [x; z] = [r; 0]
+      const double r = std::hypot(x, z);
+      const double cs = r != 0.0 ? x / r : 1.0, sn = r != 0.0 ? -z / r : 0.0;
+      if (k > lo) e[k - 1] = r;
+      // T <- G^T T G on rows/columns k, k+1
+      const double dk = d[k], dk1 = d[k + 1], ek = e[k];
+      d[k] = cs * cs * dk - 2.0 * cs * sn * ek + sn * sn * dk1;
+      d[k + 1] = sn * sn * dk + 2.0 * cs * sn * ek + cs * cs * dk1;
+      e[k] = cs * sn * (dk - dk1) + (cs * cs - sn * sn) * ek;
+      if (k + 1 < hi) {
+        z = -sn * e[k + 1];
+        e[k + 1] *= cs;
+      }
+      x = e[k];
       for (int i = 0; i < n; i++) {
-        const double a = A[i + j * n];
-        tot += a * a;
-        if (i != j) off += a * a;
+        const double qk = q(i, k), qk1 = q(i, k + 1);
+        q(i, k) = cs * qk - sn * qk1;
+        q(i, k + 1) = sn * qk + cs * qk1;
       }
-    if (off <= 1e-32 * tot || off == 0.0) break;
-    for (int p = 0; p < n - 1; p++)
-      for (int q = p + 1; q < n; q++) {
-        const double apq = A[p + q * n];
-        if (apq == 0.0) continue;
-        const double app = A[p + p * n], aqq = A[q + q * n];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
-        for (int k = 0; k < n; k++) {
-          const double akp = A[k + p * n], akq = A[k + q * n];
-          A[k + p * n] = cs * akp - sn * akq;
-          A[k + q * n] = sn * akp + cs * akq;
-        }
-        for (int k = 0; k < n; k++) {
-          const double apk = A[p + k * n], aqk = A[q + k * n];
-          A[p + k * n] = cs * apk - sn * aqk;
-          A[q + k * n] = sn * apk + cs * aqk;
-        }
-        for (int k = 0; k < n; k++) {
-          const double vkp = V[k + p * n], vkq = V[k + q * n];
-          V[k + p * n] = cs * vkp - sn * vkq;
-          V[k + q * n] = sn * vkp + cs * vkq;
-        }
-      }
+    }
   }
-  // selection sort of the eigenpairs, ascending
-  int idx[kNbMax];
+  // ascending order
+  std::vector<int> idx(n);
   for (int i = 0; i < n; i++) idx[i] = i;
-  for (int i = 0; i < n; i++) {
-    int best = i;
-    for (int j = i + 1; j < n; j++)
-      if (A[idx[j] + idx[j] * n] < A[idx[best] + idx[best] * n]) best = j;
-    const int tmp = idx[i];
-    idx[i] = idx[best];
-    idx[best] = tmp;
-  }
+  std::stable_sort(idx.begin(), idx.end(), [&](int x1, int x2) { return d[x1] < d[x2]; });
+  std::vector<double> Qs((size_t)n * n);
   for (int j = 0; j < n; j++) {
-    E[j] = A[idx[j] + idx[j] * n];
-    for (int i = 0; i < n; i++) U[i + (size_t)j * ld] = V[i + idx[j] * n];
+    E[j] = d[idx[j]];
+    for (int i = 0; i < n; i++) Qs[i + (size_t)n * j] = q(i, idx[j]);
   }
+  std::copy(Qs.begin(), Qs.end(), Q);
+}
+
+// Eigenpairs of the leading n x n block of the device subspace matrix H
+// (column-major, ld): to the host (pinned staging), host_syev, U (ld) and E
+// back to the device.  Synchronous.
+static void subspace_eig(Context &c, int n, int ld, const double *H, double *U, double *E) {
+  if (!c.hpin) SBDX_HIP(hipHostMalloc((void **)&c.hpin, sizeof(double) * (3 * kNbMax * kNbMax + kNbMax)));
+  double *hA = c.hpin, *hQ = c.hpin + kNbMax * kNbMax, *hU = c.hpin + 2 * kNbMax * kNbMax;
+  double *hE = c.hpin + 3 * kNbMax * kNbMax;
+  SBDX_HIP(hipMemcpy2DAsync(hA, sizeof(double) * n, H, sizeof(double) * ld, sizeof(double) * n, n,
+                            hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  host_syev(n, hA, hQ, hE);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < ld; i++) hU[i + (size_t)ld * j] = i < n ? hQ[i + (size_t)n * j] : 0.0;
+  SBDX_HIP(hipMemcpyAsync(U, hU, sizeof(double) * ld * n, hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(E, hE, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
 }
 
 // W = sum_k U[k] C_k ; R = -sum_k U[k] HC_k + E0*W ; partial |W|^2, |R|^2.
@@ -260,6 +317,30 @@ double device_dot(Context &c, const double *x, const double *y, size_t n) {
   return r;
 }
 
+// out[k] = fixed-order sum over blocks of part[blk*nv + k]
+__global__ void k_finish_vec(int nv, int nblk, const double *__restrict__ part, double *__restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= nv) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; b++) s += part[(size_t)b * nv + k];
+  out[k] = s;
+}
+
+// t -= sum_k coef[k] X_k (one classical Gram-Schmidt pass with coefficients
+// already reduced)
+__global__ void __launch_bounds__(kRedThreads) k_sub_combo(VecList X, int nv, const double *__restrict__ coef,
+                                                           double *__restrict__ t, size_t n) {
+  __shared__ double cf[kNbMax];
+  if (threadIdx.x < nv) cf[threadIdx.x] = coef[threadIdx.x];
+  __syncthreads();
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double acc = t[i];
+    for (int k = 0; k < nv; k++) acc -= cf[k] * X.p[k][i];
+    t[i] = acc;
+  }
+}
+
 // out_j = sum_k U[k + ld*(j0+j)] in_k for j < nout (Ritz vectors and their
 // H-products from the subspace basis)
 __global__ void __launch_bounds__(kRedThreads) k_combine(VecList in, int nin, const double *__restrict__ U,
@@ -357,9 +438,7 @@ int davidson_solve(Context &c, int max_it, int nb, double eps, double max_time, 
         hipLaunchKernelGGL(k_finish_hcol, dim3(1), dim3(kNbMax), 0, c.stream, ib + 1, kRedBlocks,
                            c.part, H, ld, ib);
         check_launch("k_finish_hcol");
-        const size_t shm = sizeof(double) * 2 * (ib + 1) * (ib + 1);
-        hipLaunchKernelGGL(k_syev, dim3(1), dim3(64), shm, c.stream, ib + 1, ld, H, U, E);
-        check_launch("k_syev");
+        subspace_eig(c, ib + 1, ld, H, U, E);
       }
       {
         ProfScope ps(c, "dav_ritz");
@@ -473,8 +552,24 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
                        H, ld, col);
     check_launch("k_finish_hcol");
   };
+  // spin-flip symmetric products when asked for: the start vector must be
+  // symmetric (x = x^T in the alpha-first layout); the diagonal step keeps
+  // every later vector symmetric, the learned step is symmetrized
+  const bool sym = c.symmetric != 0;
+  if (sym) {
+    if (!sigma_sym_available(c))
+      throw Error(1, "symmetric mode needs equal alpha and beta string sets and room for the dense blocks");
+    const double defect = sym_defect(c, c.W);
+    if (defect > 1e-12)
+      throw Error(1, "symmetric mode: the start vector is not spin-flip symmetric (defect " +
+                         std::to_string(defect) + ")");
+  }
   auto apply = [&](const double *x, double *y, bool approx) {
-    if (approx)
+    if (sym && approx)
+      sigma_sym_f32(c, x, y);
+    else if (sym)
+      sigma_sym(c, x, y);
+    else if (approx)
       sigma_alpha_first_f32(c, x, y);
     else
       sigma_alpha_first(c, x, y);
@@ -500,9 +595,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
   while (true) {
     {
       ProfScope ps(c, "dav_subspace");
-      const size_t shm = sizeof(double) * 2 * (d + 1) * (d + 1);
-      hipLaunchKernelGGL(k_syev, dim3(1), dim3(64), shm, c.stream, d + 1, ld, H, U, E);
-      check_launch("k_syev");
+      subspace_eig(c, d + 1, ld, H, U, E);
     }
     {
       ProfScope ps(c, "dav_ritz");
@@ -619,6 +712,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       double *cn = c.C[d + 1];
       if (use_mlp) {
         mlp_precond(c, c.R, E, cn);
+        if (sym) symmetrize(c, cn);
         c.mlp.used_mlp++;
       } else {
         hipLaunchKernelGGL(k_precond, g, b, 0, c.stream, c.R, c.hii, E, N, eps_reg, cn);
@@ -627,13 +721,19 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       }
     }
     {
+      // classical Gram-Schmidt twice against the d+1 basis vectors: one
+      // multi-dot and one combined update per pass
       ProfScope ps(c, "dav_orth");
       double *cn = c.C[d + 1];
-      for (int kb = 0; kb <= d; kb++) {
-        const double *X[1] = {c.C[kb]};
-        dots(c, X, 1, cn, c.part);
-        hipLaunchKernelGGL(k_axpy_part, g, b, 0, c.stream, cn, c.C[kb], N, c.part, kRedBlocks);
-        check_launch("k_axpy_part");
+      double *coef = c.dsmall + kCoefOffset;
+      VecList Cv{};
+      for (int k = 0; k <= d; k++) Cv.p[k] = c.C[k];
+      for (int pass = 0; pass < 2; pass++) {
+        dots(c, c.C.data(), d + 1, cn, c.part);
+        hipLaunchKernelGGL(k_finish_vec, dim3(1), dim3(kNbMax), 0, c.stream, d + 1, kRedBlocks, c.part, coef);
+        check_launch("k_finish_vec");
+        hipLaunchKernelGGL(k_sub_combo, g, b, 0, c.stream, Cv, d + 1, coef, cn, N);
+        check_launch("k_sub_combo");
       }
       const double *X[1] = {cn};
       dots(c, X, 1, cn, c.part);
@@ -678,15 +778,6 @@ __global__ void __launch_bounds__(kRedThreads) k_block_resid(int b, size_t n, co
   }
 }
 
-// out[k] = fixed-order sum over blocks of part[blk*nv + k]
-__global__ void k_finish_vec(int nv, int nblk, const double *__restrict__ part, double *__restrict__ out) {
-  const int k = threadIdx.x;
-  if (k >= nv) return;
-  double s = 0.0;
-  for (int b = 0; b < nblk; b++) s += part[(size_t)b * nv + k];
-  out[k] = s;
-}
-
 // Columns j in [j0, j1) of the subspace matrix hold HC_i . C_j for every row
 // i; keep the entries with i <= j (the older vector's product) and mirror
 // them: H[j][i] = H[i][j] for i < j.
@@ -699,26 +790,35 @@ __global__ void k_symm_cols(double *H, int ld, int j0, int j1) {
 // Block Davidson (no counterpart in the reference, whose solver is
 // single-root): the lowest nroots eigenpairs with `block` >= nroots
 // correction vectors per iteration, all pushed through one block sigma
-// (sigma_block: batched GEMMs and the multi-vector mixed kernel).
+// (sigma_block: batched GEMMs and the mixed kernels).
 //   basis C (contiguous, up to m vectors) and products HC, subspace matrix
 //   H[i][j] = HC_i . C_j (i <= j) by GEMM, Rayleigh-Ritz on the device;
 //   Ritz block W = C U, HW = HC U, R = HW - E W (GEMMs + one pass);
 //   corrections: preconditioned residuals of the lowest `block` Ritz pairs
 //   (diagonal, or the MLP with the stall fallback of the thick solver),
 //   classical Gram-Schmidt twice against the basis by GEMM, then modified
-//   Gram-Schmidt inside the block; vectors whose norm falls below 1e-10
-//   after projection are dropped;
-//   restart when the next block would not fit: the basis becomes W (its
-//   product HW by combination; in mixed mode an exact FP64 block product),
-//   so convergence (|R_j| < eps for j < nroots) is decided on exact FP64
-//   products in both modes.
+//   Gram-Schmidt inside the block; a direction whose norm falls below 1e-10
+//   of its norm before projection is dropped;
+//   thick restart when the next block would not fit: the basis becomes the
+//   lowest `keep` Ritz vectors with their products by combination; in mixed
+//   mode the first nroots of them get exact FP64 products, so they are the
+//   older index in every coupling (H[i][j] = HC_i . C_j) and the roots'
+//   Rayleigh quotients use exact products;
+//   convergence: |R_j| < eps for j < nroots on a basis whose root products
+//   are exact, confirmed in mixed mode by exact FP64 products of the final
+//   vectors.
+// m = min(64, max(4*block, 16)), reduced by `block` steps while the basis,
+// products and Ritz buffers would take more than 80% of free memory.
 // W0/Wout: device, nroots vectors in the alpha-first basis.
 int davidson_block(Context &c, int nroots, int block, int max_it, double eps, double max_time,
                    bool mixed, const double *W0, double *Wout, double *Eout, int *n_products) {
   if (nroots < 1 || block < nroots || block > 16) throw Error(1, "davidson_block: need 1 <= nroots <= block <= 16");
-  const int m = std::min(kNbMax, std::max(2 * block, 4 * block));
   const size_t N = c.N;
-  if (c.blk_m < m || c.blk_b < block) {
+  const int keep_want = std::max(2 * block, nroots + block);
+  int m = std::min(kNbMax, std::max(4 * block, 16));
+  m = std::max(m, keep_want + block);
+  if (m > kNbMax) throw Error(1, "davidson_block: block too large for the basis limit");
+  if (c.blk_m < m || c.blk_b < keep_want) {
     dfree(c, c.blk_C);
     dfree(c, c.blk_HC);
     dfree(c, c.blk_W);
@@ -726,14 +826,22 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
     dfree(c, c.blk_R);
     c.blk_C = c.blk_HC = c.blk_W = c.blk_HW = c.blk_R = nullptr;
     c.blk_m = c.blk_b = 0;
+    size_t free_b = 0, total_b = 0;
+    SBDX_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double budget = 0.8 * (double)free_b;
+    auto need = [&](int mm, int kk) { return (double)sizeof(double) * N * (2.0 * mm + 2.0 * kk + block); };
+    while (need(m, keep_want) > budget && m - block >= keep_want + block) m -= block;
+    if (need(m, keep_want) > budget) throw Error(2, "davidson_block: basis does not fit in device memory");
     c.blk_C = (double *)dalloc(c, sizeof(double) * N * m);
     c.blk_HC = (double *)dalloc(c, sizeof(double) * N * m);
-    c.blk_W = (double *)dalloc(c, sizeof(double) * N * block);
-    c.blk_HW = (double *)dalloc(c, sizeof(double) * N * block);
+    c.blk_W = (double *)dalloc(c, sizeof(double) * N * keep_want);
+    c.blk_HW = (double *)dalloc(c, sizeof(double) * N * keep_want);
     c.blk_R = (double *)dalloc(c, sizeof(double) * N * block);
     c.blk_m = m;
-    c.blk_b = block;
+    c.blk_b = keep_want;
   }
+  m = std::min(m, c.blk_m);
+  const int keep = std::min(keep_want, m - block);
   if (!c.blk_small) c.blk_small = (double *)dalloc(c, sizeof(double) * kNbMax * kNbMax * 2);
   rocblas_handle hb = (rocblas_handle)blas_handle(c);
   const int ld = kNbMax;
@@ -764,9 +872,15 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
           "CGS update");
     }
   };
-  // MGS of C[d..d+nv) among themselves; keeps those with norm > tol, packed
-  // at C[d..); returns how many were kept
-  auto mgs_pack = [&](int d, int nv) {
+  // norms^2 of the nv vectors at base (device nrm[0..nv)), to the host
+  std::vector<double> hn(kNbMax);
+  auto norms2 = [&](const double *base, int nv) {
+    for (int j = 0; j < nv; j++) hn[j] = device_dot(c, base + (size_t)j * N, base + (size_t)j * N, N);
+  };
+  // MGS of C[d..d+nv) among themselves; keeps those whose norm stays above
+  // 1e-10 of ref[j] (norm before the projections), packed at C[d..);
+  // returns how many were kept
+  auto mgs_pack = [&](int d, int nv, const std::vector<double> &ref) {
     int kept = 0;
     for (int j = 0; j < nv; j++) {
       double *v = col(C, d + j);
@@ -777,7 +891,7 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
         check_launch("k_axpy_part");
       }
       const double nv2 = device_dot(c, v, v, N);
-      if (!(nv2 > 1e-20)) continue;
+      if (!(nv2 > 1e-20 * ref[j]) || !(nv2 > 0.0)) continue;
       double *dst = col(C, d + kept);
       if (dst != v)
         SBDX_HIP(hipMemcpyAsync(dst, v, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
@@ -789,21 +903,43 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
     }
     return kept;
   };
-  // products of C[d..d+nv) and the new subspace columns
-  auto expand = [&](int d, int nv, bool approx) {
-    {
-      ProfScope ps(c, approx ? "blk_sigma_f32" : "blk_sigma");
-      if (approx)
-        sigma_block_f32(c, nv, col(C, d), col(HC, d));
-      else
-        sigma_block(c, nv, col(C, d), col(HC, d));
-    }
+  auto product = [&](double *x, double *y, int nv, bool approx) {
+    ProfScope ps(c, approx ? "blk_sigma_f32" : "blk_sigma");
+    if (approx)
+      sigma_block_f32(c, nv, x, y);
+    else
+      sigma_block(c, nv, x, y);
+  };
+  // subspace columns [d0, d1): H[i][j] = HC_i . C_j for i <= j
+  auto columns = [&](int d0, int d1) {
     ProfScope ps(c, "blk_subspace");
-    chk(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, d + nv, nv, (int)N, &one,
-                      HC, (int)N, col(C, d), (int)N, &zero, H + (size_t)ld * d, ld),
+    chk(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, d1, d1 - d0, (int)N, &one,
+                      HC, (int)N, col(C, d0), (int)N, &zero, H + (size_t)ld * d0, ld),
         "subspace columns");
-    hipLaunchKernelGGL(k_symm_cols, dim3(nv), dim3(kNbMax), 0, c.stream, H, ld, d, d + nv);
+    hipLaunchKernelGGL(k_symm_cols, dim3(d1 - d0), dim3(kNbMax), 0, c.stream, H, ld, d0, d1);
     check_launch("k_symm_cols");
+  };
+  auto ritz = [&](int d, int nv, double *Wd, double *HWd) {
+    subspace_eig(c, d, ld, H, U, E);
+    chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nv, d, &one, C, (int)N, U,
+                      ld, &zero, Wd, (int)N),
+        "Ritz vectors");
+    if (HWd)
+      chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nv, d, &one, HC, (int)N,
+                        U, ld, &zero, HWd, (int)N),
+          "Ritz products");
+  };
+  // residual norms of nv Ritz pairs (R = HW - E W), to the host
+  std::vector<double> hE(kNbMax), hR(kNbMax);
+  auto residuals = [&](int nv, const double *Wd, const double *HWd, const double *Ed) {
+    hipLaunchKernelGGL(k_block_resid, g, bthr, 0, c.stream, nv, N, Wd, HWd, Ed, c.blk_R, c.part);
+    check_launch("k_block_resid");
+    hipLaunchKernelGGL(k_finish_vec, dim3(1), dim3(kNbMax), 0, c.stream, nv, kRedBlocks, c.part, nrm);
+    check_launch("k_finish_vec");
+    SBDX_HIP(hipMemcpyAsync(hE.data(), Ed, sizeof(double) * nv, hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipMemcpyAsync(hR.data(), nrm, sizeof(double) * nv, hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+    for (int j = 0; j < nv; j++) hR[j] = std::sqrt(hR[j]);
   };
 
   // initial block: the given vectors, then unit vectors on the lowest
@@ -825,14 +961,15 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
                               c.stream));
     SBDX_HIP(hipStreamSynchronize(c.stream));
   }
-  int d = mgs_pack(0, block);
+  norms2(C, block);
+  int d = mgs_pack(0, block, hn);
   if (d < nroots) throw Error(1, "davidson_block: start vectors are linearly dependent");
-  expand(0, d, false);
+  product(C, HC, d, false);
+  columns(0, d);
   int products = d;
   bool done = false, exact_now = true;
   c.trace.clear();
   c.blk_E.assign(nroots, 0.0);
-  std::vector<double> hE(block), hR(block);
   std::vector<double> rhist;
   int diag_left = 0;
   c.mlp.used_mlp = c.mlp.used_diag = 0;
@@ -840,26 +977,11 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
     const int nr = std::min(block, d);
     {
       ProfScope ps(c, "blk_ritz");
-      const size_t shm = sizeof(double) * 2 * d * d;
-      hipLaunchKernelGGL(k_syev, dim3(1), dim3(64), shm, c.stream, d, ld, H, U, E);
-      check_launch("k_syev");
-      chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nr, d, &one, C, (int)N, U,
-                        ld, &zero, c.blk_W, (int)N),
-          "Ritz vectors");
-      chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nr, d, &one, HC, (int)N,
-                        U, ld, &zero, c.blk_HW, (int)N),
-          "Ritz products");
-      hipLaunchKernelGGL(k_block_resid, g, bthr, 0, c.stream, nr, N, c.blk_W, c.blk_HW, E, c.blk_R, c.part);
-      check_launch("k_block_resid");
-      hipLaunchKernelGGL(k_finish_vec, dim3(1), dim3(kNbMax), 0, c.stream, nr, kRedBlocks, c.part, nrm);
-      check_launch("k_finish_vec");
+      ritz(d, nr, c.blk_W, c.blk_HW);
+      residuals(nr, c.blk_W, c.blk_HW, E);
     }
-    SBDX_HIP(hipMemcpyAsync(hE.data(), E, sizeof(double) * nr, hipMemcpyDeviceToHost, c.stream));
-    SBDX_HIP(hipMemcpyAsync(hR.data(), nrm, sizeof(double) * nr, hipMemcpyDeviceToHost, c.stream));
-    SBDX_HIP(hipStreamSynchronize(c.stream));
     double rmax = 0.0;
     for (int j = 0; j < nroots; j++) {
-      hR[j] = std::sqrt(hR[j]);
       rmax = std::max(rmax, hR[j]);
       c.blk_E[j] = hE[j];
     }
@@ -871,36 +993,54 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
       fflush(stderr);
     }
     if (rmax < eps && exact_now) {
-      done = true;
-      break;
+      if (!mixed) {
+        done = true;
+        break;
+      }
+      // confirm on exact products of the final root vectors
+      ProfScope ps(c, "blk_confirm");
+      product(c.blk_W, c.blk_HW, nroots, false);
+      products += nroots;
+      double *Ec = nrm + kNbMax;  // Rayleigh quotients of the final vectors
+      for (int j = 0; j < nroots; j++) {
+        const double *X[1] = {col(c.blk_W, j)};
+        dots(c, X, 1, col(c.blk_HW, j), c.part);
+        hipLaunchKernelGGL(k_finish_scalar, dim3(1), dim3(1), 0, c.stream, kRedBlocks, c.part, Ec + j);
+        check_launch("k_finish_scalar");
+      }
+      residuals(nroots, c.blk_W, c.blk_HW, Ec);
+      double rx = 0.0;
+      for (int j = 0; j < nroots; j++) rx = std::max(rx, hR[j]);
+      if (c.verbose) {
+        fprintf(stderr, "[sbdx] block confirm E0 = %.15f exact max|r| = %.3e\n", hE[0], rx);
+        fflush(stderr);
+      }
+      if (rx < eps) {
+        for (int j = 0; j < nroots; j++) c.blk_E[j] = hE[j];
+        done = true;
+        break;
+      }
+      exact_now = false;  // not yet: force a restart below
+      rmax = rx;
     }
     const bool out_of_time = elapsed() > max_time || it >= max_it;
     if (d + block > m || (rmax < eps && !exact_now) || out_of_time) {
-      // restart on the Ritz block (exact products in mixed mode)
+      // thick restart on the lowest nk Ritz vectors
       ProfScope ps(c, "blk_restart");
-      SBDX_HIP(hipMemcpyAsync(C, c.blk_W, sizeof(double) * N * nr, hipMemcpyDeviceToDevice, c.stream));
+      const int nk = out_of_time ? nr : std::min(keep, d);
+      ritz(d, nk, c.blk_W, c.blk_HW);
+      SBDX_HIP(hipMemcpyAsync(C, c.blk_W, sizeof(double) * N * nk, hipMemcpyDeviceToDevice, c.stream));
+      SBDX_HIP(hipMemcpyAsync(HC, c.blk_HW, sizeof(double) * N * nk, hipMemcpyDeviceToDevice, c.stream));
       if (mixed) {
-        d = 0;
-        expand(0, nr, false);
-        products += nr;
-      } else {
-        SBDX_HIP(hipMemcpyAsync(HC, c.blk_HW, sizeof(double) * N * nr, hipMemcpyDeviceToDevice, c.stream));
-        chk(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, nr, nr, (int)N, &one, HC,
-                          (int)N, C, (int)N, &zero, H, ld),
-            "restart subspace");
-        hipLaunchKernelGGL(k_symm_cols, dim3(nr), dim3(kNbMax), 0, c.stream, H, ld, 0, nr);
-        check_launch("k_symm_cols");
+        product(C, HC, std::min(nroots, nk), false);
+        products += std::min(nroots, nk);
       }
-      d = nr;
+      d = nk;
+      columns(0, d);
       exact_now = true;
       if (out_of_time) {
-        // one more Rayleigh-Ritz on the exact block for the outputs
-        const size_t shm = sizeof(double) * 2 * d * d;
-        hipLaunchKernelGGL(k_syev, dim3(1), dim3(64), shm, c.stream, d, ld, H, U, E);
-        check_launch("k_syev");
-        chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nr, d, &one, C, (int)N,
-                          U, ld, &zero, c.blk_W, (int)N),
-            "Ritz vectors");
+        // one more Rayleigh-Ritz on the restarted basis for the outputs
+        ritz(d, nroots, c.blk_W, nullptr);
         SBDX_HIP(hipMemcpyAsync(hE.data(), E, sizeof(double) * nroots, hipMemcpyDeviceToHost, c.stream));
         SBDX_HIP(hipStreamSynchronize(c.stream));
         for (int j = 0; j < nroots; j++) c.blk_E[j] = hE[j];
@@ -931,11 +1071,13 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
     int nv;
     {
       ProfScope ps(c, "blk_orth");
+      norms2(col(C, d), nr);
       cgs(d, nr);
-      nv = mgs_pack(d, nr);
+      nv = mgs_pack(d, nr, hn);
     }
     if (nv == 0) throw Error(5, "davidson_block: no new direction (basis exhausted)");
-    expand(d, nv, mixed);
+    product(col(C, d), col(HC, d), nv, mixed);
+    columns(d, d + nv);
     products += nv;
     d += nv;
     exact_now = !mixed;

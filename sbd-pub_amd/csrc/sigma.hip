@@ -132,8 +132,15 @@ __global__ void __launch_bounds__(256) k_transpose(const T *__restrict__ in, T *
 // uniform per wavefront and each entry load is a coalesced 256-byte access
 // shared by the EB staged pairs.  Per pair the slot also adds the alpha
 // single's dressing x[J,r] * sum_{k in beta(r)} s_a (pq|kk) from the staged
-// ERI row's diagonal (zero in the spectator row).
-template <typename T, int NT, int EB>
+// ERI row's diagonal (zero in the spectator row).  A row's slots may be
+// split over gridDim.y blocks of 1024*NT slots each (fewer registers per
+// thread, more resident waves, the row staged once per block).
+// SYM: the table is in row order (slot = beta row) and only rows r >= I are
+// produced, added at y[I,r] and mirrored at y[r,I] (a spin-flip symmetric
+// vector gives a symmetric product; element (a,b) belongs to block min(a,b),
+// so no two blocks write the same element); blocks start at the slice of
+// row I and exit when their chunk lies past the table.
+template <typename T, int NT, int EB, bool SYM>
 __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int nslots,
                                                     const uint64_t *__restrict__ a_off,
                                                     const uint64_t *__restrict__ a_ent,
@@ -150,7 +157,9 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
   const int ldr = nb + K + 1;
   const int L1 = L + 1;
   const int I = blockIdx.x;
-  const int tid = threadIdx.x;
+  const int base = (SYM ? (I & ~63) : 0) + blockIdx.y * (1024 * NT);
+  if (SYM && base >= nslots) return;  // uniform per block, before any barrier
+  const int tid = base + threadIdx.x;
   T acc[NT];
   uint64_t occ[NT];
   int rr[NT];
@@ -158,7 +167,8 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
   for (int t = 0; t < NT; t++) {
     acc[t] = T(0);
     const int slot = tid + t * 1024;
-    const uint32_t r = slot < nslots ? perm[slot] : 0xffffffffu;
+    uint32_t r = 0xffffffffu;
+    if (slot < nslots) r = SYM ? (slot < nb ? (uint32_t)slot : 0xffffffffu) : perm[slot];
     rr[t] = r != 0xffffffffu ? (int)r : 0;
     occ[t] = r != 0xffffffffu ? b_str[r] : 0ull;  // padding slots: no dressing
   }
@@ -176,15 +186,15 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
         const size_t J = ent_ket(ent);
         const T sa = ent_neg(ent) ? T(-1) : T(1);
         const T *vsrc = eri + (size_t)ent_pq(ent) * K;
-        for (int i = tid; i < nb; i += 1024) crow[i] = x[J * nb + i];
-        for (int i = tid; i < K; i += 1024) vrow[i] = sa * vsrc[i];
-        if (tid == 0) vrow[K] = T(0);
+        for (int i = threadIdx.x; i < nb; i += 1024) crow[i] = x[J * nb + i];
+        for (int i = threadIdx.x; i < K; i += 1024) vrow[i] = sa * vsrc[i];
+        if (threadIdx.x == 0) vrow[K] = T(0);
       } else if (k < ne) {
-        for (int i = tid; i < nb; i += 1024) crow[i] = x[(size_t)I * nb + i];
-        for (int i = tid; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
-        if (tid == 0) vrow[K] = T(0);
+        for (int i = threadIdx.x; i < nb; i += 1024) crow[i] = x[(size_t)I * nb + i];
+        for (int i = threadIdx.x; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
+        if (threadIdx.x == 0) vrow[K] = T(0);
       } else {
-        for (int i = tid; i < ldr; i += 1024) crow[i] = T(0);
+        for (int i = threadIdx.x; i < ldr; i += 1024) crow[i] = T(0);
       }
     }
     __syncthreads();
@@ -217,8 +227,15 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
   for (int t = 0; t < NT; t++) {
     const int slot = tid + t * 1024;
     if (slot < nslots) {
-      const uint32_t r = perm[slot];
-      if (r != 0xffffffffu) y[(size_t)I * nb + r] += acc[t];
+      if (SYM) {
+        if (slot < nb && slot >= I) {
+          y[(size_t)I * nb + slot] += acc[t];
+          if (slot > I) y[(size_t)slot * nb + I] += acc[t];
+        }
+      } else {
+        const uint32_t r = perm[slot];
+        if (r != 0xffffffffu) y[(size_t)I * nb + r] += acc[t];
+      }
     }
   }
 }
@@ -244,7 +261,7 @@ __global__ void __launch_bounds__(1024) k_mixed_multi(int nb, int K, int L, int 
   T *crow = vrow + K + 1;                    // KV rows of nb
   const int L1 = L + 1;
   const int I = blockIdx.x;
-  const int tid = threadIdx.x;
+  const int tid = blockIdx.y * (1024 * NT) + threadIdx.x;
   T acc[KV][NT];
   uint64_t occ[NT];
   int rr[NT];
@@ -267,14 +284,14 @@ __global__ void __launch_bounds__(1024) k_mixed_multi(int nb, int K, int L, int 
       J = ent_ket(ent);
       const T sa = ent_neg(ent) ? T(-1) : T(1);
       const T *vsrc = eri + (size_t)ent_pq(ent) * K;
-      for (int i = tid; i < K; i += 1024) vrow[i] = sa * vsrc[i];
+      for (int i = threadIdx.x; i < K; i += 1024) vrow[i] = sa * vsrc[i];
     } else {
-      for (int i = tid; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
+      for (int i = threadIdx.x; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
     }
-    if (tid == 0) vrow[K] = T(0);
+    if (threadIdx.x == 0) vrow[K] = T(0);
 #pragma unroll
     for (int v = 0; v < KV; v++)
-      for (int i = tid; i < nb; i += 1024) crow[v * nb + i] = x[v * N + J * nb + i];
+      for (int i = threadIdx.x; i < nb; i += 1024) crow[v * nb + i] = x[v * N + J * nb + i];
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < NT; t++) {
@@ -532,6 +549,80 @@ __global__ void __launch_bounds__(256) k_densify(int n, const uint64_t *__restri
   }
 }
 
+// Spin-flip symmetric form of the same-spin terms for equal string sets:
+// y = hii*x + G + G^T with G = H X (n x n row-major), 64x64 tiles, G^T
+// through LDS.
+template <typename T>
+__global__ void __launch_bounds__(256) k_sym_combine(int n, const T *__restrict__ hii, const T *__restrict__ x,
+                                                     const T *__restrict__ G, T *__restrict__ y) {
+  __shared__ T tile[64][65];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+  for (int k = ty; k < 64; k += 4) {
+    const int gr = c0 + k, gc = r0 + tx;
+    if (gr < n && gc < n) tile[k][tx] = G[(size_t)gr * n + gc];
+  }
+  __syncthreads();
+  for (int k = ty; k < 64; k += 4) {
+    const int r = r0 + k, cc = c0 + tx;
+    if (r < n && cc < n) {
+      const size_t o = (size_t)r * n + cc;
+      y[o] = hii[o] * x[o] + G[o] + tile[tx][k];
+    }
+  }
+}
+
+// v = (v + v^T)/2 on an n x n row-major matrix; block (bx, by), bx <= by,
+// owns the tile pair (by, bx) / (bx, by).
+__global__ void __launch_bounds__(256) k_symmetrize(int n, double *__restrict__ v) {
+  if (blockIdx.x > blockIdx.y) return;
+  __shared__ double A[64][65], B[64][65];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int R0 = blockIdx.y * 64, C0 = blockIdx.x * 64;
+  for (int k = ty; k < 64; k += 4) {
+    A[k][tx] = (R0 + k < n && C0 + tx < n) ? v[(size_t)(R0 + k) * n + C0 + tx] : 0.0;
+    B[k][tx] = (C0 + k < n && R0 + tx < n) ? v[(size_t)(C0 + k) * n + R0 + tx] : 0.0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 64; k += 4) {
+    if (R0 + k < n && C0 + tx < n) v[(size_t)(R0 + k) * n + C0 + tx] = 0.5 * (A[k][tx] + B[tx][k]);
+    if (C0 + k < n && R0 + tx < n) v[(size_t)(C0 + k) * n + R0 + tx] = 0.5 * (B[k][tx] + A[tx][k]);
+  }
+}
+
+// part[2*b] = max |v - v^T| and part[2*b+1] = max |v| over tile pair b.
+__global__ void __launch_bounds__(256) k_sym_defect(int n, const double *__restrict__ v, double *__restrict__ part) {
+  __shared__ double A[64][65];
+  __shared__ double red[2][256];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int R0 = blockIdx.y * 64, C0 = blockIdx.x * 64;
+  for (int k = ty; k < 64; k += 4)
+    A[k][tx] = (C0 + k < n && R0 + tx < n) ? v[(size_t)(C0 + k) * n + R0 + tx] : 0.0;
+  __syncthreads();
+  double d = 0.0, m = 0.0;
+  for (int k = ty; k < 64; k += 4)
+    if (R0 + k < n && C0 + tx < n) {
+      const double a = v[(size_t)(R0 + k) * n + C0 + tx];
+      d = fmax(d, fabs(a - A[tx][k]));
+      m = fmax(m, fabs(a));
+    }
+  red[0][threadIdx.x] = d;
+  red[1][threadIdx.x] = m;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + st]);
+      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+    part[2 * b] = red[0][0];
+    part[2 * b + 1] = red[1][0];
+  }
+}
+
 // y = hii * x (the diagonal term of the GEMM form).
 template <typename T>
 __global__ void __launch_bounds__(256) k_diag_mul(size_t n, const T *__restrict__ hii,
@@ -585,26 +676,33 @@ void transpose_into(Context &c, const double *in, double *out, int R, int Cc) {
   transpose<double>(c, in, out, R, Cc, false);
 }
 
-template <typename T, int NT, int EB>
+// Rows of the mixed term are split over ceil(slots / (1024*NT)) blocks.
+template <typename T, int NT, int EB, bool SYM>
 static void launch_mixed_ell(Context &c, const T *eri, const T *x, T *y, size_t lds) {
-  auto fn = k_mixed_ell<T, NT, EB>;
+  auto fn = k_mixed_ell<T, NT, EB, SYM>;
   SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
-                     c.B.nslice * 64, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell, c.B.perm,
-                     c.B.sl_off, c.B.sl_len, eri, x, y);
+  const int nslots = c.B.nslice * 64;
+  const unsigned nsplit = cdiv(nslots, 1024 * NT);
+  if (SYM)
+    hipLaunchKernelGGL(fn, dim3((unsigned)c.na, nsplit), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
+                       nslots, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.nell, (const uint32_t *)nullptr,
+                       c.B.nsl_off, c.B.nsl_len, eri, x, y);
+  else
+    hipLaunchKernelGGL(fn, dim3((unsigned)c.na, nsplit), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
+                       nslots, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell, c.B.perm, c.B.sl_off,
+                       c.B.sl_len, eri, x, y);
 }
 
-template <typename T, int EB>
-static bool mixed_ell_dispatch(Context &c, const T *eri, const T *x, T *y) {
+// EB pairs staged per round, NT slots per thread (1, 2, 4 or 8); false when
+// the staging does not fit in LDS.
+template <typename T, int EB, bool SYM = false>
+static bool mixed_ell_dispatch(Context &c, int nt, const T *eri, const T *x, T *y) {
   const size_t lds = sizeof(T) * EB * ((size_t)c.nb + c.K + 1);
-  const int nt = (c.B.nslice * 64 + 1023) / 1024;
-  if (lds > c.lds_limit || nt > 32) return false;
-  if (nt <= 1) launch_mixed_ell<T, 1, EB>(c, eri, x, y, lds);
-  else if (nt <= 2) launch_mixed_ell<T, 2, EB>(c, eri, x, y, lds);
-  else if (nt <= 4) launch_mixed_ell<T, 4, EB>(c, eri, x, y, lds);
-  else if (nt <= 8) launch_mixed_ell<T, 8, EB>(c, eri, x, y, lds);
-  else if (nt <= 16) launch_mixed_ell<T, 16, EB>(c, eri, x, y, lds);
-  else launch_mixed_ell<T, 32, EB>(c, eri, x, y, lds);
+  if (lds > c.lds_limit) return false;
+  if (nt <= 1) launch_mixed_ell<T, 1, EB, SYM>(c, eri, x, y, lds);
+  else if (nt <= 2) launch_mixed_ell<T, 2, EB, SYM>(c, eri, x, y, lds);
+  else if (nt <= 4) launch_mixed_ell<T, 4, EB, SYM>(c, eri, x, y, lds);
+  else launch_mixed_ell<T, 8, EB, SYM>(c, eri, x, y, lds);
   return true;
 }
 
@@ -654,71 +752,80 @@ static bool mixed_parts_dispatch(Context &c, const T *eri, const T *x, T *y) {
   return false;
 }
 
-static bool mixed_ell_eb(Context &c, int eb, const double *eri, const double *x, double *y) {
-  return eb == 4 ? mixed_ell_dispatch<double, 4>(c, eri, x, y)
-       : eb == 2 ? mixed_ell_dispatch<double, 2>(c, eri, x, y)
-                 : mixed_ell_dispatch<double, 1>(c, eri, x, y);
+static bool mixed_ell_cfg(Context &c, int eb, int nt, const double *eri, const double *x, double *y) {
+  return eb == 4 ? mixed_ell_dispatch<double, 4>(c, nt, eri, x, y)
+       : eb == 2 ? mixed_ell_dispatch<double, 2>(c, nt, eri, x, y)
+                 : mixed_ell_dispatch<double, 1>(c, nt, eri, x, y);
 }
-static bool mixed_ell_eb(Context &c, int eb, const float *eri, const float *x, float *y) {
-  return eb == 4 ? mixed_ell_dispatch<float, 4>(c, eri, x, y)
-       : eb == 2 ? mixed_ell_dispatch<float, 2>(c, eri, x, y)
-                 : mixed_ell_dispatch<float, 1>(c, eri, x, y);
+static bool mixed_ell_cfg(Context &c, int eb, int nt, const float *eri, const float *x, float *y) {
+  return eb == 4 ? mixed_ell_dispatch<float, 4>(c, nt, eri, x, y)
+       : eb == 2 ? mixed_ell_dispatch<float, 2>(c, nt, eri, x, y)
+                 : mixed_ell_dispatch<float, 1>(c, nt, eri, x, y);
 }
 static double *scratch_vec(Context &c, const double *) { return c.T2; }
 static float *scratch_vec(Context &c, const float *) { return c.f32.T2; }
 
-// Autotuner for the sliced-ELL mixed kernel's staging width: every width
-// whose LDS footprint fits (1, 2, 4 pairs per round) runs twice on this x
-// into a scratch vector (the kernel accumulates, so y is untouched), HIP
-// events on the stream time the second run, the fastest is kept for this
-// precision; -1 when none fits.  SBDX_MIXED_EB=1|2|4 overrides.
+// Autotuner for the sliced-ELL mixed kernel: every (staging width EB in
+// {1, 2, 4}, slots per thread NT in {1, 2, 4, 8}) whose LDS footprint fits
+// and that does not cover fewer than half of a row's slots per thread
+// budget runs twice on this x into a scratch vector (the kernel
+// accumulates, so y is untouched); HIP events on the stream time the second
+// run and the fastest is kept for this precision (-1: nothing fits).
+// SBDX_MIXED_EB=1|2|4 and SBDX_MIXED_NT=1|2|4|8 fix either coordinate.
 template <typename T>
-static int mixed_eb(Context &c, const T *eri, const T *x) {
+static int mixed_cfg(Context &c, const T *eri, const T *x) {
   const int pi = sizeof(T) == 4 ? 1 : 0;
   if (c.mixed_eb[pi] != 0) return c.mixed_eb[pi];
-  const char *env = std::getenv("SBDX_MIXED_EB");
-  if (env) {
-    const int eb = std::atoi(env);
-    if (eb != 1 && eb != 2 && eb != 4) throw Error(1, "SBDX_MIXED_EB must be 1, 2 or 4");
-    return c.mixed_eb[pi] = eb;
-  }
+  const int nt_max = (int)cdiv((size_t)c.B.nslice * 64, 1024);
+  const char *env_eb = std::getenv("SBDX_MIXED_EB");
+  const char *env_nt = std::getenv("SBDX_MIXED_NT");
+  const int fix_eb = env_eb ? std::atoi(env_eb) : 0, fix_nt = env_nt ? std::atoi(env_nt) : 0;
+  if (env_eb && fix_eb != 1 && fix_eb != 2 && fix_eb != 4) throw Error(1, "SBDX_MIXED_EB must be 1, 2 or 4");
+  if (env_nt && fix_nt != 1 && fix_nt != 2 && fix_nt != 4 && fix_nt != 8)
+    throw Error(1, "SBDX_MIXED_NT must be 1, 2, 4 or 8");
   T *scratch = scratch_vec(c, x);
   hipEvent_t e0, e1;
   SBDX_HIP(hipEventCreate(&e0));
   SBDX_HIP(hipEventCreate(&e1));
-  const int widths[3] = {1, 2, 4};
-  int best = -1;
+  int best_eb = -1, best_nt = 0;
   float best_ms = 0.f;
-  for (int w = 0; w < 3; w++) {
-    c.mixed_eb_ms[pi][w] = 0.f;
-    bool ok = true;
-    float ms = 0.f;
-    for (int rep = 0; rep < 2 && ok; rep++) {
-      SBDX_HIP(hipEventRecord(e0, c.stream));
-      ok = mixed_ell_eb(c, widths[w], eri, x, scratch);
-      SBDX_HIP(hipEventRecord(e1, c.stream));
-      SBDX_HIP(hipEventSynchronize(e1));
-      SBDX_HIP(hipEventElapsedTime(&ms, e0, e1));
-    }
-    if (!ok) continue;
-    check_launch("k_mixed_ell");
-    c.mixed_eb_ms[pi][w] = ms;
-    if (best < 0 || ms < best_ms) {
-      best = widths[w];
-      best_ms = ms;
+  for (int eb : {1, 2, 4}) {
+    if (fix_eb && eb != fix_eb) continue;
+    for (int nt : {1, 2, 4, 8}) {
+      if (fix_nt && nt != fix_nt) continue;
+      if (!fix_nt && nt > 1 && (nt / 2) >= nt_max) continue;  // one block already covers the row
+      bool ok = true;
+      float ms = 0.f;
+      for (int rep = 0; rep < 2 && ok; rep++) {
+        SBDX_HIP(hipEventRecord(e0, c.stream));
+        ok = mixed_ell_cfg(c, eb, nt, eri, x, scratch);
+        SBDX_HIP(hipEventRecord(e1, c.stream));
+        SBDX_HIP(hipEventSynchronize(e1));
+        SBDX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      }
+      if (!ok) continue;
+      check_launch("k_mixed_ell");
+      if (best_eb < 0 || ms < best_ms) {
+        best_eb = eb;
+        best_nt = nt;
+        best_ms = ms;
+      }
     }
   }
   SBDX_HIP(hipEventDestroy(e0));
   SBDX_HIP(hipEventDestroy(e1));
-  return c.mixed_eb[pi] = best;
+  c.mixed_nt[pi] = best_nt;
+  c.mixed_best_ms[pi] = best_ms;
+  return c.mixed_eb[pi] = best_eb;
 }
 
 template <typename T>
 static void sigma_mixed(Context &c, const T *eri, const T *x, T *y) {
   if (c.B.has_ell) {
-    // tuned staging width, then row parts
-    const int eb = mixed_eb<T>(c, eri, x);
-    if (eb > 0 && mixed_ell_eb(c, eb, eri, x, y)) return check_launch("k_mixed_ell");
+    // tuned staging width and slots per thread, then row parts
+    const int pi = sizeof(T) == 4 ? 1 : 0;
+    const int eb = mixed_cfg<T>(c, eri, x);
+    if (eb > 0 && mixed_ell_cfg(c, eb, c.mixed_nt[pi], eri, x, y)) return check_launch("k_mixed_ell");
     if (mixed_parts_dispatch<T>(c, eri, x, y)) return check_launch("k_mixed_parts");
   }
   const int nb = (int)c.nb;
@@ -744,23 +851,21 @@ template <typename T, int NT, int KV>
 static void launch_mixed_multi(Context &c, const T *eri, const T *x, T *y, size_t lds) {
   auto fn = k_mixed_multi<T, NT, KV>;
   SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
-                     c.B.nslice * 64, c.N, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell, c.B.perm,
+  const int nslots = c.B.nslice * 64;
+  const unsigned nsplit = cdiv(nslots, 1024 * NT);
+  hipLaunchKernelGGL(fn, dim3((unsigned)c.na, nsplit), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
+                     nslots, c.N, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell, c.B.perm,
                      c.B.sl_off, c.B.sl_len, eri, x, y);
 }
 
+// KV vectors per launch with 2 slots per thread (the per-thread state grows
+// with KV); false when the staging does not fit in LDS.
 template <typename T, int KV>
 static bool mixed_multi_dispatch(Context &c, const T *eri, const T *x, T *y) {
   if (!c.B.has_ell) return false;
   const size_t lds = sizeof(T) * ((size_t)c.K + 1 + (size_t)KV * c.nb);
-  const int nt = (c.B.nslice * 64 + 1023) / 1024;
-  if (lds > c.lds_limit || nt > 32) return false;
-  if (nt <= 1) launch_mixed_multi<T, 1, KV>(c, eri, x, y, lds);
-  else if (nt <= 2) launch_mixed_multi<T, 2, KV>(c, eri, x, y, lds);
-  else if (nt <= 4) launch_mixed_multi<T, 4, KV>(c, eri, x, y, lds);
-  else if (nt <= 8) launch_mixed_multi<T, 8, KV>(c, eri, x, y, lds);
-  else if (nt <= 16) launch_mixed_multi<T, 16, KV>(c, eri, x, y, lds);
-  else launch_mixed_multi<T, 32, KV>(c, eri, x, y, lds);
+  if (lds > c.lds_limit) return false;
+  launch_mixed_multi<T, 2, KV>(c, eri, x, y, lds);
   check_launch("k_mixed_multi");
   return true;
 }
@@ -770,7 +875,43 @@ static bool mixed_multi_dispatch(Context &c, const T *eri, const T *x, T *y) {
 template <typename T>
 static void sigma_mixed_block(Context &c, const T *eri, int k, const T *x, T *y) {
   const size_t N = c.N;
+  const int pi = sizeof(T) == 4 ? 1 : 0;
+  if (k >= 2 && c.multi_mode[pi] < 0) {
+    // first block product of this precision: two vectors through the
+    // multi-vector kernel vs two single-vector launches, into a scratch pair
+    // (second of two runs each); the faster decides for later calls
+    c.multi_mode[pi] = 0;
+    const size_t lds = sizeof(T) * ((size_t)c.K + 1 + 2 * c.nb);
+    if (c.B.has_ell && lds <= c.lds_limit) {
+      TmpBuf tmp(sizeof(T) * N * 2);
+      T *s = (T *)tmp.p;
+      hipEvent_t e0, e1;
+      SBDX_HIP(hipEventCreate(&e0));
+      SBDX_HIP(hipEventCreate(&e1));
+      float ms[2] = {0.f, 0.f};
+      for (int mode = 0; mode < 2; mode++)
+        for (int rep = 0; rep < 2; rep++) {
+          SBDX_HIP(hipEventRecord(e0, c.stream));
+          if (mode == 1) {
+            mixed_multi_dispatch<T, 2>(c, eri, x, s);
+          } else {
+            sigma_mixed<T>(c, eri, x, s);
+            sigma_mixed<T>(c, eri, x + N, s + N);
+          }
+          SBDX_HIP(hipEventRecord(e1, c.stream));
+          SBDX_HIP(hipEventSynchronize(e1));
+          SBDX_HIP(hipEventElapsedTime(&ms[mode], e0, e1));
+        }
+      SBDX_HIP(hipEventDestroy(e0));
+      SBDX_HIP(hipEventDestroy(e1));
+      c.multi_mode[pi] = ms[1] < ms[0] ? 1 : 0;
+    }
+  }
   int v = 0;
+  while (v < k && c.multi_mode[pi] == 0) {
+    sigma_mixed<T>(c, eri, x + v * N, y + v * N);
+    v += 1;
+  }
   while (v < k) {
     const int left = k - v;
     if (left >= 4 && mixed_multi_dispatch<T, 4>(c, eri, x + v * N, y + v * N)) {
@@ -833,16 +974,16 @@ static void dense_blocks(Context &c, float *&Ha, float *&Hb) {
 }
 
 static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const double *A, int lda,
-                           const double *B, int ldb, double *C, int ldc) {
+                           const double *B, int ldb, double *C, int ldc, double beta = 1.0) {
   const double one = 1.0;
   return rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B,
-                       ldb, &one, C, ldc);
+                       ldb, &beta, C, ldc);
 }
 static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const float *A, int lda,
-                           const float *B, int ldb, float *C, int ldc) {
+                           const float *B, int ldb, float *C, int ldc, float beta = 1.0f) {
   const float one = 1.0f;
   return rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B,
-                       ldb, &one, C, ldc);
+                       ldb, &beta, C, ldc);
 }
 
 // y = hii*x + H_a X + X H_b.  Column-major, the na x nb row-major X is the
@@ -985,6 +1126,89 @@ static void sigma_impl(Context &c, const T *x, T *y) {
 }
 
 void sigma_alpha_first(Context &c, const double *x, double *y) { sigma_impl<double>(c, x, y); }
+
+// Whether the symmetric form applies: equal string sets, the dense blocks
+// fit (same-spin autotuner chose or can hold them), the sliced-ELL table.
+static bool sym_ok(Context &c) {
+  if (!c.B.alias || !c.B.has_ell || c.na != c.nb) return false;
+  size_t free_b = 0, total_b = 0;
+  SBDX_HIP(hipMemGetInfo(&free_b, &total_b));
+  const size_t bytes = (c.Hd_a ? 0 : sizeof(double) * c.na * c.na) +
+                       (c.f32.ready && !c.Hf_a ? sizeof(float) * c.na * c.na : 0);
+  return c.na < (1u << 31) && c.N < (1ull << 31) && bytes < free_b / 4;
+}
+
+// sigma for a spin-flip symmetric x (x = x^T as an na x na matrix, equal
+// string sets): one GEMM G = H_a X, y = hii*x + G + G^T, and the mixed term
+// for rows r >= I only, mirrored.  The result equals the general sigma for
+// such x (H commutes with the transpose when the string sets are equal).
+template <typename T>
+static void sigma_sym_impl(Context &c, const T *x, T *y) {
+  const SigmaOperands<T> o = operands(c, (T *)nullptr);
+  const bool f32 = sizeof(T) == 4;
+  const int n = (int)c.na;
+  {
+    ProfScope ps(c, f32 ? "sym_same_f32" : "sym_same");
+    T *Ha, *Hb;
+    dense_blocks(c, Ha, Hb);
+    if (gemm(blas(c), n, n, n, x, n, Ha, n, o.T1, n, T(0)) != rocblas_status_success)
+      throw Error(3, "symmetric same-spin GEMM failed");
+    hipLaunchKernelGGL((k_sym_combine<T>), dim3(cdiv(n, 64), cdiv(n, 64)), dim3(256), 0, c.stream, n, o.hii,
+                       x, o.T1, y);
+    check_launch("k_sym_combine");
+  }
+  {
+    ProfScope ps(c, f32 ? "sym_mixed_f32" : "sym_mixed");
+    build_singles_ell_natural(c, c.B);
+    const int pi = f32 ? 1 : 0;
+    const int eb = mixed_cfg<T>(c, o.eri, x);
+    const int nt = c.mixed_nt[pi];
+    bool ok = false;
+    if (eb == 4) ok = mixed_ell_dispatch<T, 4, true>(c, nt, o.eri, x, y);
+    else if (eb == 2) ok = mixed_ell_dispatch<T, 2, true>(c, nt, o.eri, x, y);
+    else if (eb == 1) ok = mixed_ell_dispatch<T, 1, true>(c, nt, o.eri, x, y);
+    if (!ok) throw Error(3, "symmetric mixed kernel: staging does not fit");
+    check_launch("k_mixed_ell<sym>");
+  }
+}
+
+bool sigma_sym_available(Context &c) { return sym_ok(c); }
+
+void sigma_sym(Context &c, const double *x, double *y) { sigma_sym_impl<double>(c, x, y); }
+
+void sigma_sym_f32(Context &c, const double *x, double *y) {
+  ensure_f32(c);
+  const unsigned g = (unsigned)std::min<size_t>(cdiv(c.N, 256), 65536);
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(g), dim3(256), 0, c.stream, c.N, x, c.f32.x);
+  check_launch("k_cast");
+  sigma_sym_impl<float>(c, c.f32.x, c.f32.y);
+  hipLaunchKernelGGL((k_cast<float, double>), dim3(g), dim3(256), 0, c.stream, c.N, c.f32.y, y);
+  check_launch("k_cast");
+}
+
+// max |v - v^T| / max |v| of an na x na vector (0 for v = 0)
+double sym_defect(Context &c, const double *v) {
+  const int n = (int)c.na;
+  const unsigned t = cdiv(n, 64);
+  TmpBuf part(sizeof(double) * 2 * t * t);
+  hipLaunchKernelGGL(k_sym_defect, dim3(t, t), dim3(256), 0, c.stream, n, v, (double *)part.p);
+  check_launch("k_sym_defect");
+  std::vector<double> h((size_t)2 * t * t);
+  SBDX_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  double d = 0.0, m = 0.0;
+  for (size_t b = 0; b < (size_t)t * t; b++) {
+    d = std::max(d, h[2 * b]);
+    m = std::max(m, h[2 * b + 1]);
+  }
+  return m > 0.0 ? d / m : 0.0;
+}
+
+void symmetrize(Context &c, double *v) {
+  const unsigned t = cdiv(c.na, 64);
+  hipLaunchKernelGGL(k_symmetrize, dim3(t, t), dim3(256), 0, c.stream, (int)c.na, v);
+  check_launch("k_symmetrize");
+}
 
 // k products at once, vectors contiguous (x + v*N -> y + v*N).
 template <typename T>

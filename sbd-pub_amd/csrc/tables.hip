@@ -218,22 +218,63 @@ void build_spin_tables(Context &c, SpinTables &t, const std::vector<uint64_t> &s
 
 // Sliced-ELL copy of the singles (see SpinTables::ell).  Needs kets < 2^16
 // and pq < 2^15; otherwise has_ell stays false and the CSR kernels run.
-void build_singles_ell(Context &c, SpinTables &t) {
-  t.has_ell = false;
-  if (t.n > 65536 || c.K >= 32768) return;
+// Order of one 64-row slice's entries (position k of lane l at
+// out[64*k + l]).  The mixed kernels read x[.., ket] and V[.., pq] from LDS
+// at the indices of entry k of all 64 lanes at once, and the row sums do
+// not depend on the entry order, so entries are dealt greedily: at each k,
+// lane by lane, the remaining entry whose ket and pq residues mod 64 are
+// least used so far at this k.  Distinct residues mod 64 across the
+// wavefront spread the reads evenly over the LDS banks for 32 or 64 banks
+// and 4- or 8-byte elements.  Padding entries (pq = pad, a zero in every
+// staged ERI row, one shared address) take an unused ket residue below nb.
+static void schedule_slice(std::vector<uint32_t> *rows, uint32_t len, uint32_t pad_pq, uint32_t nb,
+                           uint32_t *out) {
+  for (uint32_t k = 0; k < len; k++) {
+    uint8_t cj[64] = {0}, cv[64] = {0};
+    for (int l = 0; l < 64; l++) {
+      std::vector<uint32_t> &R = rows[l];
+      uint32_t w;
+      if (R.empty()) {
+        int best = 0;
+        for (int j = 1; j < 64; j++)
+          if (cj[j] < cj[best]) best = j;
+        const uint32_t jb = nb > 64 ? (uint32_t)best : (uint32_t)best % nb;
+        w = jb | (pad_pq << 16);
+        cj[jb & 63]++;
+      } else {
+        size_t bi = 0;
+        int bc = 1 << 30;
+        for (size_t i = 0; i < R.size(); i++) {
+          const uint32_t e = R[i];
+          const int cost = cj[e & 63] + cv[(e >> 16) & 63];
+          if (cost < bc) {
+            bc = cost;
+            bi = i;
+            if (cost == 0) break;
+          }
+        }
+        w = R[bi];
+        R[bi] = R.back();
+        R.pop_back();
+        cj[w & 63]++;
+        cv[(w >> 16) & 63]++;
+      }
+      out[(size_t)64 * k + l] = w;
+    }
+  }
+}
+
+// Sliced-ELL layout of the singles of t with rows taken in `order`
+// (64 rows per slice); results in host vectors.
+static void ell_layout(Context &c, const SpinTables &t, const std::vector<uint64_t> &off,
+                       const std::vector<uint64_t> &ent, const std::vector<uint32_t> &order,
+                       std::vector<uint32_t> &perm, std::vector<uint32_t> &sl_len,
+                       std::vector<uint64_t> &sl_off, std::vector<uint32_t> &ell) {
   const int n = t.n;
-  std::vector<uint64_t> off(n + 1), ent(t.ns);
-  SBDX_HIP(hipMemcpyAsync(off.data(), t.s_off, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, c.stream));
-  SBDX_HIP(hipMemcpyAsync(ent.data(), t.s_ent, sizeof(uint64_t) * t.ns, hipMemcpyDeviceToHost, c.stream));
-  SBDX_HIP(hipStreamSynchronize(c.stream));
-  std::vector<uint32_t> order(n);
-  for (int i = 0; i < n; i++) order[i] = (uint32_t)i;
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-    return (off[a + 1] - off[a]) > (off[b + 1] - off[b]);
-  });
   const int nsl = (n + 63) / 64;
-  std::vector<uint32_t> perm((size_t)nsl * 64, 0xffffffffu), sl_len(nsl);
-  std::vector<uint64_t> sl_off(nsl);
+  perm.assign((size_t)nsl * 64, 0xffffffffu);
+  sl_len.assign(nsl, 0);
+  sl_off.assign(nsl, 0);
   uint64_t total = 0;
   for (int s = 0; s < nsl; s++) {
     uint32_t mx = 0;
@@ -247,18 +288,48 @@ void build_singles_ell(Context &c, SpinTables &t) {
     total += (uint64_t)mx * 64;
   }
   const uint32_t pad = (uint32_t)c.K << 16;
-  std::vector<uint32_t> ell(total + 1, pad);
-  for (int s = 0; s < nsl; s++)
+  ell.assign(total + 1, pad);
+  std::vector<uint32_t> rows[64];
+  for (int s = 0; s < nsl; s++) {
     for (int l = 0; l < 64; l++) {
+      rows[l].clear();
       const uint32_t r = perm[(size_t)s * 64 + l];
       if (r == 0xffffffffu) continue;
       for (uint64_t k = 0; k < off[r + 1] - off[r]; k++) {
         const uint64_t e = ent[off[r] + k];
         const uint32_t ket = (uint32_t)(e & 0xffffffffull);
         const uint32_t pq = (uint32_t)((e >> 32) & 0x7fffffffull);
-        ell[sl_off[s] + 64 * k + l] = ket | (pq << 16) | ((e >> 63) ? 0x80000000u : 0u);
+        rows[l].push_back(ket | (pq << 16) | ((e >> 63) ? 0x80000000u : 0u));
       }
     }
+    schedule_slice(rows, sl_len[s], (uint32_t)c.K, (uint32_t)n, ell.data() + sl_off[s]);
+  }
+}
+
+static void download_singles(Context &c, const SpinTables &t, std::vector<uint64_t> &off,
+                             std::vector<uint64_t> &ent) {
+  off.resize(t.n + 1);
+  ent.resize(t.ns);
+  SBDX_HIP(hipMemcpyAsync(off.data(), t.s_off, sizeof(uint64_t) * (t.n + 1), hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipMemcpyAsync(ent.data(), t.s_ent, sizeof(uint64_t) * t.ns, hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+}
+
+void build_singles_ell(Context &c, SpinTables &t) {
+  t.has_ell = false;
+  if (t.n > 65536 || c.K >= 32768) return;
+  const int n = t.n;
+  std::vector<uint64_t> off, ent;
+  download_singles(c, t, off, ent);
+  std::vector<uint32_t> order(n);
+  for (int i = 0; i < n; i++) order[i] = (uint32_t)i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return (off[a + 1] - off[a]) > (off[b + 1] - off[b]);
+  });
+  std::vector<uint32_t> perm, sl_len, ell;
+  std::vector<uint64_t> sl_off;
+  ell_layout(c, t, off, ent, order, perm, sl_len, sl_off, ell);
+  const int nsl = (int)sl_len.size();
   t.nslice = nsl;
   t.ell = (uint32_t *)dalloc(c, sizeof(uint32_t) * ell.size());
   t.perm = (uint32_t *)dalloc(c, sizeof(uint32_t) * perm.size());
@@ -270,6 +341,28 @@ void build_singles_ell(Context &c, SpinTables &t) {
   SBDX_HIP(hipMemcpyAsync(t.sl_len, sl_len.data(), sizeof(uint32_t) * nsl, hipMemcpyHostToDevice, c.stream));
   SBDX_HIP(hipStreamSynchronize(c.stream));
   t.has_ell = true;
+}
+
+// The same singles with slices in row order (slice s = rows 64s..64s+63),
+// for the symmetric mixed kernel, which walks only rows r >= I of bra I.
+void build_singles_ell_natural(Context &c, SpinTables &t) {
+  if (t.has_nell || !t.has_ell) return;
+  std::vector<uint64_t> off, ent;
+  download_singles(c, t, off, ent);
+  std::vector<uint32_t> order(t.n);
+  for (int i = 0; i < t.n; i++) order[i] = (uint32_t)i;
+  std::vector<uint32_t> perm, sl_len, ell;
+  std::vector<uint64_t> sl_off;
+  ell_layout(c, t, off, ent, order, perm, sl_len, sl_off, ell);
+  const int nsl = (int)sl_len.size();
+  t.nell = (uint32_t *)dalloc(c, sizeof(uint32_t) * ell.size());
+  t.nsl_off = (uint64_t *)dalloc(c, sizeof(uint64_t) * nsl);
+  t.nsl_len = (uint32_t *)dalloc(c, sizeof(uint32_t) * nsl);
+  SBDX_HIP(hipMemcpyAsync(t.nell, ell.data(), sizeof(uint32_t) * ell.size(), hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(t.nsl_off, sl_off.data(), sizeof(uint64_t) * nsl, hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(t.nsl_len, sl_len.data(), sizeof(uint32_t) * nsl, hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  t.has_nell = true;
 }
 
 // Ket-range parts of the sliced-ELL singles (SpinTables::pell): part p holds
@@ -311,20 +404,21 @@ void build_singles_ell_parts(Context &c, SpinTables &t, int nparts) {
       total += (uint64_t)mx * 64;
     }
     std::vector<uint32_t> ell(total + 1, pad);
-    for (int s = 0; s < t.nslice; s++)
+    std::vector<uint32_t> rows[64];
+    for (int s = 0; s < t.nslice; s++) {
       for (int l = 0; l < 64; l++) {
+        rows[l].clear();
         const uint32_t r = perm[(size_t)s * 64 + l];
         if (r == 0xffffffffu) continue;
-        uint64_t k = 0;
         for (uint64_t e = off[r]; e < off[r + 1]; e++) {
           const uint64_t ket = ent[e] & 0xffffffffull;
           if (ket < lo || ket >= hi) continue;
           const uint32_t pq = (uint32_t)((ent[e] >> 32) & 0x7fffffffull);
-          ell[sl_off[s] + 64 * k + l] =
-              (uint32_t)(ket - lo) | (pq << 16) | ((ent[e] >> 63) ? 0x80000000u : 0u);
-          k++;
+          rows[l].push_back((uint32_t)(ket - lo) | (pq << 16) | ((ent[e] >> 63) ? 0x80000000u : 0u));
         }
       }
+      schedule_slice(rows, sl_len[s], (uint32_t)c.K, (uint32_t)(hi - lo), ell.data() + sl_off[s]);
+    }
     t.pell[p] = (uint32_t *)dalloc(c, sizeof(uint32_t) * ell.size());
     t.psl_off[p] = (uint64_t *)dalloc(c, sizeof(uint64_t) * t.nslice);
     t.psl_len[p] = (uint32_t *)dalloc(c, sizeof(uint32_t) * t.nslice);

@@ -44,7 +44,8 @@ ABI_SYMBOLS = [
     "sbdx_mlp_forward", "sbdx_mlp_gradient", "sbdx_mlp_train", "sbdx_mlp_record", "sbdx_mlp_collect",
     "sbdx_set_preconditioner", "sbdx_precond_stats", "sbdx_set_verbose",
     "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
-    "sbdx_mixed_staging",
+    "sbdx_mixed_staging", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry",
+    "sbdx_sigma_sym",
 ]
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -92,6 +93,10 @@ def lib():
     L.sbdx_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_sigma_fp32.argtypes = [C.c_void_p, _dp, _dp]
     L.sbdx_sigma_block.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
+    L.sbdx_symmetric_eig.argtypes = [C.c_int, _dp, _dp, _dp]
+    L.sbdx_set_symmetry.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_sigma_sym.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+    L.sbdx_sigma_block_fp32.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.sbdx_davidson_block.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_double,
                                       C.c_double, _dp, C.POINTER(C.c_int)]
     L.sbdx_set_restart.argtypes = [C.c_void_p, C.c_int]
@@ -99,7 +104,8 @@ def lib():
     L.sbdx_same_spin_form.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]
     _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
-    L.sbdx_mixed_staging.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _fp]
+    L.sbdx_mixed_staging.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_float), C.POINTER(C.c_int)]
     _u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
     L.sbdx_mlp_set_weights.argtypes = [C.c_void_p, _fp, C.c_size_t]
     L.sbdx_mlp_get_weights.argtypes = [C.c_void_p, _fp, C.c_size_t]
@@ -243,15 +249,27 @@ class TPBOperator:
         k Ritz vectors, exact FP64 residual at every restart."""
         _check(lib().sbdx_set_restart(self.h, int(keep)))
 
+    def set_symmetry(self, on):
+        """Spin-flip symmetric sigma in the thick-restart solves (equal string
+        sets, symmetric start vector)."""
+        _check(lib().sbdx_set_symmetry(self.h, 1 if on else 0))
+
+    def sigma_sym(self, W, fp32=False):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        out = np.zeros_like(W)
+        _check(lib().sbdx_sigma_sym(self.h, W, out, 1 if fp32 else 0))
+        return out
+
     def set_verbose(self, level):
         """Nonzero: one stderr line per Davidson step (E0, |r|)."""
         _check(lib().sbdx_set_verbose(self.h, int(level)))
 
     def configure(self, solver):
-        """'plain' (reference algorithm, FP64), 'thick' (FP64 thick restart)
-        or 'mixed' (thick restart, FP32 correction products)."""
+        """'plain' (reference algorithm, FP64), 'thick' (FP64 thick restart),
+        'mixed' (thick restart, FP32 correction products) or 'block' (the
+        precision of block Davidson: FP32 correction products)."""
         keep, prec = {"plain": (0, PREC_FP64), "thick": (4, PREC_FP64),
-                      "mixed": (4, PREC_MIXED)}[solver]
+                      "mixed": (4, PREC_MIXED), "block": (4, PREC_MIXED)}[solver]
         self.set_restart(keep)
         self.set_precision(prec)
 
@@ -283,11 +301,15 @@ class TPBOperator:
                     allow=(5,))
         return e.value, n.value, rc == 0
 
-    def sigma_block(self, V):
-        """Products of the k rows of V (k x ndet) in one block call."""
+    def sigma_block(self, V, fp32=False):
+        """Products of the k rows of V (k x ndet) in one block call (FP32
+        arithmetic with fp32=True)."""
         V = np.ascontiguousarray(np.atleast_2d(V), dtype=np.float64)
         out = np.zeros_like(V)
-        _check(lib().sbdx_sigma_block(self.h, V.shape[0], V.reshape(-1), out.reshape(-1), 0))
+        if fp32:
+            _check(lib().sbdx_sigma_block_fp32(self.h, V.shape[0], V.reshape(-1), out.reshape(-1)))
+        else:
+            _check(lib().sbdx_sigma_block(self.h, V.shape[0], V.reshape(-1), out.reshape(-1), 0))
         return out
 
     def davidson_block(self, nroots, block, W0=None, max_it=200, eps=1e-8, max_time=1e30):
@@ -340,10 +362,11 @@ class TPBOperator:
             _check(lib().sbdx_same_spin_form(self.h, prec, C.byref(f), C.byref(a), C.byref(b)))
             out[key] = dict(form={-1: None, 0: "sparse", 1: "dense"}[f.value],
                             ms_sparse=round(a.value, 3), ms_dense=round(b.value, 3))
-            w, ms = C.c_int(), np.zeros(3, dtype=np.float32)
-            _check(lib().sbdx_mixed_staging(self.h, prec, C.byref(w), ms))
-            out[key]["mixed_width"] = w.value
-            out[key]["mixed_ms"] = [round(float(t), 3) for t in ms]
+            w, nt, ms, mu = C.c_int(), C.c_int(), C.c_float(), C.c_int()
+            _check(lib().sbdx_mixed_staging(self.h, prec, C.byref(w), C.byref(nt), C.byref(ms),
+                                            C.byref(mu)))
+            out[key].update(mixed_width=w.value, mixed_slots_per_thread=nt.value,
+                            mixed_ms=round(ms.value, 3), block_multi_kernel=mu.value)
         return out
 
     def profile(self, enable=True):
@@ -360,6 +383,18 @@ class TPBOperator:
             _check(lib().sbdx_profile_get(self.h, i, name, 128, C.byref(cnt), C.byref(ms)))
             out[name.value.decode()] = (cnt.value, ms.value)
         return out
+
+
+def symmetric_eig(A):
+    """Eigenvalues (ascending) and eigenvectors (columns) of a symmetric
+    matrix by the library's subspace eigensolver (host code, no GPU)."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    a = np.ascontiguousarray(A.T).reshape(-1)  # column-major
+    V = np.zeros(n * n)
+    E = np.zeros(n)
+    _check(lib().sbdx_symmetric_eig(n, a, V, E))
+    return E, V.reshape(n, n).T
 
 
 def basis_init_vector(ndet):

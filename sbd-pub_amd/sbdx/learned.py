@@ -76,3 +76,85 @@ def baseline_loss(feat, tgt):
     x = d.view(np.float32).astype(np.float64)
     a = float(x @ tgt / max(x @ x, 1e-30))
     return float(np.mean((a * x - tgt) ** 2)), float(np.mean(tgt.astype(np.float64) ** 2))
+
+
+EVAL_INSTANCE = dict(norb=22, nelec_half=6, nstr=700, nchol=60, seed=211)  # held out
+
+
+def evaluate(weights, spec=None, inst=None, solver="thick", eps=1e-6, max_it=400, nb=12,
+             start="hf", log=print):
+    """Time-to-|r|<eps and sigma products of the thick-restart solver with the
+    diagonal preconditioner and with the MLP (weights), same start vector.
+    Returns {"diag": {...}, "mlp": {...}, "gain_products": diag/mlp,
+    "gain_time": diag/mlp}."""
+    if inst is None:
+        inst = instances.make_instance(**(spec or EVAL_INSTANCE))
+    op = TPBOperator.from_instance(inst)
+    if start == "hf":
+        W0 = basis_init_vector(op.ndet)
+    else:
+        W0 = np.random.default_rng(7).standard_normal(op.ndet)
+        W0 /= np.linalg.norm(W0)
+    op.mlp_set_weights(weights)
+    out = {}
+    for mode, name in ((0, "diag"), (1, "mlp")):
+        op.configure(solver)
+        op.set_preconditioner(mode)
+        op.set_vector(W0)
+        t0 = time.perf_counter()
+        E, n, ok = op.solve(max_it=max_it, nb=nb, eps=eps)
+        dt = time.perf_counter() - t0
+        out[name] = dict(E=E, products=n, converged=ok, time_s=round(dt, 4),
+                         steps_by_precond=op.precond_stats())
+        log(f"[learned] eval {name}: E={E:.12f} products={n} converged={ok} time={dt:.3f} s "
+            f"{out[name]['steps_by_precond']}")
+    op.set_preconditioner(0)
+    op.close()
+    out["dE"] = abs(out["mlp"]["E"] - out["diag"]["E"])
+    out["gain_products"] = out["diag"]["products"] / max(out["mlp"]["products"], 1)
+    out["gain_time"] = out["diag"]["time_s"] / max(out["mlp"]["time_s"], 1e-9)
+    return out
+
+
+def _main(argv=None):
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="train / evaluate the learned preconditioner")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--out", required=True, help="weights file (.npy, float32)")
+    t.add_argument("--epochs", type=int, default=20)
+    t.add_argument("--pairs", type=int, default=500_000, help="pairs per training instance")
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--seed", type=int, default=0)
+    e = sub.add_parser("eval")
+    e.add_argument("--weights", required=True)
+    e.add_argument("--eps", type=float, default=1e-6)
+    e.add_argument("--solver", default="thick")
+    args = ap.parse_args(argv)
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    if args.cmd == "train":
+        t0 = time.perf_counter()
+        feat, tgt = generate_pairs(pairs_per_instance=args.pairs, seed=args.seed, log=log)
+        base, var = baseline_loss(feat, tgt)
+        inst = instances.make_instance(**TRAIN_INSTANCES[0])
+        op = TPBOperator.from_instance(inst)
+        w, loss = train(op, feat, tgt, epochs=args.epochs, lr=args.lr, seed=args.seed)
+        op.close()
+        np.save(args.out, w)
+        res = dict(pairs=int(tgt.size), loss=[float(x) for x in loss], target_var=var,
+                   diag_step_loss=base, seconds=round(time.perf_counter() - t0, 2))
+        log(f"[learned] trained on {tgt.size} pairs: loss {loss[0]:.4g} -> {loss[-1]:.4g} "
+            f"(diagonal step {base:.4g}, target variance {var:.4g})")
+        print(json.dumps(res))
+    else:
+        w = np.load(args.weights)
+        print(json.dumps(evaluate(w, solver=args.solver, eps=args.eps, log=log)))
+
+
+if __name__ == "__main__":
+    _main()

@@ -32,6 +32,10 @@ def test_sigma_block_matches_single(k, oracle_mod):
         r = orc.sigma(V[j])
         assert np.max(np.abs(B[j] - s)) < 1e-12 * np.max(np.abs(s))
         assert np.max(np.abs(B[j] - r)) < 1e-12 * np.max(np.abs(r))
+    B32 = op.sigma_block(V, fp32=True)
+    for j in range(k):
+        r = orc.sigma(V[j])
+        assert np.linalg.norm(B32[j] - r) / np.linalg.norm(r) < 2e-6, j
     op.close()
     orc.close()
 
