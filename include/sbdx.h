@@ -205,7 +205,7 @@ int sbdx_sigma_fp32(sbdx_handle_t h, const double *Wk, double *Wb);
 /*
  * Learned preconditioner (no counterpart in the reference, whose
  * preconditioner is R/(E0-d), davidson.h:861-868): a per-determinant MLP
- * 160 -> 128 -> 128 -> 1 with GELU, BF16 weights on MFMA with FP32
+ * 160 -> 128 -> 128 -> 1 with GELU (tanh form), BF16 weights on MFMA with FP32
  * accumulation, FP32 master weights trained with Adam by the library's own
  * forward/backward kernels.  Features are BF16, sbdx_mlp_nfeat() per row;
  * parameters are FP32, sbdx_mlp_nparam() in the order W1[160][128], b1[128],
@@ -235,6 +235,11 @@ int sbdx_mlp_collect(sbdx_handle_t h, size_t max_pairs, uint64_t seed, uint16_t 
  * the diagonal step for 3 expansions whenever |r| has not dropped by 10%
  * over the last 3 steps).  Counts of each kind in the last solve: */
 int sbdx_set_preconditioner(sbdx_handle_t h, int mode);
+/* The learned step itself: out[I] = MLP(features of determinant I for the
+ * residual R and Ritz value E), R and out indexed I = ia*n_bdet + ib in the
+ * handle's alpha-first sign convention (the basis sign is not applied);
+ * the solver normalizes R to unit norm before this step. */
+int sbdx_mlp_apply(sbdx_handle_t h, const double *R, double E, double *out);
 int sbdx_precond_stats(sbdx_handle_t h, long *mlp_steps, long *diag_steps);
 
 /* History of the last solve: trace[2*k] = E0, trace[2*k+1] = |r| of step k,

@@ -519,6 +519,20 @@ int sbdx_sigma_sym(sbdx_handle_t h, const double *Wk, double *Wb, int fp32) {
   SBDX_GUARD_END
 }
 
+int sbdx_mlp_apply(sbdx_handle_t h, const double *R, double E, double *out) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!R || !out) throw Error(SBDX_ERR_ARG, "null vector");
+  if (!c.mlp.valid) throw Error(SBDX_ERR_ARG, "no preconditioner weights set");
+  upload(c, c.vin, R);
+  double *dE = c.dsmall + kScalarOffset + 16;
+  SBDX_HIP(hipMemcpyAsync(dE, &E, sizeof(double), hipMemcpyHostToDevice, c.stream));
+  mlp_precond(c, c.vin, dE, c.vout);
+  download(c, out, c.vout);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
 int sbdx_set_verbose(sbdx_handle_t h, int level) {
   SBDX_GUARD_BEGIN
   ctx_of(h).verbose = level;

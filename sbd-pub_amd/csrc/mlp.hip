@@ -2,6 +2,7 @@
 // the reference, whose preconditioner is R/(E0-d), davidson.h:861-868).
 //
 //   x_I (160 features) -> Linear(160,128) -> GELU -> Linear(128,128) -> GELU
+//   (GELU in tanh form, z * sigmoid(2u), u = sqrt(2/pi)(z + 0.044715 z^3))
 //                      -> Linear(128,1) = c_I
 // Weights are kept in FP32 (master copy, Adam state) and used in BF16; all
 // products run on MFMA (v_mfma_f32_32x32x8_bf16) with FP32 accumulation.
@@ -36,15 +37,20 @@ constexpr int kDin = 160, kHid = 128;
 constexpr int kW1 = 0, kB1 = kW1 + kDin * kHid, kW2 = kB1 + kHid, kB2 = kW2 + kHid * kHid,
               kW3 = kB2 + kHid, kB3 = kW3 + kHid, kNParam = kB3 + 4;
 
-__device__ __forceinline__ unsigned short f2bf(float f) {
-  const unsigned u = __float_as_uint(f);
-  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // round to nearest even
-}
+// FP32 -> BF16, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU in tanh form, 0.5 z (1 + tanh(u)) = z * sigmoid(2u) with
+// u = sqrt(2/pi) (z + 0.044715 z^3): one exp and one reciprocal
+constexpr float kGeluC = 0.79788456080286536f;  // sqrt(2/pi)
+__device__ __forceinline__ float gelu_sig(float z) {
+  const float u = kGeluC * z * (1.0f + 0.044715f * z * z);
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
 __device__ __forceinline__ float gelu_d(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-         z * 0.39894228040143268f * expf(-0.5f * z * z);
+  const float s = gelu_sig(z);
+  return s + z * s * (1.0f - s) * 2.0f * kGeluC * (1.0f + 3.0f * 0.044715f * z * z);
 }
 __device__ __forceinline__ int d_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
@@ -169,69 +175,158 @@ __global__ void __launch_bounds__(64) k_mlp_forward(size_t rows, const unsigned 
 // vector (rows = n_alpha*n_beta), features built on the fly.  Four waves per
 // block share the LDS weights; each wave keeps only its X and H1 tiles and
 // reduces the last layer from the MFMA accumulators.
-__global__ void __launch_bounds__(256) k_mlp_precond(size_t N, int nb, const uint64_t *__restrict__ astr,
+// Learned preconditioner over all N determinants: C[I] = MLP(features of I).
+// Computed transposed, H^T = W^T X^T, so that the weights are the A operand
+// (contiguous 8-byte LDS reads from [hidden][k] copies) and the features the
+// B operand, built in registers from the determinant's strings: lane l
+// handles determinant l%32 of a 32-determinant tile and features
+// ks + 4*(l/32) + j; features 136..159 are zero and their k-steps are
+// skipped.  Four independent accumulators (32 hidden units each) keep the
+// matrix cores busy.  H1 goes through a per-wave LDS tile [32][kH1s] so
+// that layer 2 reads its B operand (4 consecutive hidden units of one
+// determinant) as one 8-byte load; layer 3 reduces each determinant's
+// hidden units over the lanes that hold them and the two half-waves.
+// BF16 rounding points as in forward_tile (X, Z1, H1, Z2, H2, w3).
+constexpr int kK1 = 136;           // used layer-1 inputs (134 features, padded to a k-step)
+constexpr int kH1s = kHid + 8;     // H1 tile row stride (BF16), spreads LDS banks
+constexpr int kPrecondWaves = 8;   // waves per block
+__global__ void __launch_bounds__(64 * kPrecondWaves) k_mlp_precond(size_t N, int nb, const uint64_t *__restrict__ astr,
                                                      const uint64_t *__restrict__ bstr,
                                                      const double *__restrict__ R,
                                                      const double *__restrict__ hii,
                                                      const double *__restrict__ E,
                                                      const float *__restrict__ prm, double *__restrict__ C) {
   extern __shared__ __align__(16) unsigned char sm_raw[];
-  unsigned short *W1 = (unsigned short *)sm_raw;
-  unsigned short *W2 = W1 + kDin * kHid;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  unsigned short *X = W2 + kHid * kHid + wave * (32 * kDin + 32 * kHid);
-  unsigned short *H1 = X + 32 * kDin;
-  load_weights(W1, W2, prm);
+  unsigned short *W1t = (unsigned short *)sm_raw;  // [kHid][kK1]
+  unsigned short *W2t = W1t + kHid * kK1;          // [kHid][kHid]
+  float *bias1 = (float *)(W2t + kHid * kHid);     // [kHid]
+  float *bias2 = bias1 + kHid, *w3 = bias2 + kHid;
+  unsigned short *H1all = (unsigned short *)(w3 + kHid);
+  for (int i = threadIdx.x; i < kK1 * kHid; i += blockDim.x) {
+    const int k = i / kHid, h = i % kHid;  // prm W1 is [kDin][kHid]
+    W1t[h * kK1 + k] = f2bf(prm[kW1 + i]);
+  }
+  for (int i = threadIdx.x; i < kHid * kHid; i += blockDim.x) {
+    const int k = i / kHid, h = i % kHid;
+    W2t[h * kHid + k] = f2bf(prm[kW2 + i]);
+  }
+  for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+    bias1[i] = prm[kB1 + i];
+    bias2[i] = prm[kB2 + i];
+    w3[i] = bf2f(f2bf(prm[kW3 + i]));
+  }
   __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 31, half = lane >> 5;
+  unsigned short *H1 = H1all + wave * 32 * kH1s;
   const double e = E[0];
   const double sq = sqrt((double)N);
   const uint64_t a0 = astr[0], b0 = bstr[0];
-  const int col = lane & 31;
-  float w3[4], b2[4], b1[4];
-  for (int nt = 0; nt < 4; nt++) {
-    w3[nt] = bf2f(f2bf(prm[kW3 + 32 * nt + col]));
-    b2[nt] = prm[kB2 + 32 * nt + col];
-    b1[nt] = prm[kB1 + 32 * nt + col];
-  }
   const float b3 = prm[kB3];
   const size_t ntiles = (N + 31) / 32;
-  for (size_t tile = (size_t)blockIdx.x * 4 + wave; tile < ntiles; tile += (size_t)gridDim.x * 4) {
-    const size_t r0 = tile * 32;
-    if (lane < 32) {
-      const size_t I = r0 + lane;
-      if (I < N) {
-        const size_t ia = I / nb, ib = I % nb;
-        features(X + lane * kDin, astr[ia], bstr[ib], a0, b0, R[I] * sq, hii[I] - e);
-      } else {
-        for (int k = 0; k < kDin; k++) X[lane * kDin + k] = 0;
-      }
+  for (size_t tile = (size_t)blockIdx.x * kPrecondWaves + wave; tile < ntiles;
+       tile += (size_t)gridDim.x * kPrecondWaves) {
+    const size_t I = tile * 32 + col;
+    const bool live = I < N;
+    const size_t Ic = live ? I : N - 1;
+    const size_t ia = Ic / nb, ib = Ic % nb;
+    const uint64_t sa = astr[ia], sb = bstr[ib];
+    // scalar features 128..133 exactly as features() writes them
+    const double r_scaled = R[Ic] * sq, hmE = hii[Ic] - e;
+    double den = -hmE;  // E - h_II
+    if (fabs(den) < 1e-3) den = den < 0 ? -1e-3 : 1e-3;
+    const double dstep = fmax(-50.0, fmin(50.0, r_scaled / den));
+    bf16x4 bvs;
+    if (half == 0) {
+      bvs[0] = (short)f2bf((float)r_scaled);
+      bvs[1] = (short)f2bf((float)(hmE / 8.0));
+      bvs[2] = (short)f2bf((float)(dstep / 10.0));
+      bvs[3] = (short)f2bf((float)(__popcll(sa ^ a0) / 8.0));
+    } else {
+      bvs[0] = (short)f2bf((float)(__popcll(sb ^ b0) / 8.0));
+      bvs[1] = (short)f2bf(1.0f);
+      bvs[2] = 0;
+      bvs[3] = 0;
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
+    // layer 1: acc[m] = rows 32m..32m+31 of W1^T X^T
     f32x16 acc[4];
-    gemm_rows(X, kDin, W1, kDin, acc, lane);
-    for (int nt = 0; nt < 4; nt++)
-      for (int r = 0; r < 16; r++) {
-        const int row = d_row(r, lane);
-        H1[row * kHid + 32 * nt + col] = f2bf(gelu(bf2f(f2bf(acc[nt][r] + b1[nt]))));
-      }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-    gemm_rows(H1, kHid, W2, kHid, acc, lane);
-    float part[16];
-    for (int r = 0; r < 16; r++) {
-      float s = 0.f;
-      for (int nt = 0; nt < 4; nt++) s += bf2f(f2bf(gelu(bf2f(f2bf(acc[nt][r] + b2[nt]))))) * w3[nt];
-      part[r] = s;
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
+#pragma unroll 2
+    for (int ks = 0; ks < 128; ks += 8) {
+      const uint64_t bits = ks < 64 ? sa : sb;
+      const int sh = (ks & 63) + 4 * half;
+      bf16x4 bv;
+#pragma unroll
+      for (int j = 0; j < 4; j++) bv[j] = ((bits >> (sh + j)) & 1ull) ? (short)0x3f80 : (short)0;
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+        acc[m] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+            *reinterpret_cast<const bf16x4 *>(W1t + (32 * m + col) * kK1 + ks + 4 * half), bv, acc[m], 0, 0, 0);
     }
-    for (int off = 1; off < 32; off <<= 1)
-      for (int r = 0; r < 16; r++) part[r] += __shfl_xor(part[r], off, 64);
-    if (col == 0)
-      for (int r = 0; r < 16; r++) {
-        const size_t I = r0 + d_row(r, lane);
-        if (I < N) C[I] = (double)(part[r] + b3);
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+      acc[m] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+          *reinterpret_cast<const bf16x4 *>(W1t + (32 * m + col) * kK1 + 128 + 4 * half), bvs, acc[m], 0, 0, 0);
+    // Z1 + b1 in BF16 into the wave's tile, [determinant][hidden]; lane l
+    // holds hidden units 8t + 4*(l/32) + j (t < 16) of determinant l%32
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int h0 = 32 * m + 8 * q + 4 * half;
+        bf16x4 v;
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = (short)f2bf(acc[m][4 * q + j] + bias1[h0 + j]);
+        *reinterpret_cast<bf16x4 *>(H1 + col * kH1s + h0) = v;
       }
+    // H1 = GELU(Z1) in place, one 4-unit group at a time (rolled: few registers)
+    for (int t = 0; t < 16; t++) {
+      bf16x4 *p4 = reinterpret_cast<bf16x4 *>(H1 + col * kH1s + 8 * t + 4 * half);
+      bf16x4 v = *p4;
+#pragma unroll
+      for (int j = 0; j < 4; j++) v[j] = (short)f2bf(gelu(bf2f((unsigned short)v[j])));
+      *p4 = v;
+    }
     __builtin_amdgcn_wave_barrier();
+    // layer 2: acc[m] = rows 32m..32m+31 of W2^T H1^T
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
+#pragma unroll 2
+    for (int ks = 0; ks < kHid; ks += 8) {
+      const bf16x4 bv = *reinterpret_cast<const bf16x4 *>(H1 + col * kH1s + ks + 4 * half);
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+        acc[m] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+            *reinterpret_cast<const bf16x4 *>(W2t + (32 * m + col) * kHid + ks + 4 * half), bv, acc[m], 0, 0, 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // Z2 + b2 in BF16 into the same tile (layer 2 has read H1)
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int h0 = 32 * m + 8 * q + 4 * half;
+        bf16x4 v;
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = (short)f2bf(acc[m][4 * q + j] + bias2[h0 + j]);
+        *reinterpret_cast<bf16x4 *>(H1 + col * kH1s + h0) = v;
+      }
+    // layer 3: y = sum_h w3[h] GELU(Z2[h]) + b3 over both half-waves
+    float s = 0.f;
+    for (int t = 0; t < 16; t++) {
+      const int h0 = 8 * t + 4 * half;
+      const bf16x4 v = *reinterpret_cast<const bf16x4 *>(H1 + col * kH1s + h0);
+#pragma unroll
+      for (int j = 0; j < 4; j++) s += bf2f(f2bf(gelu(bf2f((unsigned short)v[j])))) * w3[h0 + j];
+    }
+    __builtin_amdgcn_wave_barrier();
+    s += __shfl_xor(s, 32, 64);
+    if (half == 0 && live) C[I] = (double)(s + b3);
   }
 }
 
@@ -459,10 +554,12 @@ void mlp_adam_step(Context &c, float lr) {
 }
 
 void mlp_precond(Context &c, const double *R, const double *E, double *Cout) {
-  const size_t lds = sizeof(unsigned short) * ((size_t)kDin * kHid + kHid * kHid + 4 * (32 * kDin + 32 * kHid));
+  const size_t lds = sizeof(unsigned short) * ((size_t)kK1 * kHid + kHid * kHid) + sizeof(float) * 3 * kHid +
+                     sizeof(unsigned short) * kPrecondWaves * 32 * kH1s;
   SBDX_HIP(hipFuncSetAttribute((const void *)k_mlp_precond, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const unsigned g = (unsigned)std::min<size_t>((c.N + 127) / 128, (size_t)c.n_cu * 2);
-  hipLaunchKernelGGL(k_mlp_precond, dim3(g), dim3(256), lds, c.stream, c.N, (int)c.nb, c.A.str, c.B.str,
+  const unsigned g = (unsigned)std::min<size_t>((c.N + 32 * kPrecondWaves - 1) / (32 * kPrecondWaves),
+                                                (size_t)c.n_cu * 2);
+  hipLaunchKernelGGL(k_mlp_precond, dim3(g), dim3(64 * kPrecondWaves), lds, c.stream, c.N, (int)c.nb, c.A.str, c.B.str,
                      R, c.hii, E, c.mlp.prm, Cout);
   check_launch("k_mlp_precond");
 }

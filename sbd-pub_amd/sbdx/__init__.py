@@ -45,7 +45,7 @@ ABI_SYMBOLS = [
     "sbdx_set_preconditioner", "sbdx_precond_stats", "sbdx_set_verbose",
     "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
     "sbdx_mixed_staging", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry",
-    "sbdx_sigma_sym",
+    "sbdx_sigma_sym", "sbdx_mlp_apply",
 ]
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -95,6 +95,7 @@ def lib():
     L.sbdx_sigma_block.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
     L.sbdx_symmetric_eig.argtypes = [C.c_int, _dp, _dp, _dp]
     L.sbdx_set_symmetry.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_mlp_apply.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
     L.sbdx_sigma_sym.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
     L.sbdx_sigma_block_fp32.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.sbdx_davidson_block.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_double,
@@ -234,6 +235,13 @@ class TPBOperator:
         n = C.c_size_t()
         _check(lib().sbdx_mlp_collect(self.h, max_pairs, seed, feat, tgt, C.byref(n)))
         return feat[: n.value * nf].reshape(n.value, nf), tgt[: n.value]
+
+    def mlp_apply(self, R, E):
+        """The learned step for residual R (alpha-first order) at Ritz value E."""
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        out = np.zeros_like(R)
+        _check(lib().sbdx_mlp_apply(self.h, R, float(E), out))
+        return out
 
     def set_preconditioner(self, mode):
         """0: diagonal R/(E0-d); 1: learned MLP with diagonal fallback on stall."""

@@ -1,14 +1,13 @@
 """numpy reference of the learned preconditioner's MLP (test code).
 
 Same network and parameter layout as sbd-pub_amd/csrc/mlp.hip:
-160 -> 128 -> 128 -> 1, GELU (erf form), BF16 weights and BF16 rounding of
+160 -> 128 -> 128 -> 1, GELU (tanh form), BF16 weights and BF16 rounding of
 X, Z1, H1, Z2, H2, dZ1, dZ2 at the same points as the kernels (rounding
 treated as identity in the derivative); accumulation in float64 here
 (FP32 on the GPU).  With bf16=False everything stays float64, which the
 finite-difference test uses to pin the backward formulas themselves.
 """
 import numpy as np
-from scipy.special import erf
 
 DIN, HID = 160, 128
 W1, B1 = 0, DIN * HID
@@ -36,12 +35,22 @@ def from_bits(b):
     return (np.asarray(b, dtype=np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
+C_GELU = np.sqrt(2.0 / np.pi)
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    """Tanh-form GELU, 0.5 z (1 + tanh(u)) = z * sigmoid(2u),
+    u = sqrt(2/pi) (z + 0.044715 z^3)."""
+    z = np.asarray(z, dtype=np.float64)
+    u = C_GELU * (z + 0.044715 * z ** 3)
+    return z / (1.0 + np.exp(-2.0 * u))
 
 
 def gelu_d(z):
-    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+    z = np.asarray(z, dtype=np.float64)
+    u = C_GELU * (z + 0.044715 * z ** 3)
+    s = 1.0 / (1.0 + np.exp(-2.0 * u))
+    return s + z * s * (1.0 - s) * 2.0 * C_GELU * (1.0 + 3.0 * 0.044715 * z * z)
 
 
 def unpack(p):

@@ -86,6 +86,45 @@ class TestGPU:
         loss = op.mlp_train(M.bf16_bits(X), t, epochs=12, batch=512, lr=2e-3, seed=1)
         assert loss[-1] < 0.2 * loss[0], loss
 
+    def test_precond_kernel_matches_forward(self, op):
+        """The solver's fused inference kernel (features built in registers,
+        transposed two-layer MFMA chain) against the training-side forward
+        kernel on the same features built here."""
+        m = molecule("h2o", "1em3")
+        N = op.ndet
+        nb = len(m["bstr"])
+        op.mlp_set_weights(M.init_params(5))
+        rng = np.random.default_rng(8)
+        R = rng.standard_normal(N)
+        R /= np.linalg.norm(R)
+        hii = op.make_diag()
+        E = float(hii.min()) - 0.05
+        out = op.mlp_apply(R, E)
+        ia, ib = np.divmod(np.arange(N), nb)
+        a = np.asarray(m["astr"], dtype=np.uint64)[ia]
+        b = np.asarray(m["bstr"], dtype=np.uint64)[ib]
+        X = np.zeros((N, M.DIN), dtype=np.uint16)
+        for k in range(64):
+            X[:, k] = np.where((a >> np.uint64(k)) & np.uint64(1), 0x3F80, 0)
+            X[:, 64 + k] = np.where((b >> np.uint64(k)) & np.uint64(1), 0x3F80, 0)
+        rs = R * np.sqrt(N)
+        hme = hii - E
+        den = -hme
+        den = np.where(np.abs(den) < 1e-3, np.where(den < 0, -1e-3, 1e-3), den)
+        dstep = np.clip(rs / den, -50.0, 50.0)
+        pc = lambda v: np.array([bin(int(x)).count("1") for x in v], dtype=np.float64)
+        X[:, 128] = M.bf16_bits(rs.astype(np.float32))
+        X[:, 129] = M.bf16_bits((hme / 8.0).astype(np.float32))
+        X[:, 130] = M.bf16_bits((dstep / 10.0).astype(np.float32))
+        X[:, 131] = M.bf16_bits((pc(a ^ a[0]) / 8.0).astype(np.float32))
+        X[:, 132] = M.bf16_bits((pc(b ^ b[0]) / 8.0).astype(np.float32))
+        X[:, 133] = 0x3F80
+        ref = op.mlp_forward(X)
+        err = np.abs(out - ref)
+        scale = np.max(np.abs(ref))
+        assert np.median(err) < 1e-4 * scale
+        assert np.max(err) < 2e-2 * scale
+
     def test_collect_and_preconditioned_solve(self, op):
         import sbdx
         W0 = sbdx.basis_init_vector(op.ndet)
