@@ -62,6 +62,21 @@ def kernel_bytes(wc, na, nb, e=8.0):
     }
 
 
+def profiled_traffic(name, width, slots):
+    """HBM bytes per launch of kernel group `name` at the given mixed-kernel
+    configuration, from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
+    profiles/ (None when that configuration was not profiled)."""
+    path = os.path.join(ROOT, "profiles", "c36_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    want = f"eb={width} nt={slots}"
+    for line in open(path):
+        f = line.split()
+        if len(f) == 5 and f[0] == "traffic_bytes" and f[1] == name and f"{f[2]} {f[3]}" == want:
+            return float(f[4])
+    return None
+
+
 def cpu_baseline(inst, n_steps, seconds, nthreads):
     """Oracle sigma on a random sample of alpha rows, scaled to one full
     sigma and multiplied by the GPU solve's sigma count."""
@@ -110,6 +125,9 @@ def main():
     ap.add_argument("--no-symmetry", dest="symmetry", action="store_false",
                     help="general sigma in the optimized solver (default: spin-flip symmetric "
                          "form when the string sets are equal)")
+    ap.add_argument("--coarse", type=int, default=500,
+                    help="coarse start on the first M strings of each set (optimized solver; 0: off)")
+    ap.add_argument("--coarse-eps", type=float, default=1e-2)
     ap.add_argument("--keep", type=int, default=4,
                     help="Ritz vectors kept at a thick restart (thick / mixed solvers)")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
@@ -164,6 +182,7 @@ def main():
         if solver in ("thick", "mixed"):
             op.set_restart(args.keep)
             op.set_symmetry(args.symmetry and same_sets)
+            op.set_coarse_start(args.coarse if args.coarse < min(na, nb) else 0, args.coarse_eps)
         op.set_preconditioner(1 if precond == "mlp" else 0)
         op.set_verbose(0 if args.quiet else 1)
         op.profile(True)
@@ -184,12 +203,14 @@ def main():
             nsteps.append(n)
             conv.append(ok)
             tr = op.trace()
+            extra = f" coarse={op.coarse_stats()}" if solver in ("thick", "mixed") else ""
             log(f"{solver}/{precond} solve {k}: E0={E:.12f} |r|={tr[-1, 1]:.2e} sigma-products={n} "
-                f"converged={ok} time={dt:.3f} s precond-steps={op.precond_stats()}")
+                f"converged={ok} time={dt:.3f} s precond-steps={op.precond_stats()}{extra}")
         stats = op.profile_stats()
         op.profile(False)
         return dict(time=float(np.median(times)), E=energies[-1], steps=int(np.median(nsteps)),
-                    converged=all(conv), stats=stats, precond_steps=op.precond_stats())
+                    converged=all(conv), stats=stats, precond_steps=op.precond_stats(),
+                    coarse=op.coarse_stats() if solver in ("thick", "mixed") else None)
 
     best = args.solver
     plain = run("plain", 1 if best != "plain" else args.steps) if args.compare or best == "plain" else None
@@ -226,7 +247,7 @@ def main():
 
     def sigma_rate(kernels, suffix):
         names = [n + suffix for n in SIGMA_PARTS if n + suffix in kernels]
-        if not any(n.startswith("mixed") for n in names):
+        if not any("mixed" in n for n in names):
             return None, None
         ms = sum(kernels[n]["avg_ms"] for n in names)
         by = sum(kernels[n]["alg_bytes"] for n in names)
@@ -241,8 +262,15 @@ def main():
         roofline = dict(kernel=dom, bound="mfma", achieved=dk["TFLOPs"], peak=peak, unit="TFLOP/s",
                         frac=round(dk["TFLOPs"] / peak, 4), traffic=None)
     else:
+        forms = op.same_spin_forms()["fp32" if dom.endswith("_f32") else "fp64"]
+        traffic = None
+        if args.workload == "c36":
+            traffic = profiled_traffic(dom, forms["mixed_width"], forms["mixed_slots_per_thread"])
         roofline = dict(kernel=dom, bound="hbm", achieved=dk["GBps"], peak=HBM_PEAK_GBPS,
-                        unit="GB/s", frac=round(dk["GBps"] / HBM_PEAK_GBPS, 4), traffic=None)
+                        unit="GB/s", frac=round(dk["GBps"] / HBM_PEAK_GBPS, 4), traffic=traffic,
+                        traffic_source="profiles/c36_hbm_traffic.md (rocprofv3 FETCH_SIZE x2.0 + "
+                                       "WRITE_SIZE x1.0, calibrated on streaming kernels)"
+                        if traffic else None)
     s64_ms, s64_gbps = sigma_rate(kernels, "")
     s32_ms, s32_gbps = sigma_rate(kernels, "_f32")
     if plain is not None:
@@ -273,6 +301,8 @@ def main():
             "subspace": args.nb,
             "restart_keep": args.keep if best in ("thick", "mixed") else None,
             "spin_flip_symmetric": bool(args.symmetry and same_sets and best in ("thick", "mixed")),
+            "coarse_start": None if res["coarse"] is None or not args.coarse else
+            dict(strings=args.coarse, eps=args.coarse_eps, **res["coarse"]),
             "preconditioner": args.precond,
             "precond_steps": res["precond_steps"],
             "sigma_products": res["steps"],

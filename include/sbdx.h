@@ -193,6 +193,18 @@ int sbdx_set_symmetry(sbdx_handle_t h, int on);
  * symmetric form, FP64 (fp32 = 0) or FP32 arithmetic; Wb is overwritten. */
 int sbdx_sigma_sym(sbdx_handle_t h, const double *Wk, double *Wb, int fp32);
 
+/* Coarse start of the thick-restart solves (no counterpart in the
+ * reference): before the full solve, the start vector restricted to the
+ * first m alpha and m beta strings (the strings are sorted, so these hold
+ * the lowest excitations) is refined on that m x m sub-problem (same
+ * integrals and solver settings, diagonal preconditioner) until its
+ * residual is below eps, then lifted back (zero on the other determinants)
+ * as the full solve's start.  The full solve still converges to eps on
+ * exact FP64 products.  m = 0 (default) switches it off.  Statistics of the
+ * last solve's coarse stage: sub-problem products, energy, wall seconds. */
+int sbdx_set_coarse_start(sbdx_handle_t h, size_t m, double eps);
+int sbdx_coarse_stats(sbdx_handle_t h, int *products, double *energy, double *seconds);
+
 /* verbose != 0: the solves print one line per step to stderr, step index,
  * E0 and |r| (as davidson.h:839-851 prints per step); thick-restart solves
  * mark the exact FP64 residual of each restart.  0 (default) prints nothing. */

@@ -129,6 +129,9 @@ static void download(Context &c, double *h, const double *d) {
   SBDX_HIP(hipStreamSynchronize(c.stream));
 }
 
+static Context *create_core(int L, std::vector<uint64_t> as, std::vector<uint64_t> bs, double I0,
+                            std::vector<double> h1, std::vector<double> eri);
+
 static Context *create(int L, size_t bl, const uint64_t *aw, size_t na, const uint64_t *bw, size_t nb,
                        double I0, const double *I1, const double *I2, size_t nI2) {
   if (L < 1 || L > 64) throw Error(SBDX_ERR_UNSUPPORTED, "norb must be in [1, 64]");
@@ -139,6 +142,26 @@ static Context *create(int L, size_t bl, const uint64_t *aw, size_t na, const ui
     throw Error(SBDX_ERR_UNSUPPORTED, "string sets are limited to 2^32-1 strings");
   if (nI2 != i2_size(L))
     throw Error(SBDX_ERR_ARG, "n_I2 must be B(B+1)/2 with B = L(L+1)/2 (" + std::to_string(i2_size(L)) + ")");
+  // integrals: spatial one-body from the alpha block of I1, dense (pq|rs)
+  const size_t K = (size_t)L * L;
+  std::vector<double> h1(K), eri(K * K);
+  for (int p = 0; p < L; p++)
+    for (int q = 0; q < L; q++) h1[p * L + q] = I1[(size_t)(2 * p) * (2 * L) + 2 * q];
+  for (int p = 0; p < L; p++)
+    for (int q = 0; q < L; q++) {
+      const size_t pq = tri(p, q);
+      for (int r = 0; r < L; r++)
+        for (int s = 0; s < L; s++) eri[(p * L + q) * K + r * L + s] = I2[tri(pq, tri(r, s))];
+    }
+  return create_core(L, unpack_words(aw, na, bl, L), unpack_words(bw, nb, bl, L), I0, std::move(h1),
+                     std::move(eri));
+}
+
+// A context from unpacked strings and dense spatial integrals (kept on the
+// host too, for sub-contexts of the coarse start).
+static Context *create_core(int L, std::vector<uint64_t> as, std::vector<uint64_t> bs, double I0,
+                            std::vector<double> h1, std::vector<double> eri) {
+  const size_t na = as.size(), nb = bs.size();
   auto ctx = std::make_unique<Context>();
   Context &c = *ctx;
   try {
@@ -161,23 +184,11 @@ static Context *create(int L, size_t bl, const uint64_t *aw, size_t na, const ui
     c.nb = nb;
     c.N = na * nb;
     c.I0 = I0;
-    // integrals: spatial one-body from the alpha block of I1, dense (pq|rs)
     const size_t K = (size_t)c.K;
-    std::vector<double> h1(K), eri(K * K);
-    for (int p = 0; p < L; p++)
-      for (int q = 0; q < L; q++) h1[p * L + q] = I1[(size_t)(2 * p) * (2 * L) + 2 * q];
-    for (int p = 0; p < L; p++)
-      for (int q = 0; q < L; q++) {
-        const size_t pq = tri(p, q);
-        for (int r = 0; r < L; r++)
-          for (int s = 0; s < L; s++) eri[(p * L + q) * K + r * L + s] = I2[tri(pq, tri(r, s))];
-      }
     c.h1 = (double *)dalloc(c, sizeof(double) * K);
     c.eri = (double *)dalloc(c, sizeof(double) * K * K);
     SBDX_HIP(hipMemcpyAsync(c.h1, h1.data(), sizeof(double) * K, hipMemcpyHostToDevice, c.stream));
     SBDX_HIP(hipMemcpyAsync(c.eri, eri.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, c.stream));
-    const auto as = unpack_words(aw, na, bl, L);
-    const auto bs = unpack_words(bw, nb, bl, L);
     build_spin_tables(c, c.A, as);
     build_singles_ell(c, c.A);
     if (as == bs) {
@@ -199,12 +210,27 @@ static Context *create(int L, size_t bl, const uint64_t *aw, size_t na, const ui
     c.dsmall = (double *)dalloc(c, sizeof(double) * kSmallDoubles);
     SBDX_HIP(hipMemsetAsync(c.dsmall, 0, sizeof(double) * kSmallDoubles, c.stream));
     SBDX_HIP(hipStreamSynchronize(c.stream));
+    c.h_astr = std::move(as);
+    c.h_bstr = std::move(bs);
+    c.h_h1 = std::move(h1);
+    c.h_eri = std::move(eri);
   } catch (...) {
     dfree_all(c);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     throw;
   }
   return ctx.release();
+}
+
+// The problem restricted to the first m alpha and m beta strings (same
+// integrals), for the coarse start of the thick-restart solves.
+Context *make_sub_context(const Context &c, size_t m) {
+  if (m < 1 || m > c.h_astr.size() || m > c.h_bstr.size()) throw Error(1, "sub-context size out of range");
+  std::vector<uint64_t> as(c.h_astr.begin(), c.h_astr.begin() + m);
+  std::vector<uint64_t> bs(c.h_bstr.begin(), c.h_bstr.begin() + m);
+  Context *s = create_core(c.L, std::move(as), std::move(bs), c.I0, c.h_h1, c.h_eri);
+  s->lds_limit = c.lds_limit;
+  return s;
 }
 
 static void destroy(Context *c) {
@@ -217,6 +243,8 @@ static void destroy(Context *c) {
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
+
+void destroy_context(Context *c) { destroy(c); }
 
 }  // namespace sbdx
 
@@ -508,7 +536,7 @@ int sbdx_sigma_sym(sbdx_handle_t h, const double *Wk, double *Wb, int fp32) {
   if (!sigma_sym_available(c)) throw Error(SBDX_ERR_ARG, "spin-flip symmetry not available for this handle");
   upload(c, c.vout, Wk);
   sign_convert(c, c.vout, c.vin);
-  if (sym_defect(c, c.vin) > 1e-12) throw Error(SBDX_ERR_ARG, "vector is not spin-flip symmetric");
+  if (sym_defect(c, c.vin) > 1e-6) throw Error(SBDX_ERR_ARG, "vector is not spin-flip symmetric");
   if (fp32)
     sigma_sym_f32(c, c.vin, c.vout);
   else
@@ -529,6 +557,27 @@ int sbdx_mlp_apply(sbdx_handle_t h, const double *R, double E, double *out) {
   SBDX_HIP(hipMemcpyAsync(dE, &E, sizeof(double), hipMemcpyHostToDevice, c.stream));
   mlp_precond(c, c.vin, dE, c.vout);
   download(c, out, c.vout);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_set_coarse_start(sbdx_handle_t h, size_t m, double eps) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (m > 0 && (m >= c.na || m >= c.nb)) throw Error(SBDX_ERR_ARG, "coarse start needs m below both string counts");
+  if (!(eps > 0.0)) throw Error(SBDX_ERR_ARG, "eps must be positive");
+  c.coarse_m = m;
+  c.coarse_eps = eps;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_coarse_stats(sbdx_handle_t h, int *products, double *energy, double *seconds) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (products) *products = c.coarse_products;
+  if (energy) *energy = c.coarse_energy;
+  if (seconds) *seconds = c.coarse_seconds;
   return SBDX_OK;
   SBDX_GUARD_END
 }

@@ -64,12 +64,6 @@ struct SpinTables {
   uint32_t *sl_len = nullptr;  // [nslice]
   int nslice = 0;
   bool has_ell = false;
-  // the same singles with slices in row order (identity perm), built on
-  // first use by the symmetric mixed kernel
-  uint32_t *nell = nullptr;
-  uint64_t *nsl_off = nullptr;
-  uint32_t *nsl_len = nullptr;
-  bool has_nell = false;
   // The same singles split by ket range into `nparts` parts of `part_h`
   // kets (ket stored relative to its part), same perm and slices; built on
   // demand when one ket row does not fit in LDS.
@@ -142,6 +136,16 @@ struct Context {
   int precision = 0;       // 0: FP64 throughout; 1: FP32 sigma on corrections, FP64 restarts
   int verbose = 0;         // 1: one stderr line per Davidson step (E0, |r|)
   int symmetric = 0;       // 1: thick-restart solves use the spin-flip symmetric sigma
+  // Coarse start of the thick-restart solves: the problem on the first
+  // coarse_m strings of each set solved to coarse_eps first (0: off), and
+  // what it cost in the last solve.
+  size_t coarse_m = 0;
+  double coarse_eps = 1e-2;
+  int coarse_products = 0;
+  double coarse_energy = 0.0, coarse_seconds = 0.0;
+  // host copies of the unpacked strings and dense spatial integrals
+  std::vector<uint64_t> h_astr, h_bstr;
+  std::vector<double> h_h1, h_eri;
   int restart_keep = 0;    // 0: restart from W alone (reference); k: thick restart keeping k Ritz vectors
   std::vector<double *> spareY, spareHY;  // thick-restart staging vectors
   // Block Davidson: contiguous basis and products (blk_m vectors each), the
@@ -188,6 +192,8 @@ constexpr int kCoefOffset = kScalarOffset + 64;  // kNbMax projection coefficien
 void *dalloc(Context &c, size_t bytes);
 void dfree_all(Context &c);
 void dfree(Context &c, void *p);  // one dalloc'ed buffer
+Context *make_sub_context(const Context &c, size_t m);  // first m strings of each set
+void destroy_context(Context *c);
 void *blas_handle(Context &c);     // rocblas_handle bound to the stream
 
 // Device scratch released at scope exit (callers synchronize first).
@@ -213,7 +219,6 @@ void check_launch(const char *what);
 void build_spin_tables(Context &c, SpinTables &t, const std::vector<uint64_t> &strs);
 void build_singles_ell(Context &c, SpinTables &t);
 void build_singles_ell_parts(Context &c, SpinTables &t, int nparts);
-void build_singles_ell_natural(Context &c, SpinTables &t);
 void compute_diag(Context &c);
 void sign_convert(Context &c, const double *in, double *out);      // out = S*in
 void sign_convert_add(Context &c, const double *in, double *out);  // out += S*in

@@ -135,11 +135,13 @@ __global__ void __launch_bounds__(256) k_transpose(const T *__restrict__ in, T *
 // ERI row's diagonal (zero in the spectator row).  A row's slots may be
 // split over gridDim.y blocks of 1024*NT slots each (fewer registers per
 // thread, more resident waves, the row staged once per block).
-// SYM: the table is in row order (slot = beta row) and only rows r >= I are
-// produced, added at y[I,r] and mirrored at y[r,I] (a spin-flip symmetric
-// vector gives a symmetric product; element (a,b) belongs to block min(a,b),
-// so no two blocks write the same element); blocks start at the slice of
-// row I and exit when their chunk lies past the table.
+// SYM (equal string sets, symmetric x): block p handles bra I = perm[p] and
+// only the slots s >= p of the length-sorted table, adding each value at
+// y[I,r] and, for s > p, at y[r,I] (r = perm[s]): every unordered pair
+// {I, r} is produced once, and element (a,b) is written by exactly one block
+// (a's when pos(b) >= pos(a), b's otherwise).  Blocks start at the slice of
+// position p and exit when their chunk lies past the table; p ascending
+// puts the longest rows first.
 template <typename T, int NT, int EB, bool SYM>
 __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int nslots,
                                                     const uint64_t *__restrict__ a_off,
@@ -156,8 +158,9 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
   T *lds = reinterpret_cast<T *>(lds_raw);
   const int ldr = nb + K + 1;
   const int L1 = L + 1;
-  const int I = blockIdx.x;
-  const int base = (SYM ? (I & ~63) : 0) + blockIdx.y * (1024 * NT);
+  const int pos = blockIdx.x;
+  const int I = SYM ? (int)perm[pos] : blockIdx.x;
+  const int base = (SYM ? (pos & ~63) : 0) + blockIdx.y * (1024 * NT);
   if (SYM && base >= nslots) return;  // uniform per block, before any barrier
   const int tid = base + threadIdx.x;
   T acc[NT];
@@ -167,8 +170,7 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
   for (int t = 0; t < NT; t++) {
     acc[t] = T(0);
     const int slot = tid + t * 1024;
-    uint32_t r = 0xffffffffu;
-    if (slot < nslots) r = SYM ? (slot < nb ? (uint32_t)slot : 0xffffffffu) : perm[slot];
+    const uint32_t r = slot < nslots ? perm[slot] : 0xffffffffu;
     rr[t] = r != 0xffffffffu ? (int)r : 0;
     occ[t] = r != 0xffffffffu ? b_str[r] : 0ull;  // padding slots: no dressing
   }
@@ -228,9 +230,10 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
     const int slot = tid + t * 1024;
     if (slot < nslots) {
       if (SYM) {
-        if (slot < nb && slot >= I) {
-          y[(size_t)I * nb + slot] += acc[t];
-          if (slot > I) y[(size_t)slot * nb + I] += acc[t];
+        const uint32_t r = perm[slot];
+        if (r != 0xffffffffu && slot >= pos) {
+          y[(size_t)I * nb + r] += acc[t];
+          if (slot > pos) y[(size_t)r * nb + I] += acc[t];
         }
       } else {
         const uint32_t r = perm[slot];
@@ -550,8 +553,8 @@ __global__ void __launch_bounds__(256) k_densify(int n, const uint64_t *__restri
 }
 
 // Spin-flip symmetric form of the same-spin terms for equal string sets:
-// y = hii*x + G + G^T with G = H X (n x n row-major), 64x64 tiles, G^T
-// through LDS.
+// y = hii*x + (G + G^T) with G = H X (n x n row-major), 64x64 tiles, G^T
+// through LDS; for symmetric x the result is bitwise symmetric.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sym_combine(int n, const T *__restrict__ hii, const T *__restrict__ x,
                                                      const T *__restrict__ G, T *__restrict__ y) {
@@ -567,7 +570,7 @@ __global__ void __launch_bounds__(256) k_sym_combine(int n, const T *__restrict_
     const int r = r0 + k, cc = c0 + tx;
     if (r < n && cc < n) {
       const size_t o = (size_t)r * n + cc;
-      y[o] = hii[o] * x[o] + G[o] + tile[tx][k];
+      y[o] = hii[o] * x[o] + (G[o] + tile[tx][k]);  // G + G^T commutes: bitwise symmetric
     }
   }
 }
@@ -683,14 +686,9 @@ static void launch_mixed_ell(Context &c, const T *eri, const T *x, T *y, size_t 
   SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int nslots = c.B.nslice * 64;
   const unsigned nsplit = cdiv(nslots, 1024 * NT);
-  if (SYM)
-    hipLaunchKernelGGL(fn, dim3((unsigned)c.na, nsplit), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
-                       nslots, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.nell, (const uint32_t *)nullptr,
-                       c.B.nsl_off, c.B.nsl_len, eri, x, y);
-  else
-    hipLaunchKernelGGL(fn, dim3((unsigned)c.na, nsplit), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
-                       nslots, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell, c.B.perm, c.B.sl_off,
-                       c.B.sl_len, eri, x, y);
+  hipLaunchKernelGGL(fn, dim3((unsigned)c.na, nsplit), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
+                     nslots, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell, c.B.perm, c.B.sl_off,
+                     c.B.sl_len, eri, x, y);
 }
 
 // EB pairs staged per round, NT slots per thread (1, 2, 4 or 8); false when
@@ -1159,7 +1157,6 @@ static void sigma_sym_impl(Context &c, const T *x, T *y) {
   }
   {
     ProfScope ps(c, f32 ? "sym_mixed_f32" : "sym_mixed");
-    build_singles_ell_natural(c, c.B);
     const int pi = f32 ? 1 : 0;
     const int eb = mixed_cfg<T>(c, o.eri, x);
     const int nt = c.mixed_nt[pi];

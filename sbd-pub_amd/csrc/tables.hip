@@ -343,28 +343,6 @@ void build_singles_ell(Context &c, SpinTables &t) {
   t.has_ell = true;
 }
 
-// The same singles with slices in row order (slice s = rows 64s..64s+63),
-// for the symmetric mixed kernel, which walks only rows r >= I of bra I.
-void build_singles_ell_natural(Context &c, SpinTables &t) {
-  if (t.has_nell || !t.has_ell) return;
-  std::vector<uint64_t> off, ent;
-  download_singles(c, t, off, ent);
-  std::vector<uint32_t> order(t.n);
-  for (int i = 0; i < t.n; i++) order[i] = (uint32_t)i;
-  std::vector<uint32_t> perm, sl_len, ell;
-  std::vector<uint64_t> sl_off;
-  ell_layout(c, t, off, ent, order, perm, sl_len, sl_off, ell);
-  const int nsl = (int)sl_len.size();
-  t.nell = (uint32_t *)dalloc(c, sizeof(uint32_t) * ell.size());
-  t.nsl_off = (uint64_t *)dalloc(c, sizeof(uint64_t) * nsl);
-  t.nsl_len = (uint32_t *)dalloc(c, sizeof(uint32_t) * nsl);
-  SBDX_HIP(hipMemcpyAsync(t.nell, ell.data(), sizeof(uint32_t) * ell.size(), hipMemcpyHostToDevice, c.stream));
-  SBDX_HIP(hipMemcpyAsync(t.nsl_off, sl_off.data(), sizeof(uint64_t) * nsl, hipMemcpyHostToDevice, c.stream));
-  SBDX_HIP(hipMemcpyAsync(t.nsl_len, sl_len.data(), sizeof(uint32_t) * nsl, hipMemcpyHostToDevice, c.stream));
-  SBDX_HIP(hipStreamSynchronize(c.stream));
-  t.has_nell = true;
-}
-
 // Ket-range parts of the sliced-ELL singles (SpinTables::pell): part p holds
 // the entries with ket in [p*h, (p+1)*h), h = ceil(n/nparts), ket stored as
 // ket - p*h, with the slices and row order of t.perm.

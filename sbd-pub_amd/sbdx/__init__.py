@@ -45,7 +45,7 @@ ABI_SYMBOLS = [
     "sbdx_set_preconditioner", "sbdx_precond_stats", "sbdx_set_verbose",
     "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
     "sbdx_mixed_staging", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry",
-    "sbdx_sigma_sym", "sbdx_mlp_apply",
+    "sbdx_sigma_sym", "sbdx_mlp_apply", "sbdx_set_coarse_start", "sbdx_coarse_stats",
 ]
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -95,6 +95,9 @@ def lib():
     L.sbdx_sigma_block.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
     L.sbdx_symmetric_eig.argtypes = [C.c_int, _dp, _dp, _dp]
     L.sbdx_set_symmetry.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_set_coarse_start.argtypes = [C.c_void_p, C.c_size_t, C.c_double]
+    L.sbdx_coarse_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
     L.sbdx_mlp_apply.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
     L.sbdx_sigma_sym.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
     L.sbdx_sigma_block_fp32.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
@@ -267,6 +270,16 @@ class TPBOperator:
         out = np.zeros_like(W)
         _check(lib().sbdx_sigma_sym(self.h, W, out, 1 if fp32 else 0))
         return out
+
+    def set_coarse_start(self, m, eps=1e-2):
+        """Refine the start vector on the first m strings of each set first
+        (thick-restart solves); m = 0 switches it off."""
+        _check(lib().sbdx_set_coarse_start(self.h, int(m), float(eps)))
+
+    def coarse_stats(self):
+        n, e, t = C.c_int(), C.c_double(), C.c_double()
+        _check(lib().sbdx_coarse_stats(self.h, C.byref(n), C.byref(e), C.byref(t)))
+        return dict(products=n.value, energy=e.value, seconds=round(t.value, 4))
 
     def set_verbose(self, level):
         """Nonzero: one stderr line per Davidson step (E0, |r|)."""

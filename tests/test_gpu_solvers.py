@@ -56,3 +56,32 @@ def test_solver_matches_plain(case, solver, oracle_mod):
     r = op.sigma(W) - E * W
     assert np.linalg.norm(r) < 5e-9
     assert abs(op.energy(W) - ref["E"]) < 1e-9
+
+
+# Coarse start: the start vector refined on the first m strings of each set
+# before the full solve; the full solve must still reach the oracle's energy
+# (symmetric mixed form on equal sets, general thick form on distinct sets).
+@pytest.mark.parametrize("name,solver,sym,m", [("h2o_1em3", "mixed", True, 100),
+                                              ("tiny_ab", "thick", False, 10)])
+def test_coarse_start_reaches_oracle(name, solver, sym, m, oracle_mod):
+    import sbdx
+    from test_gpu_parity import CASES
+    inst = CASES[name]()
+    op = sbdx.TPBOperator.from_instance(inst)
+    orc = oracle_mod.Oracle.from_instance(inst)
+    W0 = sbdx.basis_init_vector(op.ndet)
+    ref = orc.davidson(W=W0, nb=12, eps=1e-9, max_it=300)
+    assert ref["converged"]
+    print(f"[coarse] {name} {solver} m={m}", flush=True)
+    op.configure(solver)
+    op.set_symmetry(sym)
+    op.set_coarse_start(m, 1e-2)
+    op.set_vector(W0)
+    E, steps, conv = op.solve(max_it=300, nb=12, eps=1e-9)
+    st = op.coarse_stats()
+    assert conv and st["products"] > 0
+    assert abs(E - ref["E"]) < 1e-9
+    W = op.get_vector()
+    assert np.linalg.norm(orc.sigma(W) - E * W) < 5e-9
+    op.close()
+    orc.close()
