@@ -200,10 +200,18 @@ int sbdx_sigma_sym(sbdx_handle_t h, const double *Wk, double *Wb, int fp32);
  * integrals and solver settings, diagonal preconditioner) until its
  * residual is below eps, then lifted back (zero on the other determinants)
  * as the full solve's start.  The full solve still converges to eps on
- * exact FP64 products.  m = 0 (default) switches it off.  Statistics of the
+ * exact FP64 products.  Levels nest: the sub-problem itself starts from its
+ * first m/4 strings while m/4 >= 256.  m = 0 (default) switches it off.  Statistics of the
  * last solve's coarse stage: sub-problem products, energy, wall seconds. */
 int sbdx_set_coarse_start(sbdx_handle_t h, size_t m, double eps);
 int sbdx_coarse_stats(sbdx_handle_t h, int *products, double *energy, double *seconds);
+
+/* hipGraph replay (no counterpart in the reference): with on != 0 the FP32
+ * symmetric sigma core (same-spin GEMM, combine, mixed-term kernel on the
+ * handle's fixed FP32 buffers) is captured once, after one eager call, and
+ * replayed by later symmetric FP32 products (mixed-precision thick solves
+ * with sbdx_set_symmetry).  on = 0 drops the graph and runs eagerly. */
+int sbdx_set_graphs(sbdx_handle_t h, int on);
 
 /* verbose != 0: the solves print one line per step to stderr, step index,
  * E0 and |r| (as davidson.h:839-851 prints per step); thick-restart solves

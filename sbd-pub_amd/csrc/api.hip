@@ -59,7 +59,7 @@ void check_launch(const char *what) {
 }
 
 ProfScope::ProfScope(Context &cx, const char *nm) : c(cx), name(nm) {
-  if (!c.prof) return;
+  if (!c.prof || c.capturing) return;
   SBDX_HIP(hipEventCreate(&b));
   SBDX_HIP(hipEventCreate(&e));
   SBDX_HIP(hipEventRecord(b, c.stream));
@@ -237,6 +237,7 @@ static void destroy(Context *c) {
   if (!c) return;
   (void)hipStreamSynchronize(c->stream);
   prof_clear(*c);
+  graphs_release(*c);
   same_dense_release(*c);
   if (c->hpin) (void)hipHostFree(c->hpin);
   dfree_all(*c);
@@ -578,6 +579,15 @@ int sbdx_coarse_stats(sbdx_handle_t h, int *products, double *energy, double *se
   if (products) *products = c.coarse_products;
   if (energy) *energy = c.coarse_energy;
   if (seconds) *seconds = c.coarse_seconds;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_set_graphs(sbdx_handle_t h, int on) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!on) graphs_release(c);
+  c.use_graphs = on ? 1 : 0;
   return SBDX_OK;
   SBDX_GUARD_END
 }

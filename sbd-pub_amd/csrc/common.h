@@ -136,6 +136,11 @@ struct Context {
   int precision = 0;       // 0: FP64 throughout; 1: FP32 sigma on corrections, FP64 restarts
   int verbose = 0;         // 1: one stderr line per Davidson step (E0, |r|)
   int symmetric = 0;       // 1: thick-restart solves use the spin-flip symmetric sigma
+  // hipGraph replay of the FP32 symmetric sigma core (fixed FP32 staging
+  // buffers): captured once after an eager warm-up when use_graphs is set
+  int use_graphs = 0;
+  bool capturing = false;  // no profiling events while a stream capture runs
+  hipGraphExec_t g_sym32 = nullptr;
   // Coarse start of the thick-restart solves: the problem on the first
   // coarse_m strings of each set solved to coarse_eps first (0: off), and
   // what it cost in the last solve.
@@ -237,6 +242,7 @@ double sym_defect(Context &c, const double *v);  // max|v - v^T| / max|v|
 void symmetrize(Context &c, double *v);         // v = (v + v^T)/2
 void sigma_block_f32(Context &c, int k, const double *x, double *y);
 void same_dense_release(Context &c);  // frees the rocBLAS handle
+void graphs_release(Context &c);      // frees captured graphs
 // eigenpairs of a symmetric n x n column-major A (overwritten), ascending:
 // E[n], eigenvectors in the columns of Q (n x n)
 void host_syev(int n, double *A, double *Q, double *E);

@@ -596,6 +596,9 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
     } owner{sub};
     sub->precision = c.precision;
     sub->restart_keep = c.restart_keep;
+    // nested levels: the sub-problem starts from a quarter of its strings
+    sub->coarse_m = mc / 4 >= 256 ? mc / 4 : 0;
+    sub->coarse_eps = c.coarse_eps;
     sub->symmetric = (sym && sigma_sym_available(*sub)) ? 1 : 0;
     // copies between the two contexts run on the consumer's stream after the
     // producer's stream has drained (both are non-blocking streams)

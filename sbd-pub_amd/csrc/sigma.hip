@@ -1145,6 +1145,12 @@ static void sigma_sym_impl(Context &c, const T *x, T *y) {
   const SigmaOperands<T> o = operands(c, (T *)nullptr);
   const bool f32 = sizeof(T) == 4;
   const int n = (int)c.na;
+  if (sizeof(T) * ((size_t)c.nb + c.K + 1) > c.lds_limit) {
+    // one staged pair does not fit in LDS (e.g. FP64 at 20000 strings): the
+    // general form (row parts) gives the same product for symmetric x
+    sigma_impl<T>(c, x, y);
+    return;
+  }
   {
     ProfScope ps(c, f32 ? "sym_same_f32" : "sym_same");
     T *Ha, *Hb;
@@ -1173,12 +1179,50 @@ bool sigma_sym_available(Context &c) { return sym_ok(c); }
 
 void sigma_sym(Context &c, const double *x, double *y) { sigma_sym_impl<double>(c, x, y); }
 
+// The FP32 symmetric core (same-spin GEMM + combine + mixed kernel on the
+// fixed FP32 staging buffers) as a replayed hipGraph: the first call runs
+// eagerly (autotuning, dense blocks and rocBLAS workspace are settled then)
+// and captures the same sequence for later calls.
+static void sym_core_f32(Context &c) {
+  if (!c.use_graphs || sizeof(float) * ((size_t)c.nb + c.K + 1) > c.lds_limit) {
+    sigma_sym_impl<float>(c, c.f32.x, c.f32.y);
+    return;
+  }
+  if (c.g_sym32) {
+    ProfScope ps(c, "sym_graph_f32");
+    SBDX_HIP(hipGraphLaunch(c.g_sym32, c.stream));
+    return;
+  }
+  sigma_sym_impl<float>(c, c.f32.x, c.f32.y);  // eager; also this call's result
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  hipGraph_t graph = nullptr;
+  c.capturing = true;
+  SBDX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+  try {
+    sigma_sym_impl<float>(c, c.f32.x, c.f32.y);
+  } catch (...) {
+    (void)hipStreamEndCapture(c.stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    c.capturing = false;
+    throw;
+  }
+  c.capturing = false;
+  SBDX_HIP(hipStreamEndCapture(c.stream, &graph));
+  SBDX_HIP(hipGraphInstantiate(&c.g_sym32, graph, nullptr, nullptr, 0));
+  SBDX_HIP(hipGraphDestroy(graph));
+}
+
+void graphs_release(Context &c) {
+  if (c.g_sym32) (void)hipGraphExecDestroy(c.g_sym32);
+  c.g_sym32 = nullptr;
+}
+
 void sigma_sym_f32(Context &c, const double *x, double *y) {
   ensure_f32(c);
   const unsigned g = (unsigned)std::min<size_t>(cdiv(c.N, 256), 65536);
   hipLaunchKernelGGL((k_cast<double, float>), dim3(g), dim3(256), 0, c.stream, c.N, x, c.f32.x);
   check_launch("k_cast");
-  sigma_sym_impl<float>(c, c.f32.x, c.f32.y);
+  sym_core_f32(c);
   hipLaunchKernelGGL((k_cast<float, double>), dim3(g), dim3(256), 0, c.stream, c.N, c.f32.y, y);
   check_launch("k_cast");
 }

@@ -46,6 +46,7 @@ ABI_SYMBOLS = [
     "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
     "sbdx_mixed_staging", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry",
     "sbdx_sigma_sym", "sbdx_mlp_apply", "sbdx_set_coarse_start", "sbdx_coarse_stats",
+    "sbdx_set_graphs",
 ]
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -95,6 +96,7 @@ def lib():
     L.sbdx_sigma_block.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
     L.sbdx_symmetric_eig.argtypes = [C.c_int, _dp, _dp, _dp]
     L.sbdx_set_symmetry.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_set_coarse_start.argtypes = [C.c_void_p, C.c_size_t, C.c_double]
     L.sbdx_coarse_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]
@@ -280,6 +282,10 @@ class TPBOperator:
         n, e, t = C.c_int(), C.c_double(), C.c_double()
         _check(lib().sbdx_coarse_stats(self.h, C.byref(n), C.byref(e), C.byref(t)))
         return dict(products=n.value, energy=e.value, seconds=round(t.value, 4))
+
+    def set_graphs(self, on):
+        """Replay the FP32 symmetric sigma core as a captured hipGraph."""
+        _check(lib().sbdx_set_graphs(self.h, 1 if on else 0))
 
     def set_verbose(self, level):
         """Nonzero: one stderr line per Davidson step (E0, |r|)."""
