@@ -222,31 +222,38 @@ void build_spin_tables(Context &c, SpinTables &t, const std::vector<uint64_t> &s
 // out[64*k + l]).  The mixed kernels read x[.., ket] and V[.., pq] from LDS
 // at the indices of entry k of all 64 lanes at once, and the row sums do
 // not depend on the entry order, so entries are dealt greedily: at each k,
-// lane by lane, the remaining entry whose ket and pq residues mod 64 are
-// least used so far at this k.  Distinct residues mod 64 across the
-// wavefront spread the reads evenly over the LDS banks for 32 or 64 banks
-// and 4- or 8-byte elements.  Padding entries (pq = pad, a zero in every
-// staged ERI row, one shared address) take an unused ket residue below nb.
+// lane by lane, the remaining entry whose ket and pq residues mod 32 are
+// least used so far at this k within the lane's half-wavefront.  gfx950
+// serves an LDS read 32 lanes at a time, and two distinct addresses of one
+// half-wavefront conflict when their element indices agree mod 32 (measured
+// for 4-byte and 8-byte reads; lanes l and l + 32 never conflict), so the
+// residues are balanced per half.  Padding entries (pq = pad, a zero in
+// every staged ERI row, one shared address) take an unused ket residue
+// below nb.
 static void schedule_slice(std::vector<uint32_t> *rows, uint32_t len, uint32_t pad_pq, uint32_t nb,
                            uint32_t *out) {
   for (uint32_t k = 0; k < len; k++) {
-    uint8_t cj[64] = {0}, cv[64] = {0};
+    uint8_t cj[32], cv[32];
     for (int l = 0; l < 64; l++) {
+      if ((l & 31) == 0) {
+        std::fill(cj, cj + 32, 0);
+        std::fill(cv, cv + 32, 0);
+      }
       std::vector<uint32_t> &R = rows[l];
       uint32_t w;
       if (R.empty()) {
         int best = 0;
-        for (int j = 1; j < 64; j++)
+        for (int j = 1; j < 32; j++)
           if (cj[j] < cj[best]) best = j;
-        const uint32_t jb = nb > 64 ? (uint32_t)best : (uint32_t)best % nb;
+        const uint32_t jb = nb > 32 ? (uint32_t)best : (uint32_t)best % nb;
         w = jb | (pad_pq << 16);
-        cj[jb & 63]++;
+        cj[jb & 31]++;
       } else {
         size_t bi = 0;
         int bc = 1 << 30;
         for (size_t i = 0; i < R.size(); i++) {
           const uint32_t e = R[i];
-          const int cost = cj[e & 63] + cv[(e >> 16) & 63];
+          const int cost = cj[e & 31] + cv[(e >> 16) & 31];
           if (cost < bc) {
             bc = cost;
             bi = i;
@@ -256,8 +263,8 @@ static void schedule_slice(std::vector<uint32_t> *rows, uint32_t len, uint32_t p
         w = R[bi];
         R[bi] = R.back();
         R.pop_back();
-        cj[w & 63]++;
-        cv[(w >> 16) & 63]++;
+        cj[w & 31]++;
+        cv[(w >> 16) & 31]++;
       }
       out[(size_t)64 * k + l] = w;
     }
