@@ -122,19 +122,37 @@ __global__ void __launch_bounds__(256) k_transpose(const T *__restrict__ in, T *
   }
 }
 
+// EB values of one element type read as one vector (one LDS instruction of
+// 4, 8 or 16 bytes); EB == 1 is the scalar.
+template <typename T, int EB> struct Pack { typedef T V __attribute__((ext_vector_type(EB))); };
+template <typename T> struct Pack<T, 1> { typedef T V; };
+template <typename T, int EB> __device__ __forceinline__ T &lane_of(typename Pack<T, EB>::V &v, int q) {
+  return reinterpret_cast<T *>(&v)[q];
+}
+template <typename T, int EB> __device__ __forceinline__ T hsum(const typename Pack<T, EB>::V &v) {
+  T s = reinterpret_cast<const T *>(&v)[0];
+#pragma unroll
+  for (int q = 1; q < EB; q++) s += reinterpret_cast<const T *>(&v)[q];
+  return s;
+}
+
 // Mixed alpha-beta term, beta singles in sliced-ELL form (SpinTables::ell).
 // One block per alpha bra row I.  Each LDS round stages EB "pairs": for an
 // alpha single of I its ket row x[J,:] and sign-folded ERI row s_a*(pq|..);
 // after the singles, one spectator pair x[I,:] with the row
-// sum_{k in alpha(I)} (kk|..) (spectator_val).  Index K of each ERI row is a
-// zero for padding entries.  Thread slot = tid + t*1024 walks beta row
-// perm[slot]; slices are 64 rows of similar length, so the trip count is
-// uniform per wavefront and each entry load is a coalesced 256-byte access
-// shared by the EB staged pairs.  Per pair the slot also adds the alpha
-// single's dressing x[J,r] * sum_{k in beta(r)} s_a (pq|kk) from the staged
-// ERI row's diagonal (zero in the spectator row).  A row's slots may be
-// split over gridDim.y blocks of 1024*NT slots each (fewer registers per
-// thread, more resident waves, the row staged once per block).
+// sum_{k in alpha(I)} (kk|..) (spectator_val).  The pairs are interleaved,
+// X[j][q] = x_q[j] and W[i][q] = v_q[i], so a sliced-ELL entry (ket j, index
+// i) costs two LDS reads of EB values each (4 to 16 bytes: gfx950 serves a
+// 16-byte read per half-wavefront at about the cycles of a 4-byte one) for
+// EB multiply-adds.  Index K of W is a zero for padding entries.  Thread
+// slot = tid + t*1024 walks beta row perm[slot]; slices are 64 rows of
+// similar length, so the trip count is uniform per wavefront and each entry
+// load is a coalesced 256-byte access shared by the EB staged pairs.  Per
+// pair the slot also adds the alpha single's dressing x[J,r] *
+// sum_{k in beta(r)} s_a (pq|kk) from the staged ERI row's diagonal (zero in
+// the spectator row).  A row's slots may be split over gridDim.y blocks of
+// 1024*NT slots each (fewer registers per thread, more resident waves, the
+// row staged once per block).
 // SYM (equal string sets, symmetric x): block p handles bra I = perm[p] and
 // only the slots s >= p of the length-sorted table, adding each value at
 // y[I,r] and, for s > p, at y[r,I] (r = perm[s]): every unordered pair
@@ -154,21 +172,22 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
                                                     const uint32_t *__restrict__ sl_len,
                                                     const T *__restrict__ eri,
                                                     const T *__restrict__ x, T *__restrict__ y) {
+  typedef typename Pack<T, EB>::V V;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  T *lds = reinterpret_cast<T *>(lds_raw);
-  const int ldr = nb + K + 1;
+  V *X = reinterpret_cast<V *>(lds_raw);  // nb ket values per pair
+  V *W = X + nb;                          // K + 1 ERI values per pair
   const int L1 = L + 1;
   const int pos = blockIdx.x;
   const int I = SYM ? (int)perm[pos] : blockIdx.x;
   const int base = (SYM ? (pos & ~63) : 0) + blockIdx.y * (1024 * NT);
   if (SYM && base >= nslots) return;  // uniform per block, before any barrier
   const int tid = base + threadIdx.x;
-  T acc[NT];
+  V acc[NT];
   uint64_t occ[NT];
   int rr[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) {
-    acc[t] = T(0);
+    acc[t] = V(T(0));
     const int slot = tid + t * 1024;
     const uint32_t r = slot < nslots ? perm[slot] : 0xffffffffu;
     rr[t] = r != 0xffffffffu ? (int)r : 0;
@@ -177,27 +196,47 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
   const uint64_t occ_a = a_str[I];
   const uint64_t e0 = a_off[I], e1 = a_off[I + 1] + 1;  // + the spectator pair
   for (uint64_t e = e0; e < e1; e += EB) {
-    const int ne = (int)((e1 - e) < (uint64_t)EB ? (e1 - e) : (uint64_t)EB);
-    __syncthreads();
+    // pair q of this round: kind 1 = alpha single, 2 = spectator, 0 = empty
+    int kind[EB];
+    const T *xsrc[EB];
+    const T *vsrc[EB];
+    T sgn[EB];
 #pragma unroll
-    for (int k = 0; k < EB; k++) {
-      T *crow = lds + k * ldr;
-      T *vrow = crow + nb;
-      if (k < ne && e + k + 1 < e1) {
-        const uint64_t ent = a_ent[e + k];
-        const size_t J = ent_ket(ent);
-        const T sa = ent_neg(ent) ? T(-1) : T(1);
-        const T *vsrc = eri + (size_t)ent_pq(ent) * K;
-        for (int i = threadIdx.x; i < nb; i += 1024) crow[i] = x[J * nb + i];
-        for (int i = threadIdx.x; i < K; i += 1024) vrow[i] = sa * vsrc[i];
-        if (threadIdx.x == 0) vrow[K] = T(0);
-      } else if (k < ne) {
-        for (int i = threadIdx.x; i < nb; i += 1024) crow[i] = x[(size_t)I * nb + i];
-        for (int i = threadIdx.x; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
-        if (threadIdx.x == 0) vrow[K] = T(0);
-      } else {
-        for (int i = threadIdx.x; i < ldr; i += 1024) crow[i] = T(0);
+    for (int q = 0; q < EB; q++) {
+      kind[q] = 0;
+      xsrc[q] = x;
+      vsrc[q] = eri;
+      sgn[q] = T(1);
+      if (e + q + 1 < e1) {
+        const uint64_t ent = a_ent[e + q];
+        kind[q] = 1;
+        xsrc[q] = x + (size_t)ent_ket(ent) * nb;
+        vsrc[q] = eri + (size_t)ent_pq(ent) * K;
+        sgn[q] = ent_neg(ent) ? T(-1) : T(1);
+      } else if (e + q + 1 == e1) {
+        kind[q] = 2;
+        xsrc[q] = x + (size_t)I * nb;
       }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += 1024) {
+      V v;
+#pragma unroll
+      for (int q = 0; q < EB; q++) lane_of<T, EB>(v, q) = kind[q] ? xsrc[q][i] : T(0);
+      X[i] = v;
+    }
+    for (int i = threadIdx.x; i <= K; i += 1024) {
+      V v;
+#pragma unroll
+      for (int q = 0; q < EB; q++) {
+        T w = T(0);
+        if (i < K) {
+          if (kind[q] == 1) w = sgn[q] * vsrc[q][i];
+          else if (kind[q] == 2) w = spectator_val(eri, occ_a, L1, K, i);
+        }
+        lane_of<T, EB>(v, q) = w;
+      }
+      W[i] = v;
     }
     __syncthreads();
 #pragma unroll
@@ -207,20 +246,14 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
         const int sl = slot >> 6;
         const uint32_t len = sl_len[sl];
         const uint32_t *p = ell + sl_off[sl] + (slot & 63);
-        T a = T(0);
+        V a = V(T(0));
 #pragma unroll 4
         for (uint32_t k = 0; k < len; k++) {
           const uint32_t w = p[64 * (size_t)k];
-          const int jb = (int)(w & 0xffffu);
-          const int rs = (int)((w >> 16) & 0x7fffu);
-          T v = lds[nb + rs] * lds[jb];
-#pragma unroll
-          for (int q = 1; q < EB; q++) v += lds[q * ldr + nb + rs] * lds[q * ldr + jb];
+          const V v = W[(w >> 16) & 0x7fffu] * X[w & 0xffffu];
           a += (w >> 31) ? -v : v;
         }
-#pragma unroll
-        for (int q = 0; q < EB; q++)
-          a += lds[q * ldr + rr[t]] * diag_sum(lds + q * ldr + nb, occ[t], L1);
+        a += X[rr[t]] * diag_sum(W, occ[t], L1);
         acc[t] += a;
       }
     }
@@ -229,15 +262,15 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
   for (int t = 0; t < NT; t++) {
     const int slot = tid + t * 1024;
     if (slot < nslots) {
+      const T v = hsum<T, EB>(acc[t]);
+      const uint32_t r = perm[slot];
       if (SYM) {
-        const uint32_t r = perm[slot];
         if (r != 0xffffffffu && slot >= pos) {
-          y[(size_t)I * nb + r] += acc[t];
-          if (slot > pos) y[(size_t)r * nb + I] += acc[t];
+          y[(size_t)I * nb + r] += v;
+          if (slot > pos) y[(size_t)r * nb + I] += v;
         }
       } else {
-        const uint32_t r = perm[slot];
-        if (r != 0xffffffffu) y[(size_t)I * nb + r] += acc[t];
+        if (r != 0xffffffffu) y[(size_t)I * nb + r] += v;
       }
     }
   }

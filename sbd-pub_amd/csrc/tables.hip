@@ -222,38 +222,48 @@ void build_spin_tables(Context &c, SpinTables &t, const std::vector<uint64_t> &s
 // out[64*k + l]).  The mixed kernels read x[.., ket] and V[.., pq] from LDS
 // at the indices of entry k of all 64 lanes at once, and the row sums do
 // not depend on the entry order, so entries are dealt greedily: at each k,
-// lane by lane, the remaining entry whose ket and pq residues mod 32 are
-// least used so far at this k within the lane's half-wavefront.  gfx950
-// serves an LDS read 32 lanes at a time, and two distinct addresses of one
-// half-wavefront conflict when their element indices agree mod 32 (measured
-// for 4-byte and 8-byte reads; lanes l and l + 32 never conflict), so the
-// residues are balanced per half.  Padding entries (pq = pad, a zero in
-// every staged ERI row, one shared address) take an unused ket residue
-// below nb.
+// lane by lane, the remaining entry whose ket and pq residues are least
+// used so far at this k.  Measured gfx950 LDS bank rule: 4- and 8-byte
+// reads are served 32 lanes at a time and two distinct addresses of one
+// half-wavefront conflict when their element indices agree mod 32; 16-byte
+// reads (4 FP32 or 2 FP64 pairs per entry) are served 16 lanes at a time and
+// conflict when the indices agree mod 16 within the quarter-wavefront.  The
+// cost of a candidate is 2 x (uses of its residues mod 16 in the lane's
+// quarter) + (uses of its residues mod 32 in the lane's half), which keeps
+// both rules nearly balanced.  Padding entries (pq = pad, a zero in every
+// staged ERI row, one shared address) take an unused ket residue below nb.
 static void schedule_slice(std::vector<uint32_t> *rows, uint32_t len, uint32_t pad_pq, uint32_t nb,
                            uint32_t *out) {
   for (uint32_t k = 0; k < len; k++) {
-    uint8_t cj[32], cv[32];
+    uint8_t hj[32], hv[32], qj[16], qv[16];
     for (int l = 0; l < 64; l++) {
       if ((l & 31) == 0) {
-        std::fill(cj, cj + 32, 0);
-        std::fill(cv, cv + 32, 0);
+        std::fill(hj, hj + 32, 0);
+        std::fill(hv, hv + 32, 0);
+      }
+      if ((l & 15) == 0) {
+        std::fill(qj, qj + 16, 0);
+        std::fill(qv, qv + 16, 0);
       }
       std::vector<uint32_t> &R = rows[l];
       uint32_t w;
       if (R.empty()) {
-        int best = 0;
-        for (int j = 1; j < 32; j++)
-          if (cj[j] < cj[best]) best = j;
+        int best = 0, bc = 1 << 30;
+        for (int j = 0; j < 32; j++) {
+          const int cost = 2 * qj[j & 15] + hj[j];
+          if (cost < bc) {
+            bc = cost;
+            best = j;
+          }
+        }
         const uint32_t jb = nb > 32 ? (uint32_t)best : (uint32_t)best % nb;
         w = jb | (pad_pq << 16);
-        cj[jb & 31]++;
       } else {
         size_t bi = 0;
         int bc = 1 << 30;
         for (size_t i = 0; i < R.size(); i++) {
-          const uint32_t e = R[i];
-          const int cost = cj[e & 31] + cv[(e >> 16) & 31];
+          const uint32_t e = R[i], v = e >> 16;
+          const int cost = 2 * (qj[e & 15] + qv[v & 15]) + hj[e & 31] + hv[v & 31];
           if (cost < bc) {
             bc = cost;
             bi = i;
@@ -263,9 +273,11 @@ static void schedule_slice(std::vector<uint32_t> *rows, uint32_t len, uint32_t p
         w = R[bi];
         R[bi] = R.back();
         R.pop_back();
-        cj[w & 31]++;
-        cv[(w >> 16) & 31]++;
+        qv[(w >> 16) & 15]++;
+        hv[(w >> 16) & 31]++;
       }
+      qj[w & 15]++;
+      hj[w & 31]++;
       out[(size_t)64 * k + l] = w;
     }
   }
