@@ -169,10 +169,17 @@ def main():
         W0 /= np.linalg.norm(W0)
     # warmup: also settles the same-spin form (dense GEMM or sparse rows)
     # for each precision before any timed solve
+    # (and the symmetric form's mixed-kernel configuration, on the HF vector,
+    # which is spin-flip symmetric)
     for _ in range(args.warmup):
         op.sigma(W0)
         if args.solver == "mixed":
             op.sigma_fp32(W0)
+        if args.symmetry and same_sets and args.solver in ("thick", "mixed"):
+            e0 = sbdx.basis_init_vector(op.ndet)
+            op.sigma_sym(e0)
+            if args.solver == "mixed":
+                op.sigma_sym(e0, fp32=True)
     log(f"warmup done; same-spin form {op.same_spin_forms()}")
 
     if args.precond == "mlp":

@@ -500,11 +500,11 @@ int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, int *slots_pe
                        float *ms, int *multi) {
   SBDX_GUARD_BEGIN
   Context &c = ctx_of(h);
-  if (precision != 0 && precision != 1) throw Error(SBDX_ERR_ARG, "precision must be 0 or 1");
+  if (precision < 0 || precision > 3) throw Error(SBDX_ERR_ARG, "precision must be 0, 1, 2 or 3");
   if (width) *width = c.mixed_eb[precision];
   if (slots_per_thread) *slots_per_thread = c.mixed_nt[precision];
   if (ms) *ms = c.mixed_best_ms[precision];
-  if (multi) *multi = c.multi_mode[precision];
+  if (multi) *multi = precision < 2 ? c.multi_mode[precision] : -1;
   return SBDX_OK;
   SBDX_GUARD_END
 }

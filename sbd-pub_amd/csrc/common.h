@@ -124,12 +124,13 @@ struct Context {
   // 0 sparse row kernels, 1 dense GEMM; the timings that decided it (ms).
   int same_mode[2] = {-1, -1};
   float same_ms[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
-  // Sliced-ELL mixed kernel configuration per precision: staging width
-  // (pairs per LDS round; 0 not tuned, -1 none fits, else 1, 2 or 4), slots
-  // per thread, and the tuned time (ms).
-  int mixed_eb[2] = {0, 0};
-  int mixed_nt[2] = {0, 0};
-  float mixed_best_ms[2] = {0.f, 0.f};
+  // Sliced-ELL mixed kernel configuration per form (0 FP64, 1 FP32 general;
+  // 2 FP64, 3 FP32 spin-flip symmetric): staging width (pairs per LDS round;
+  // 0 not tuned, -1 none fits, else 1, 2 or 4), slots per thread, and the
+  // tuned time (ms).
+  int mixed_eb[4] = {0, 0, 0, 0};
+  int mixed_nt[4] = {0, 0, 0, 0};
+  float mixed_best_ms[4] = {0.f, 0.f, 0.f, 0.f};
   // block products: 1 when the multi-vector kernel beat per-vector launches
   // on first use, 0 when not, -1 not decided
   int multi_mode[2] = {-1, -1};

@@ -129,6 +129,20 @@ template <typename T> struct Pack<T, 1> { typedef T V; };
 template <typename T, int EB> __device__ __forceinline__ T &lane_of(typename Pack<T, EB>::V &v, int q) {
   return reinterpret_cast<T *>(&v)[q];
 }
+// v with the sign bit sb (0 or 0x80000000) flipped in each of its EB values
+// (high word of a double): one XOR per value, so the product folds into an FMA.
+__device__ __forceinline__ float flip_sign(float v, uint32_t sb) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) ^ sb);
+}
+__device__ __forceinline__ double flip_sign(double v, uint32_t sb) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v) ^ ((uint64_t)sb << 32));
+}
+template <typename T, int EB>
+__device__ __forceinline__ typename Pack<T, EB>::V flip_sign(typename Pack<T, EB>::V v, uint32_t sb) {
+#pragma unroll
+  for (int q = 0; q < EB; q++) lane_of<T, EB>(v, q) = flip_sign(lane_of<T, EB>(v, q), sb);
+  return v;
+}
 template <typename T, int EB> __device__ __forceinline__ T hsum(const typename Pack<T, EB>::V &v) {
   T s = reinterpret_cast<const T *>(&v)[0];
 #pragma unroll
@@ -250,8 +264,7 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
 #pragma unroll 4
         for (uint32_t k = 0; k < len; k++) {
           const uint32_t w = p[64 * (size_t)k];
-          const V v = W[(w >> 16) & 0x7fffu] * X[w & 0xffffu];
-          a += (w >> 31) ? -v : v;
+          a += W[(w >> 16) & 0x7fffu] * flip_sign<T, EB>(X[w & 0xffffu], w & 0x80000000u);
         }
         a += X[rr[t]] * diag_sum(W, occ[t], L1);
         acc[t] += a;
@@ -783,15 +796,17 @@ static bool mixed_parts_dispatch(Context &c, const T *eri, const T *x, T *y) {
   return false;
 }
 
+template <bool SYM = false>
 static bool mixed_ell_cfg(Context &c, int eb, int nt, const double *eri, const double *x, double *y) {
-  return eb == 4 ? mixed_ell_dispatch<double, 4>(c, nt, eri, x, y)
-       : eb == 2 ? mixed_ell_dispatch<double, 2>(c, nt, eri, x, y)
-                 : mixed_ell_dispatch<double, 1>(c, nt, eri, x, y);
+  return eb == 4 ? mixed_ell_dispatch<double, 4, SYM>(c, nt, eri, x, y)
+       : eb == 2 ? mixed_ell_dispatch<double, 2, SYM>(c, nt, eri, x, y)
+                 : mixed_ell_dispatch<double, 1, SYM>(c, nt, eri, x, y);
 }
+template <bool SYM = false>
 static bool mixed_ell_cfg(Context &c, int eb, int nt, const float *eri, const float *x, float *y) {
-  return eb == 4 ? mixed_ell_dispatch<float, 4>(c, nt, eri, x, y)
-       : eb == 2 ? mixed_ell_dispatch<float, 2>(c, nt, eri, x, y)
-                 : mixed_ell_dispatch<float, 1>(c, nt, eri, x, y);
+  return eb == 4 ? mixed_ell_dispatch<float, 4, SYM>(c, nt, eri, x, y)
+       : eb == 2 ? mixed_ell_dispatch<float, 2, SYM>(c, nt, eri, x, y)
+                 : mixed_ell_dispatch<float, 1, SYM>(c, nt, eri, x, y);
 }
 static double *scratch_vec(Context &c, const double *) { return c.T2; }
 static float *scratch_vec(Context &c, const float *) { return c.f32.T2; }
@@ -803,9 +818,9 @@ static float *scratch_vec(Context &c, const float *) { return c.f32.T2; }
 // accumulates, so y is untouched); HIP events on the stream time the second
 // run and the fastest is kept for this precision (-1: nothing fits).
 // SBDX_MIXED_EB=1|2|4 and SBDX_MIXED_NT=1|2|4|8 fix either coordinate.
-template <typename T>
+template <typename T, bool SYM = false>
 static int mixed_cfg(Context &c, const T *eri, const T *x) {
-  const int pi = sizeof(T) == 4 ? 1 : 0;
+  const int pi = (sizeof(T) == 4 ? 1 : 0) + (SYM ? 2 : 0);
   if (c.mixed_eb[pi] != 0) return c.mixed_eb[pi];
   const int nt_max = (int)cdiv((size_t)c.B.nslice * 64, 1024);
   const char *env_eb = std::getenv("SBDX_MIXED_EB");
@@ -829,7 +844,7 @@ static int mixed_cfg(Context &c, const T *eri, const T *x) {
       float ms = 0.f;
       for (int rep = 0; rep < 2 && ok; rep++) {
         SBDX_HIP(hipEventRecord(e0, c.stream));
-        ok = mixed_ell_cfg(c, eb, nt, eri, x, scratch);
+        ok = mixed_ell_cfg<SYM>(c, eb, nt, eri, x, scratch);
         SBDX_HIP(hipEventRecord(e1, c.stream));
         SBDX_HIP(hipEventSynchronize(e1));
         SBDX_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -1196,8 +1211,8 @@ static void sigma_sym_impl(Context &c, const T *x, T *y) {
   }
   {
     ProfScope ps(c, f32 ? "sym_mixed_f32" : "sym_mixed");
-    const int pi = f32 ? 1 : 0;
-    const int eb = mixed_cfg<T>(c, o.eri, x);
+    const int pi = (f32 ? 1 : 0) + 2;
+    const int eb = mixed_cfg<T, true>(c, o.eri, x);
     const int nt = c.mixed_nt[pi];
     bool ok = false;
     if (eb == 4) ok = mixed_ell_dispatch<T, 4, true>(c, nt, o.eri, x, y);

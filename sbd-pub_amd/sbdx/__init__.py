@@ -382,7 +382,9 @@ class TPBOperator:
 
     def same_spin_forms(self):
         """Per precision: the same-spin form in use ('dense' GEMM, 'sparse'
-        rows, None before the first sigma) and the autotuner's timings (ms)."""
+        rows, None before the first sigma) and the autotuner's timings (ms),
+        and the mixed kernel's tuned staging width and slots per thread for
+        the general and the spin-flip symmetric form (0 before first use)."""
         out = {}
         for prec, key in ((0, "fp64"), (1, "fp32")):
             f, a, b = C.c_int(), C.c_float(), C.c_float()
@@ -394,6 +396,10 @@ class TPBOperator:
                                             C.byref(mu)))
             out[key].update(mixed_width=w.value, mixed_slots_per_thread=nt.value,
                             mixed_ms=round(ms.value, 3), block_multi_kernel=mu.value)
+            _check(lib().sbdx_mixed_staging(self.h, prec + 2, C.byref(w), C.byref(nt), C.byref(ms),
+                                            C.byref(mu)))
+            out[key].update(sym_mixed_width=w.value, sym_mixed_slots_per_thread=nt.value,
+                            sym_mixed_ms=round(ms.value, 3))
         return out
 
     def profile(self, enable=True):
