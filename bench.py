@@ -77,6 +77,24 @@ def profiled_traffic(name, width, slots):
     return None
 
 
+def profiled_limiter(name, width, slots):
+    """Measured on-chip utilisation of kernel group `name` at the given
+    mixed-kernel configuration (LDS busy fraction, share of it stalled on
+    bank conflicts, VALU busy fraction), from the rocprofv3 SQ-counter
+    summary in profiles/ (None when not profiled)."""
+    path = os.path.join(ROOT, "profiles", "c36_sq_counters.md")
+    if not os.path.exists(path):
+        return None
+    want = f"eb={width} nt={slots}"
+    for line in open(path):
+        f = line.split()
+        if len(f) == 7 and f[0] == "limiter" and f[1] == name and f"{f[2]} {f[3]}" == want:
+            out = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in f[4:]}
+            out["source"] = "profiles/c36_sq_counters.md (rocprofv3 SQ counters)"
+            return out
+    return None
+
+
 def cpu_baseline(inst, n_steps, seconds, nthreads):
     """Oracle sigma on a random sample of alpha rows, scaled to one full
     sigma and multiplied by the GPU solve's sigma count."""
@@ -291,14 +309,18 @@ def main():
                         frac=round(dk["TFLOPs"] / peak, 4), traffic=None)
     else:
         forms = op.same_spin_forms()["fp32" if dom.endswith("_f32") else "fp64"]
-        traffic = None
+        pre = "sym_" if dom.startswith("sym_") else ""
+        width, slots = forms[pre + "mixed_width"], forms[pre + "mixed_slots_per_thread"]
+        traffic = limiter = None
         if args.workload == "c36":
-            traffic = profiled_traffic(dom, forms["mixed_width"], forms["mixed_slots_per_thread"])
+            traffic = profiled_traffic(dom, width, slots)
+            limiter = profiled_limiter(dom, width, slots)
         roofline = dict(kernel=dom, bound="hbm", achieved=dk["GBps"], peak=HBM_PEAK_GBPS,
                         unit="GB/s", frac=round(dk["GBps"] / HBM_PEAK_GBPS, 4), traffic=traffic,
                         traffic_source="profiles/c36_hbm_traffic.md (rocprofv3 FETCH_SIZE x2.0 + "
                                        "WRITE_SIZE x1.0, calibrated on streaming kernels)"
-                        if traffic else None)
+                        if traffic else None,
+                        limiter=limiter)
     s64_ms, s64_gbps = sigma_rate(kernels, "")
     s32_ms, s32_gbps = sigma_rate(kernels, "_f32")
     if plain is not None:
