@@ -137,9 +137,10 @@ def main():
     ap.add_argument("--solver", choices=["plain", "thick", "mixed", "block"], default="mixed",
                     help="timed solver: plain = the reference algorithm in FP64; thick = FP64 "
                          "thick restart; mixed = thick restart with FP32 correction products; "
-                         "block = block Davidson (--block corrections per iteration, one root) "
+                         "block = block Davidson (--block corrections per iteration, --roots roots) "
                          "with FP32 correction products")
     ap.add_argument("--block", type=int, default=4)
+    ap.add_argument("--roots", type=int, default=1, help="roots of the block solver")
     ap.add_argument("--no-symmetry", dest="symmetry", action="store_false",
                     help="general sigma in the optimized solver (default: spin-flip symmetric "
                          "form when the string sets are equal)")
@@ -199,6 +200,12 @@ def main():
             if args.solver == "mixed":
                 op.sigma_sym(e0, fp32=True)
     log(f"warmup done; same-spin form {op.same_spin_forms()}")
+    # one FP64 sigma through the C ABI with host buffers in and out (PCIe
+    # transfers included; every solve keeps its vectors in HBM instead)
+    t0 = time.perf_counter()
+    op.sigma(W0)
+    sigma_host_ms = (time.perf_counter() - t0) * 1e3
+    log(f"sigma with host buffers (PCIe included): {sigma_host_ms:.1f} ms")
 
     if args.precond == "mlp":
         op.mlp_set_weights(np.load(args.weights))
@@ -214,13 +221,17 @@ def main():
         op.set_verbose(0 if args.quiet else 1)
         op.profile(True)
         times, energies, nsteps, conv = [], [], [], []
+        roots = None
         for k in range(nsolve):
             op.set_vector(W0)
             t0 = time.perf_counter()
             if solver == "block":
-                Eb, _, n, ok = op.davidson_block(1, args.block, W0=W0[None, :], max_it=args.max_it,
-                                                 eps=args.eps, max_time=args.max_time)
+                Eb, _, n, ok = op.davidson_block(args.roots, args.block,
+                                                 W0=W0[None, :] if args.roots == 1 else None,
+                                                 max_it=args.max_it, eps=args.eps, max_time=args.max_time)
                 E = float(Eb[0])
+                roots = [float(e) for e in Eb]
+                log(f"block roots: {roots}")
             else:
                 E, n, ok = op.solve(max_it=args.max_it, nb=args.nb, eps=args.eps,
                                     max_time=args.max_time)
@@ -236,7 +247,7 @@ def main():
         stats = op.profile_stats()
         op.profile(False)
         return dict(time=float(np.median(times)), E=energies[-1], steps=int(np.median(nsteps)),
-                    converged=all(conv), stats=stats, precond_steps=op.precond_stats(),
+                    converged=all(conv), stats=stats, precond_steps=op.precond_stats(), roots=roots,
                     coarse=op.coarse_stats() if solver in ("thick", "mixed") else None)
 
     best = args.solver
@@ -365,7 +376,9 @@ def main():
         "speedup_vs_plain": None if plain is None else round(plain["time"] / T, 3),
         "hipgraph_solve": graph_res,
         "dE_vs_plain": None if plain is None else abs(res["E"] - plain["E"]),
+        "block_roots": res.get("roots"),
         "sigma_fp64_ms": s64_ms,
+        "sigma_fp64_host_buffers_ms": round(sigma_host_ms, 1),
         "sigma_GBps": s64_gbps,
         "sigma_fp32_ms": s32_ms,
         "sigma_fp32_GBps": s32_gbps,

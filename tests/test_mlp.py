@@ -147,3 +147,23 @@ class TestGPU:
         op.configure("plain")
         assert ok and abs(E - E_ref) < 1e-9, (E, E_ref)
         assert st["mlp"] > 0
+
+
+@pytest.mark.gpu
+def test_shipped_network_gain_on_held_out_instance():
+    """What the trained network (the shipped weights) gains over the diagonal
+    preconditioner on the held-out norb-22 instance (not among the training
+    instances): both solves reach the same energy; the products and times of
+    both are printed, and the network must not cost more than 10% extra
+    products (measured gain so far: about 1.03x fewer products)."""
+    import os
+    import sbdx
+    from sbdx import learned
+    w = np.load(os.path.join(os.path.dirname(sbdx.__file__), "data", "precond_mlp.npy"))
+    print("[mlp] held-out evaluation", learned.EVAL_INSTANCE, flush=True)
+    out = learned.evaluate(w, eps=1e-6, log=lambda s: print(s, flush=True))
+    print(f"[mlp] gain: products x{out['gain_products']:.3f}, time x{out['gain_time']:.3f}", flush=True)
+    assert out["diag"]["converged"] and out["mlp"]["converged"]
+    assert out["dE"] < 1e-8
+    assert out["mlp"]["steps_by_precond"]["mlp"] > 0
+    assert out["mlp"]["products"] <= 1.1 * out["diag"]["products"]
