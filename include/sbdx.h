@@ -169,6 +169,15 @@ int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_spa
 int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, int *slots_per_thread,
                        float *ms, int *multi);
 
+/* Form of the GEMM in the spin-flip symmetric FP32 same-spin term (no
+ * counterpart in the reference): *mode -1 before the first such product, 0
+ * FP32 SGEMM, 1 split-FP16x3 (x and H_alpha each written as 2^-e (hi + lo)
+ * in FP16 with a power-of-two scale, about 22 significant bits; three FP16
+ * MFMA GEMMs with FP32 accumulation, the lo x lo product dropped). Decided
+ * on the first product by timing both; *ms_sgemm, *ms_split those timings
+ * (0 when SBDX_SPLIT_GEMM=0|1 in the environment fixed the mode). */
+int sbdx_split_gemm(sbdx_handle_t h, int *mode, float *ms_sgemm, float *ms_split);
+
 /* The subspace eigensolver of the Davidson solves, on the host: eigenvalues
  * E[n] ascending and orthonormal eigenvectors (columns of V, n x n
  * column-major) of the symmetric n x n column-major A (not modified), by

@@ -496,6 +496,16 @@ int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_spa
   SBDX_GUARD_END
 }
 
+int sbdx_split_gemm(sbdx_handle_t h, int *mode, float *ms_sgemm, float *ms_split) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (mode) *mode = c.split_mode;
+  if (ms_sgemm) *ms_sgemm = c.split_ms[0];
+  if (ms_split) *ms_split = c.split_ms[1];
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
 int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, int *slots_per_thread,
                        float *ms, int *multi) {
   SBDX_GUARD_BEGIN

@@ -128,6 +128,14 @@ struct Context {
   // 2 FP64, 3 FP32 spin-flip symmetric): staging width (pairs per LDS round;
   // 0 not tuned, -1 none fits, else 1, 2 or 4), slots per thread, and the
   // tuned time (ms).
+  // split-FP16x3 GEMM of the symmetric FP32 same-spin term: mode (-1 not
+  // decided, 0 FP32 SGEMM, 1 split), the deciding timings (ms: SGEMM,
+  // split), FP16 hi/lo halves of H_alpha and of the current x, per-block
+  // maxima, and the two inverse power-of-two scales (x's, H's)
+  int split_mode = -1;
+  float split_ms[2] = {0.f, 0.f};
+  void *Hh = nullptr, *Hl = nullptr, *Xh = nullptr, *Xl = nullptr;
+  float *split_part = nullptr, *split_inv = nullptr;
   int mixed_eb[4] = {0, 0, 0, 0};
   int mixed_nt[4] = {0, 0, 0, 0};
   float mixed_best_ms[4] = {0.f, 0.f, 0.f, 0.f};

@@ -601,9 +601,13 @@ __global__ void __launch_bounds__(256) k_densify(int n, const uint64_t *__restri
 // Spin-flip symmetric form of the same-spin terms for equal string sets:
 // y = hii*x + (G + G^T) with G = H X (n x n row-major), 64x64 tiles, G^T
 // through LDS; for symmetric x the result is bitwise symmetric.
+// gscale (device, may be null): G holds 2^(ex+eh) times the product (the
+// split-FP16 GEMM), and gscale[0] * gscale[1] = 2^-(ex+eh) undoes it exactly.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sym_combine(int n, const T *__restrict__ hii, const T *__restrict__ x,
-                                                     const T *__restrict__ G, T *__restrict__ y) {
+                                                     const T *__restrict__ G, T *__restrict__ y,
+                                                     const float *__restrict__ gscale) {
+  const T s = gscale ? (T)(gscale[0] * gscale[1]) : T(1);
   __shared__ T tile[64][65];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
@@ -616,8 +620,58 @@ __global__ void __launch_bounds__(256) k_sym_combine(int n, const T *__restrict_
     const int r = r0 + k, cc = c0 + tx;
     if (r < n && cc < n) {
       const size_t o = (size_t)r * n + cc;
-      y[o] = hii[o] * x[o] + (G[o] + tile[tx][k]);  // G + G^T commutes: bitwise symmetric
+      y[o] = hii[o] * x[o] + (G[o] + tile[tx][k]) * s;  // G + G^T commutes: bitwise symmetric
     }
+  }
+}
+
+// Split-FP16 operands of the symmetric FP32 same-spin GEMM.  An FP32 array
+// v is written as 2^-e (hi + lo) with hi = fp16(v 2^e), lo = fp16(v 2^e - hi)
+// and 2^e putting max|v| 2^e in [2^14, 2^15): about 22 significant bits
+// down to an absolute floor of 2^-25 in scaled units.  Products of FP16
+// values are exact in FP32 and the FP16 MFMA GEMMs accumulate in FP32, so
+// X H = 2^-(ex+eh) (Xh Hh + Xh Hl + Xl Hh) up to the dropped Xl Hl
+// (~2^-22 relative) and FP32 summation.
+constexpr int kSplitParts = 1024;
+
+__device__ __forceinline__ float block_max256(float m) {
+  __shared__ float wmax[4];
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  return fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+}
+
+// part[b] = max |v[i]| over block b's grid-stride share (gridDim.x blocks).
+__global__ void __launch_bounds__(256) k_absmax_part(size_t n, const float *__restrict__ v,
+                                                     float *__restrict__ part) {
+  float m = 0.f;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    m = fmaxf(m, fabsf(v[i]));
+  m = block_max256(m);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// hi/lo halves of v with the scale taken from the np block maxima; block 0
+// stores 2^-e at *inv.  e is capped at 100 (vectors below 2^-85 in max
+// norm keep the absolute floor 2^-125).
+__global__ void __launch_bounds__(256) k_split_f16(size_t n, const float *__restrict__ v,
+                                                   const float *__restrict__ part, int np,
+                                                   _Float16 *__restrict__ hi, _Float16 *__restrict__ lo,
+                                                   float *__restrict__ inv) {
+  float m = 0.f;
+  for (int i = threadIdx.x; i < np; i += 256) m = fmaxf(m, part[i]);
+  m = block_max256(m);
+  int k = 0;
+  if (m > 0.f) (void)frexpf(m, &k);  // m = f 2^k, f in [0.5, 1)
+  const int e = m > 0.f ? std::min(15 - k, 100) : 0;
+  const float sc = ldexpf(1.f, e);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *inv = ldexpf(1.f, -e);
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const float t = v[i] * sc;
+    const _Float16 h = (_Float16)t;
+    hi[i] = h;
+    lo[i] = (_Float16)(t - (float)h);
   }
 }
 
@@ -1173,6 +1227,100 @@ static void sigma_impl(Context &c, const T *x, T *y) {
 
 void sigma_alpha_first(Context &c, const double *x, double *y) { sigma_impl<double>(c, x, y); }
 
+// C = C*beta + A B for n x n FP16 operands, FP32 accumulation and output.
+static rocblas_status gemm_f16(rocblas_handle h, int n, const void *A, const void *B, float *C, float beta) {
+  const float one = 1.0f;
+  return rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, A,
+                         rocblas_datatype_f16_r, n, B, rocblas_datatype_f16_r, n, &beta, C,
+                         rocblas_datatype_f32_r, n, C, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
+                         rocblas_gemm_algo_standard, 0, 0);
+}
+
+static void split_f16(Context &c, size_t n, const float *v, void *hi, void *lo, float *inv) {
+  const unsigned g = (unsigned)std::min<size_t>(cdiv(n, 256), 8192);
+  hipLaunchKernelGGL(k_absmax_part, dim3(kSplitParts), dim3(256), 0, c.stream, n, v, c.split_part);
+  check_launch("k_absmax_part");
+  hipLaunchKernelGGL(k_split_f16, dim3(g), dim3(256), 0, c.stream, n, v, (const float *)c.split_part,
+                     kSplitParts, (_Float16 *)hi, (_Float16 *)lo, inv);
+  check_launch("k_split_f16");
+}
+
+// Symmetric FP32 same-spin term with the split-FP16x3 GEMM:
+// y = hii*x + (G + G^T), G = X H_alpha from three FP16 MFMA GEMMs.
+static void sym_same_split(Context &c, const float *hii, const float *x, float *G, float *y) {
+  const int n = (int)c.na;
+  const size_t nn = (size_t)n * n;
+  if (!c.Hh) {
+    c.split_part = (float *)dalloc(c, sizeof(float) * kSplitParts);
+    c.split_inv = (float *)dalloc(c, sizeof(float) * 2);
+    c.Hh = dalloc(c, 2 * nn);
+    c.Hl = dalloc(c, 2 * nn);
+    c.Xh = dalloc(c, 2 * nn);
+    c.Xl = dalloc(c, 2 * nn);
+    float *Ha, *Hb;
+    dense_blocks(c, Ha, Hb);
+    split_f16(c, nn, Ha, c.Hh, c.Hl, c.split_inv + 1);
+  }
+  split_f16(c, nn, x, c.Xh, c.Xl, c.split_inv);
+  rocblas_handle hb = blas(c);
+  if (gemm_f16(hb, n, c.Xh, c.Hh, G, 0.f) != rocblas_status_success ||
+      gemm_f16(hb, n, c.Xh, c.Hl, G, 1.f) != rocblas_status_success ||
+      gemm_f16(hb, n, c.Xl, c.Hh, G, 1.f) != rocblas_status_success)
+    throw Error(3, "split-FP16 same-spin GEMM failed");
+  hipLaunchKernelGGL((k_sym_combine<float>), dim3(cdiv(n, 64), cdiv(n, 64)), dim3(256), 0, c.stream, n, hii, x,
+                     (const float *)G, y, (const float *)c.split_inv);
+  check_launch("k_sym_combine");
+}
+
+// The symmetric same-spin term with the GEMM in the element type itself.
+template <typename T>
+static void sym_same_gemm(Context &c, const SigmaOperands<T> &o, const T *x, T *y) {
+  const int n = (int)c.na;
+  T *Ha, *Hb;
+  dense_blocks(c, Ha, Hb);
+  if (gemm(blas(c), n, n, n, x, n, Ha, n, o.T1, n, T(0)) != rocblas_status_success)
+    throw Error(3, "symmetric same-spin GEMM failed");
+  hipLaunchKernelGGL((k_sym_combine<T>), dim3(cdiv(n, 64), cdiv(n, 64)), dim3(256), 0, c.stream, n, o.hii, x,
+                     (const T *)o.T1, y, (const float *)nullptr);
+  check_launch("k_sym_combine");
+}
+
+static void sym_same(Context &c, const SigmaOperands<double> &o, const double *x, double *y) {
+  sym_same_gemm<double>(c, o, x, y);
+}
+
+// FP32: the SGEMM form or the split-FP16x3 form, decided on the first call
+// by timing both after one untimed split run (which builds H's halves);
+// SBDX_SPLIT_GEMM=0|1 fixes the choice.
+static void sym_same(Context &c, const SigmaOperands<float> &o, const float *x, float *y) {
+  if (c.split_mode < 0) {
+    const char *env = std::getenv("SBDX_SPLIT_GEMM");
+    if (env) {
+      if (std::string(env) != "0" && std::string(env) != "1") throw Error(1, "SBDX_SPLIT_GEMM must be 0 or 1");
+      c.split_mode = std::atoi(env);
+    } else {
+      sym_same_split(c, o.hii, x, o.T1, y);
+      hipEvent_t e[3];
+      for (auto &ev : e) SBDX_HIP(hipEventCreate(&ev));
+      SBDX_HIP(hipEventRecord(e[0], c.stream));
+      sym_same_gemm<float>(c, o, x, y);
+      SBDX_HIP(hipEventRecord(e[1], c.stream));
+      sym_same_split(c, o.hii, x, o.T1, y);
+      SBDX_HIP(hipEventRecord(e[2], c.stream));
+      SBDX_HIP(hipEventSynchronize(e[2]));
+      SBDX_HIP(hipEventElapsedTime(&c.split_ms[0], e[0], e[1]));
+      SBDX_HIP(hipEventElapsedTime(&c.split_ms[1], e[1], e[2]));
+      for (auto &ev : e) SBDX_HIP(hipEventDestroy(ev));
+      c.split_mode = c.split_ms[1] < c.split_ms[0] ? 1 : 0;
+      if (c.split_mode == 1) return;  // y holds the split form's result
+    }
+  }
+  if (c.split_mode == 1)
+    sym_same_split(c, o.hii, x, o.T1, y);
+  else
+    sym_same_gemm<float>(c, o, x, y);
+}
+
 // Whether the symmetric form applies: equal string sets, the dense blocks
 // fit (same-spin autotuner chose or can hold them), the sliced-ELL table.
 static bool sym_ok(Context &c) {
@@ -1201,13 +1349,7 @@ static void sigma_sym_impl(Context &c, const T *x, T *y) {
   }
   {
     ProfScope ps(c, f32 ? "sym_same_f32" : "sym_same");
-    T *Ha, *Hb;
-    dense_blocks(c, Ha, Hb);
-    if (gemm(blas(c), n, n, n, x, n, Ha, n, o.T1, n, T(0)) != rocblas_status_success)
-      throw Error(3, "symmetric same-spin GEMM failed");
-    hipLaunchKernelGGL((k_sym_combine<T>), dim3(cdiv(n, 64), cdiv(n, 64)), dim3(256), 0, c.stream, n, o.hii,
-                       x, o.T1, y);
-    check_launch("k_sym_combine");
+    sym_same(c, o, x, y);
   }
   {
     ProfScope ps(c, f32 ? "sym_mixed_f32" : "sym_mixed");

@@ -44,7 +44,7 @@ ABI_SYMBOLS = [
     "sbdx_mlp_forward", "sbdx_mlp_gradient", "sbdx_mlp_train", "sbdx_mlp_record", "sbdx_mlp_collect",
     "sbdx_set_preconditioner", "sbdx_precond_stats", "sbdx_set_verbose",
     "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
-    "sbdx_mixed_staging", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry",
+    "sbdx_mixed_staging", "sbdx_split_gemm", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry",
     "sbdx_sigma_sym", "sbdx_mlp_apply", "sbdx_set_coarse_start", "sbdx_coarse_stats",
     "sbdx_set_graphs",
 ]
@@ -110,6 +110,7 @@ def lib():
     L.sbdx_same_spin_form.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]
     _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.sbdx_split_gemm.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.sbdx_mixed_staging.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_float), C.POINTER(C.c_int)]
     _u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
@@ -384,7 +385,9 @@ class TPBOperator:
         """Per precision: the same-spin form in use ('dense' GEMM, 'sparse'
         rows, None before the first sigma) and the autotuner's timings (ms),
         and the mixed kernel's tuned staging width and slots per thread for
-        the general and the spin-flip symmetric form (0 before first use)."""
+        the general and the spin-flip symmetric form (0 before first use);
+        for FP32 also the symmetric same-spin GEMM form (SGEMM or
+        split-FP16x3) and its deciding timings."""
         out = {}
         for prec, key in ((0, "fp64"), (1, "fp32")):
             f, a, b = C.c_int(), C.c_float(), C.c_float()
@@ -400,6 +403,10 @@ class TPBOperator:
                                             C.byref(mu)))
             out[key].update(sym_mixed_width=w.value, sym_mixed_slots_per_thread=nt.value,
                             sym_mixed_ms=round(ms.value, 3))
+        m, a, b = C.c_int(), C.c_float(), C.c_float()
+        _check(lib().sbdx_split_gemm(self.h, C.byref(m), C.byref(a), C.byref(b)))
+        out["fp32"].update(sym_gemm={-1: None, 0: "sgemm", 1: "split_fp16x3"}[m.value],
+                           sym_gemm_ms_sgemm=round(a.value, 3), sym_gemm_ms_split=round(b.value, 3))
         return out
 
     def profile(self, enable=True):

@@ -10,16 +10,19 @@ from conftest import molecule
 pytestmark = pytest.mark.gpu
 
 
-def test_graph_replays_match_eager():
+@pytest.mark.parametrize("split", ["0", "1"])
+def test_graph_replays_match_eager(split):
     import sbdx
-    from test_gpu_symmetry import symmetric_vector
+    from test_gpu_symmetry import _env, symmetric_vector
     inst = molecule("h2o", "1em3")
-    op = sbdx.TPBOperator.from_instance(inst)
+    with _env("SBDX_SPLIT_GEMM", split):
+        op = sbdx.TPBOperator.from_instance(inst)
+        op.sigma_sym(symmetric_vector(inst, 19), fp32=True)  # settles the GEMM form
     vs = [symmetric_vector(inst, 20 + k) for k in range(5)]
     eager = [op.sigma_sym(v, fp32=True) for v in vs]
     op.set_graphs(True)
     rng = np.random.default_rng(4)
-    print("[graph] h2o replays", flush=True)
+    print(f"[graph] h2o replays split={split}", flush=True)
     for k, v in enumerate(vs):
         op.sigma(rng.standard_normal(op.ndet))  # unrelated eager work between replays
         g = op.sigma_sym(v, fp32=True)

@@ -6,6 +6,8 @@ between the reference's interleaved order and the alpha-first order.  The
 symmetric-form product must equal the general product (FP64 to 1e-12, FP32
 to 2e-6), and a mixed-precision thick-restart solve in symmetric form from
 the Hartree-Fock vector must reach the oracle's Davidson energy."""
+import os
+
 import numpy as np
 import pytest
 
@@ -26,6 +28,14 @@ def basis_sign(astr, bstr, norb):
     return np.where(cnt % 2 == 1, -1.0, 1.0)
 
 
+def symmetric_part(inst, w):
+    """The spin-flip symmetric part of a vector in the reference convention."""
+    n = len(inst["astr"])
+    S = basis_sign(inst["astr"], inst["bstr"], inst["norb"])
+    X = (S * w.reshape(n, n))
+    return (S * (X + X.T) / 2).reshape(-1)
+
+
 def symmetric_vector(inst, seed):
     n = len(inst["astr"])
     rng = np.random.default_rng(seed)
@@ -40,19 +50,50 @@ def _cases():
     return {"tiny_same": CASES["tiny_same"], "h2o_1em3": CASES["h2o_1em3"]}
 
 
+class _env:
+    """Set one environment variable for the duration of a with-block."""
+
+    def __init__(self, name, value):
+        self.name, self.value = name, value
+
+    def __enter__(self):
+        self.old = os.environ.get(self.name)
+        os.environ[self.name] = self.value
+
+    def __exit__(self, *exc):
+        if self.old is None:
+            del os.environ[self.name]
+        else:
+            os.environ[self.name] = self.old
+
+
+# split "0": FP32 SGEMM in the symmetric same-spin term; "1": the
+# split-FP16x3 GEMM (FP16 hi/lo halves with power-of-two scales, three FP16
+# MFMA GEMMs accumulating in FP32), each checked against the FP64 product.
+@pytest.mark.parametrize("split", ["0", "1"])
 @pytest.mark.parametrize("case", ["tiny_same", "h2o_1em3"])
-def test_symmetric_sigma_matches_general(case):
+def test_symmetric_sigma_matches_general(case, split):
     import sbdx
     inst = _cases()[case]()
     assert np.array_equal(inst["astr"], inst["bstr"])
-    print(f"[symmetry] sigma {case}", flush=True)
-    op = sbdx.TPBOperator.from_instance(inst)
-    v = symmetric_vector(inst, 5)
-    s = op.sigma(v)
-    ss = op.sigma_sym(v)
-    assert np.max(np.abs(ss - s)) < 1e-12 * np.max(np.abs(s))
-    s32 = op.sigma_sym(v, fp32=True)
+    print(f"[symmetry] sigma {case} split={split}", flush=True)
+    with _env("SBDX_SPLIT_GEMM", split):
+        op = sbdx.TPBOperator.from_instance(inst)
+        v = symmetric_vector(inst, 5)
+        s = op.sigma(v)
+        ss = op.sigma_sym(v)
+        assert np.max(np.abs(ss - s)) < 1e-12 * np.max(np.abs(s))
+        s32 = op.sigma_sym(v, fp32=True)
+        # a vector with components over many decades: the scale and the
+        # absolute floor of the split keep the small ones accurate too
+        w = v * np.exp(-np.abs(np.random.default_rng(6).standard_normal(v.size)) * 8)
+        w = symmetric_part(inst, w)
+        w32 = op.sigma_sym(w, fp32=True)
+        forms = op.same_spin_forms()["fp32"]
+    assert forms["sym_gemm"] == ("split_fp16x3" if split == "1" else "sgemm")
     assert np.linalg.norm(s32 - s) / np.linalg.norm(s) < 2e-6
+    sw = op.sigma(w)
+    assert np.linalg.norm(w32 - sw) / np.linalg.norm(sw) < 2e-6
     op.close()
 
 
@@ -66,11 +107,13 @@ def test_symmetric_sigma_rejects_asymmetric_vector():
     op.close()
 
 
-@pytest.mark.parametrize("precond", ["diag", "mlp"])
-def test_symmetric_mixed_solve_matches_oracle(precond, oracle_mod):
+@pytest.mark.parametrize("precond,split", [("diag", "0"), ("diag", "1"), ("mlp", "1")])
+def test_symmetric_mixed_solve_matches_oracle(precond, split, oracle_mod):
     import sbdx
     inst = molecule("h2o", "1em3")
-    op = sbdx.TPBOperator.from_instance(inst)
+    with _env("SBDX_SPLIT_GEMM", split):
+        op = sbdx.TPBOperator.from_instance(inst)
+        op.sigma_sym(sbdx.basis_init_vector(op.ndet), fp32=True)  # settles the GEMM form
     orc = oracle_mod.Oracle.from_instance(inst)
     W0 = sbdx.basis_init_vector(op.ndet)
     ref = orc.davidson(W=W0, nb=12, eps=1e-9, max_it=300)
@@ -78,11 +121,10 @@ def test_symmetric_mixed_solve_matches_oracle(precond, oracle_mod):
     op.configure("mixed")
     op.set_symmetry(True)
     if precond == "mlp":
-        import os
         w = np.load(os.path.join(os.path.dirname(sbdx.__file__), "data", "precond_mlp.npy"))
         op.mlp_set_weights(w)
         op.set_preconditioner(1)
-    print(f"[symmetry] h2o mixed solve, {precond}", flush=True)
+    print(f"[symmetry] h2o mixed solve, {precond}, split={split}", flush=True)
     op.set_vector(W0)
     E, steps, conv = op.solve(max_it=300, nb=12, eps=1e-9)
     assert conv
