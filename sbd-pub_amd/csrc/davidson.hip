@@ -47,8 +47,12 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 }
 
 // part[b*nv + k] = sum over this block's slice of X_k . y
+// gate (may be null): the kernel does nothing when *gate == 0 (the
+// conditional second Gram-Schmidt pass, decided on the device).
 __global__ void __launch_bounds__(kRedThreads) k_dots(VecList X, int nv, const double *__restrict__ y,
-                                                      size_t n, double *__restrict__ part) {
+                                                      size_t n, double *__restrict__ part,
+                                                      const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;  // uniform per block, before any barrier
   __shared__ double sh[kRedThreads];
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,10 +301,11 @@ __global__ void k_finish_scalar(int nblk, const double *__restrict__ part, doubl
   if (threadIdx.x == 0 && blockIdx.x == 0) *out = sum_part(part, nblk, 1, 0);
 }
 
-static void dots(Context &c, const double *const *X, int nv, const double *y, double *part) {
+static void dots(Context &c, const double *const *X, int nv, const double *y, double *part,
+                 const int *gate = nullptr) {
   VecList L{};
   for (int k = 0; k < nv; k++) L.p[k] = X[k];
-  hipLaunchKernelGGL(k_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, L, nv, y, c.N, part);
+  hipLaunchKernelGGL(k_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, L, nv, y, c.N, part, gate);
   check_launch("k_dots");
 }
 
@@ -318,9 +323,10 @@ double device_dot(Context &c, const double *x, const double *y, size_t n) {
 }
 
 // out[k] = fixed-order sum over blocks of part[blk*nv + k]
-__global__ void k_finish_vec(int nv, int nblk, const double *__restrict__ part, double *__restrict__ out) {
+__global__ void k_finish_vec(int nv, int nblk, const double *__restrict__ part, double *__restrict__ out,
+                             const int *__restrict__ gate) {
   const int k = threadIdx.x;
-  if (k >= nv) return;
+  if (k >= nv || (gate && *gate == 0)) return;
   double s = 0.0;
   for (int b = 0; b < nblk; b++) s += part[(size_t)b * nv + k];
   out[k] = s;
@@ -329,7 +335,9 @@ __global__ void k_finish_vec(int nv, int nblk, const double *__restrict__ part, 
 // t -= sum_k coef[k] X_k (one classical Gram-Schmidt pass with coefficients
 // already reduced)
 __global__ void __launch_bounds__(kRedThreads) k_sub_combo(VecList X, int nv, const double *__restrict__ coef,
-                                                           double *__restrict__ t, size_t n) {
+                                                           double *__restrict__ t, size_t n,
+                                                           const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;  // uniform per block, before any barrier
   __shared__ double cf[kNbMax];
   if (threadIdx.x < nv) cf[threadIdx.x] = coef[threadIdx.x];
   __syncthreads();
@@ -338,6 +346,30 @@ __global__ void __launch_bounds__(kRedThreads) k_sub_combo(VecList X, int nv, co
     double acc = t[i];
     for (int k = 0; k < nv; k++) acc -= cf[k] * X.p[k][i];
     t[i] = acc;
+  }
+}
+
+// First Gram-Schmidt pass of t against nv basis vectors: part holds the
+// partial dots of the nv vectors and, as entry nv, of t with itself.
+// coef[k] = <X_k|t>; *gate = 1 when the pass leaves less than half of
+// |t|^2 (|t|^2 - sum coef^2 < |t|^2 / 2), so that a second pass is needed
+// to keep t orthogonal (Daniel, Gragg, Kaufman and Stewart 1976,
+// threshold 1/sqrt(2) on the norm), else 0.  One block of kNbMax threads.
+__global__ void k_finish_dgks(int nv, int nblk, const double *__restrict__ part, double *__restrict__ coef,
+                              int *__restrict__ gate) {
+  __shared__ double s[kNbMax + 1];
+  const int k = threadIdx.x;
+  if (k <= nv) {
+    double a = 0.0;
+    for (int b = 0; b < nblk; b++) a += part[(size_t)b * (nv + 1) + k];
+    s[k] = a;
+    if (k < nv) coef[k] = a;
+  }
+  __syncthreads();
+  if (k == 0) {
+    double hh = 0.0;
+    for (int j = 0; j < nv; j++) hh += s[j] * s[j];
+    *gate = (s[nv] - hh < 0.5 * s[nv]) ? 1 : 0;
   }
 }
 
@@ -772,20 +804,30 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       }
     }
     {
-      // classical Gram-Schmidt twice against the d+1 basis vectors: one
-      // multi-dot and one combined update per pass
+      // classical Gram-Schmidt against the d+1 basis vectors (one multi-dot
+      // and one combined update per pass); the second pass runs only when
+      // the first removed more than half of |cn|^2, decided on the device
+      // (k_finish_dgks sets the gate the second pass's kernels read)
       ProfScope ps(c, "dav_orth");
       double *cn = c.C[d + 1];
       double *coef = c.dsmall + kCoefOffset;
+      int *gate = reinterpret_cast<int *>(c.dsmall + kScalarOffset + 24);
       VecList Cv{};
       for (int k = 0; k <= d; k++) Cv.p[k] = c.C[k];
-      for (int pass = 0; pass < 2; pass++) {
-        dots(c, c.C.data(), d + 1, cn, c.part);
-        hipLaunchKernelGGL(k_finish_vec, dim3(1), dim3(kNbMax), 0, c.stream, d + 1, kRedBlocks, c.part, coef);
-        check_launch("k_finish_vec");
-        hipLaunchKernelGGL(k_sub_combo, g, b, 0, c.stream, Cv, d + 1, coef, cn, N);
-        check_launch("k_sub_combo");
-      }
+      const double *Xw[kNbMax];
+      for (int k = 0; k <= d; k++) Xw[k] = c.C[k];
+      Xw[d + 1] = cn;  // d + 2 <= nb <= kNbMax
+      dots(c, Xw, d + 2, cn, c.part);
+      hipLaunchKernelGGL(k_finish_dgks, dim3(1), dim3(kNbMax), 0, c.stream, d + 1, kRedBlocks, c.part, coef, gate);
+      check_launch("k_finish_dgks");
+      hipLaunchKernelGGL(k_sub_combo, g, b, 0, c.stream, Cv, d + 1, coef, cn, N, (const int *)nullptr);
+      check_launch("k_sub_combo");
+      dots(c, c.C.data(), d + 1, cn, c.part, gate);
+      hipLaunchKernelGGL(k_finish_vec, dim3(1), dim3(kNbMax), 0, c.stream, d + 1, kRedBlocks, c.part, coef,
+                         (const int *)gate);
+      check_launch("k_finish_vec");
+      hipLaunchKernelGGL(k_sub_combo, g, b, 0, c.stream, Cv, d + 1, coef, cn, N, (const int *)gate);
+      check_launch("k_sub_combo");
       const double *X[1] = {cn};
       dots(c, X, 1, cn, c.part);
       hipLaunchKernelGGL(k_scale_part, g, b, 0, c.stream, cn, N, c.part, kRedBlocks);
@@ -985,7 +1027,8 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
   auto residuals = [&](int nv, const double *Wd, const double *HWd, const double *Ed) {
     hipLaunchKernelGGL(k_block_resid, g, bthr, 0, c.stream, nv, N, Wd, HWd, Ed, c.blk_R, c.part);
     check_launch("k_block_resid");
-    hipLaunchKernelGGL(k_finish_vec, dim3(1), dim3(kNbMax), 0, c.stream, nv, kRedBlocks, c.part, nrm);
+    hipLaunchKernelGGL(k_finish_vec, dim3(1), dim3(kNbMax), 0, c.stream, nv, kRedBlocks, c.part, nrm,
+                       (const int *)nullptr);
     check_launch("k_finish_vec");
     SBDX_HIP(hipMemcpyAsync(hE.data(), Ed, sizeof(double) * nv, hipMemcpyDeviceToHost, c.stream));
     SBDX_HIP(hipMemcpyAsync(hR.data(), nrm, sizeof(double) * nv, hipMemcpyDeviceToHost, c.stream));
