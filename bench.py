@@ -241,13 +241,15 @@ def main():
             nsteps.append(n)
             conv.append(ok)
             tr = op.trace()
-            extra = f" coarse={op.coarse_stats()}" if solver in ("thick", "mixed") else ""
+            extra = (f" coarse={op.coarse_stats()} fp64-from-step={op.fp64_switch_step()}"
+                     if solver in ("thick", "mixed") else "")
             log(f"{solver}/{precond} solve {k}: E0={E:.12f} |r|={tr[-1, 1]:.2e} sigma-products={n} "
                 f"converged={ok} time={dt:.3f} s precond-steps={op.precond_stats()}{extra}")
         stats = op.profile_stats()
         op.profile(False)
         return dict(time=float(np.median(times)), E=energies[-1], steps=int(np.median(nsteps)),
                     converged=all(conv), stats=stats, precond_steps=op.precond_stats(), roots=roots,
+                    fp64_switch_step=op.fp64_switch_step() if solver in ("thick", "mixed") else None,
                     coarse=op.coarse_stats() if solver in ("thick", "mixed") else None)
 
     best = args.solver
@@ -366,6 +368,7 @@ def main():
             dict(strings=args.coarse, eps=args.coarse_eps, **res["coarse"]),
             "preconditioner": args.precond,
             "precond_steps": res["precond_steps"],
+            "fp64_products_from_step": res.get("fp64_switch_step"),
             "sigma_products": res["steps"],
             "converged": res["converged"],
             "E0": res["E"],

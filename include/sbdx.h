@@ -169,6 +169,15 @@ int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_spa
 int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, int *slots_per_thread,
                        float *ms, int *multi);
 
+/* Mixed-precision thick solves (no counterpart in the reference): FP32
+ * correction products cannot take the residual below roughly 1e-7 |H|, and
+ * below that floor their errors feed back through the kept Ritz vectors.
+ * When an exact restart residual fails to halve while below
+ * 1000 x 2^-24 x |E0|, the solve continues with FP64 products (that restart
+ * keeps W alone). *step receives the step of the last solve at which this
+ * happened, or -1 when every correction product of it was FP32. */
+int sbdx_fp64_switch_step(sbdx_handle_t h, int *step);
+
 /* Form of the GEMM in the spin-flip symmetric FP32 same-spin term (no
  * counterpart in the reference): *mode -1 before the first such product, 0
  * FP32 SGEMM, 1 split-FP16x3 (x and H_alpha each written as 2^-e (hi + lo)

@@ -496,6 +496,15 @@ int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_spa
   SBDX_GUARD_END
 }
 
+int sbdx_fp64_switch_step(sbdx_handle_t h, int *step) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!step) throw Error(SBDX_ERR_ARG, "null pointer");
+  *step = c.fp64_switch_step;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
 int sbdx_split_gemm(sbdx_handle_t h, int *mode, float *ms_sgemm, float *ms_split) {
   SBDX_GUARD_BEGIN
   Context &c = ctx_of(h);

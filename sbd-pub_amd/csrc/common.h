@@ -132,6 +132,7 @@ struct Context {
   // decided, 0 FP32 SGEMM, 1 split), the deciding timings (ms: SGEMM,
   // split), FP16 hi/lo halves of H_alpha and of the current x, per-block
   // maxima, and the two inverse power-of-two scales (x's, H's)
+  int fp64_switch_step = -1;  // last thick solve: step from which products were FP64 (-1 never)
   int split_mode = -1;
   float split_ms[2] = {0.f, 0.f};
   void *Hh = nullptr, *Hl = nullptr, *Xh = nullptr, *Xl = nullptr;

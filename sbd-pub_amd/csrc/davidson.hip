@@ -22,6 +22,9 @@
 namespace sbdx {
 
 constexpr int kRedBlocks = 1024;
+// FP32 noise level of a product relative to |E0| (1000 x 2^-24): below
+// kFp32Floor |E0| a stalled mixed-precision solve switches to FP64 products
+constexpr double kFp32Floor = 1000.0 * 5.9604644775390625e-8;
 constexpr int kRedThreads = 256;
 
 struct OutList {
@@ -671,6 +674,8 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
   bool done = false, fresh = true;
   double E0 = 0.0, nr = 0.0;
   std::vector<double> rhist;
+  double prev_exact = -1.0;  // exact residual of the previous restart
+  c.fp64_switch_step = -1;
   int diag_left = 0;
   c.mlp.used_mlp = c.mlp.used_diag = 0;
   c.mlp.rec_n = 0;
@@ -735,6 +740,24 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
         break;
       }
       if (timeout) break;
+      // precision switch: FP32 product errors (~1e-7 |H| per product) bound
+      // how far FP32 corrections can take the residual, and below that
+      // floor they feed errors back through the kept Ritz vectors.  When an
+      // exact restart residual fails to halve while already small against
+      // the FP32 noise level of this spectrum, the rest of the solve uses
+      // FP64 products, and this restart keeps W alone (its product exact).
+      int keep_now = m;
+      if (mixed && prev_exact > 0.0 && nr > 0.5 * prev_exact && nr < kFp32Floor * std::fabs(E0)) {
+        mixed = false;
+        keep_now = 1;
+        c.fp64_switch_step = steps;
+        if (c.verbose) {
+          fprintf(stderr, "[sbdx] FP32 product floor reached at |r| = %.3e: FP64 products from step %d\n", nr,
+                  steps);
+          fflush(stderr);
+        }
+      }
+      prev_exact = nr;
       // R /= |R| for the preconditioner (the reference normalizes R too)
       {
         const double *Y[1] = {c.R};
@@ -742,30 +765,30 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
         hipLaunchKernelGGL(k_scale_part, g, b, 0, c.stream, c.R, N, c.part, kRedBlocks);
         check_launch("k_scale_part");
       }
-      // thick restart basis: W (exact product) + Ritz vectors 1..m-1
-      if (m > 1) {
+      // thick restart basis: W (exact product) + Ritz vectors 1..keep_now-1
+      if (keep_now > 1) {
         VecList Cv{}, HCv{};
         for (int k = 0; k <= d; k++) {
           Cv.p[k] = c.C[k];
           HCv.p[k] = c.HC[k];
         }
         OutList Yo{}, HYo{};
-        for (int j = 1; j < m; j++) {
+        for (int j = 1; j < keep_now; j++) {
           Yo.p[j - 1] = c.spareY[j];
           HYo.p[j - 1] = c.spareHY[j];
         }
-        hipLaunchKernelGGL(k_combine, g, b, 0, c.stream, Cv, d + 1, U, ld, 1, m - 1, Yo, N);
+        hipLaunchKernelGGL(k_combine, g, b, 0, c.stream, Cv, d + 1, U, ld, 1, keep_now - 1, Yo, N);
         check_launch("k_combine");
-        hipLaunchKernelGGL(k_combine, g, b, 0, c.stream, HCv, d + 1, U, ld, 1, m - 1, HYo, N);
+        hipLaunchKernelGGL(k_combine, g, b, 0, c.stream, HCv, d + 1, U, ld, 1, keep_now - 1, HYo, N);
         check_launch("k_combine");
       }
       SBDX_HIP(hipMemcpyAsync(c.C[0], c.W, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
       std::swap(c.HC[0], c.spareHY[0]);
-      for (int j = 1; j < m; j++) {
+      for (int j = 1; j < keep_now; j++) {
         std::swap(c.C[j], c.spareY[j]);
         std::swap(c.HC[j], c.spareHY[j]);
       }
-      d = m - 1;
+      d = keep_now - 1;
       for (int j = 0; j <= d; j++) add_column(j);
       fresh = true;
       // expand from the exact residual (E0 = exact energy for the preconditioner)

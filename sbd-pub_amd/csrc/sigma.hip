@@ -260,14 +260,39 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
     for (int t = 0; t < NT; t++) {
       const int slot = tid + t * 1024;
       if (slot < nslots) {
-        const int sl = slot >> 6;
-        const uint32_t len = sl_len[sl];
-        const uint32_t *p = ell + sl_off[sl] + (slot & 63);
+        // a wavefront's 64 slots lie in one slice (slot bases are multiples
+        // of 64), so the slice's length and offset are wave-uniform
+        const int sl = __builtin_amdgcn_readfirstlane(slot >> 6);
+        const uint32_t len = __builtin_amdgcn_readfirstlane(sl_len[sl]);
+        const uint32_t *p = ell + sl_off[sl] + (slot & 63);  // entry k at p[64k]
         V a = V(T(0));
-#pragma unroll 4
-        for (uint32_t k = 0; k < len; k++) {
-          const uint32_t w = p[64 * (size_t)k];
-          a += W[(w >> 16) & 0x7fffu] * flip_sign<T, EB>(X[w & 0xffffu], w & 0x80000000u);
+        if (len > 0) {
+          // software-pipelined over groups of 4 entries: the next group's
+          // table words load while this group's LDS reads and FMAs run; the
+          // loads run up to 4 positions past the slice's end (into the next
+          // slice or the table's 4 x 64 padding entries) and those words are
+          // never used
+          uint32_t nxt[4];
+          const uint32_t *q = p;  // first entry of the group being loaded
+#pragma unroll
+          for (uint32_t u = 0; u < 4; u++) nxt[u] = q[64 * u];
+          uint32_t k = 0;
+          for (; k + 4 <= len; k += 4) {
+            uint32_t cur[4];
+            q += 4 * 64;
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) {
+              cur[u] = nxt[u];
+              nxt[u] = q[64 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+              a += W[(cur[u] >> 16) & 0x7fffu] * flip_sign<T, EB>(X[cur[u] & 0xffffu], cur[u] & 0x80000000u);
+          }
+#pragma unroll
+          for (uint32_t u = 0; u < 3; u++)  // the last len - k < 4 entries, already in nxt
+            if (k + u < len)
+              a += W[(nxt[u] >> 16) & 0x7fffu] * flip_sign<T, EB>(X[nxt[u] & 0xffffu], nxt[u] & 0x80000000u);
         }
         a += X[rr[t]] * diag_sum(W, occ[t], L1);
         acc[t] += a;

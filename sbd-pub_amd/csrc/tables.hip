@@ -307,7 +307,10 @@ static void ell_layout(Context &c, const SpinTables &t, const std::vector<uint64
     total += (uint64_t)mx * 64;
   }
   const uint32_t pad = (uint32_t)c.K << 16;
-  ell.assign(total + 1, pad);
+  // 4 x 64 padding entries past the end: the mixed kernel prefetches the
+  // next 4 positions of a slice unconditionally (past a slice's end it
+  // reads the next slice's entries or these, and never uses them)
+  ell.assign(total + 4 * 64 + 1, pad);
   std::vector<uint32_t> rows[64];
   for (int s = 0; s < nsl; s++) {
     for (int l = 0; l < 64; l++) {

@@ -44,7 +44,7 @@ ABI_SYMBOLS = [
     "sbdx_mlp_forward", "sbdx_mlp_gradient", "sbdx_mlp_train", "sbdx_mlp_record", "sbdx_mlp_collect",
     "sbdx_set_preconditioner", "sbdx_precond_stats", "sbdx_set_verbose",
     "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
-    "sbdx_mixed_staging", "sbdx_split_gemm", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry",
+    "sbdx_mixed_staging", "sbdx_split_gemm", "sbdx_fp64_switch_step", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry",
     "sbdx_sigma_sym", "sbdx_mlp_apply", "sbdx_set_coarse_start", "sbdx_coarse_stats",
     "sbdx_set_graphs",
 ]
@@ -110,6 +110,7 @@ def lib():
     L.sbdx_same_spin_form.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]
     _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.sbdx_fp64_switch_step.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.sbdx_split_gemm.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.sbdx_mixed_staging.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_float), C.POINTER(C.c_int)]
@@ -278,6 +279,13 @@ class TPBOperator:
         """Refine the start vector on the first m strings of each set first
         (thick-restart solves); m = 0 switches it off."""
         _check(lib().sbdx_set_coarse_start(self.h, int(m), float(eps)))
+
+    def fp64_switch_step(self):
+        """Step of the last thick solve from which its products were FP64
+        (the mixed-precision floor was reached), or -1."""
+        n = C.c_int()
+        _check(lib().sbdx_fp64_switch_step(self.h, C.byref(n)))
+        return n.value
 
     def coarse_stats(self):
         n, e, t = C.c_int(), C.c_double(), C.c_double()

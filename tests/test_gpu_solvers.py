@@ -85,3 +85,28 @@ def test_coarse_start_reaches_oracle(name, solver, sym, m, oracle_mod):
     assert np.linalg.norm(orc.sigma(W) - E * W) < 5e-9
     op.close()
     orc.close()
+
+
+# A mixed-precision solve asked for a residual far below what FP32 correction
+# products can reach on their own: the solver must still converge (switching
+# to FP64 products when its exact restart residuals stall) to the oracle's
+# energy, with an exact residual below the target.
+def test_mixed_solve_below_fp32_floor(oracle_mod):
+    import sbdx
+    from test_gpu_parity import CASES
+    inst = CASES["h2o_1em3"]()
+    op = sbdx.TPBOperator.from_instance(inst)
+    orc = oracle_mod.Oracle.from_instance(inst)
+    W0 = sbdx.basis_init_vector(op.ndet)
+    ref = orc.davidson(W=W0, nb=12, eps=1e-11, max_it=400)
+    print("[fp32 floor] h2o mixed eps 1e-12", flush=True)
+    op.configure("mixed")
+    op.set_vector(W0)
+    E, steps, conv = op.solve(max_it=400, nb=12, eps=1e-12)
+    print(f"[fp32 floor] products {steps}, FP64 from step {op.fp64_switch_step()}", flush=True)
+    assert conv
+    assert abs(E - ref["E"]) < 1e-10
+    W = op.get_vector()
+    assert np.linalg.norm(orc.sigma(W) - E * W) < 1e-11
+    op.close()
+    orc.close()
