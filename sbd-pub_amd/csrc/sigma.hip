@@ -419,7 +419,7 @@ struct PartTables {
   const uint32_t *len[8];
 };
 
-template <typename T, int NT>
+template <typename T, int NT, bool SYM = false>
 __global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int L, int nslots, int nparts,
                                                       int h, const uint64_t *__restrict__ a_off,
                                                       const uint64_t *__restrict__ a_ent,
@@ -432,8 +432,10 @@ __global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int L, int 
   T *crow = reinterpret_cast<T *>(lds_raw);
   T *vrow = crow + h;
   const int L1 = L + 1;
-  const int I = blockIdx.x;
-  const int tid = threadIdx.x;
+  const int pos = blockIdx.x;
+  const int I = SYM ? (int)perm[pos] : blockIdx.x;
+  // SYM: slots from pos's slice on (as in k_mixed_ell); NT covers the rest
+  const int tid = (SYM ? (pos & ~63) : 0) + threadIdx.x;
   T acc[NT];
   uint64_t occ[NT];
   int rr[NT];
@@ -456,16 +458,16 @@ __global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int L, int 
       J = ent_ket(ent);
       const T sa = ent_neg(ent) ? T(-1) : T(1);
       const T *vsrc = eri + (size_t)ent_pq(ent) * K;
-      for (int i = tid; i < K; i += 1024) vrow[i] = sa * vsrc[i];
+      for (int i = threadIdx.x; i < K; i += 1024) vrow[i] = sa * vsrc[i];
     } else {
-      for (int i = tid; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
+      for (int i = threadIdx.x; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
     }
-    if (tid == 0) vrow[K] = T(0);
+    if (threadIdx.x == 0) vrow[K] = T(0);
     for (int p = 0; p < nparts; p++) {
       const int lo = p * h;
       const int len_p = (nb - lo) < h ? (nb - lo) : h;
       __syncthreads();
-      for (int i = tid; i < len_p; i += 1024) crow[i] = x[J * nb + lo + i];
+      for (int i = threadIdx.x; i < len_p; i += 1024) crow[i] = x[J * nb + lo + i];
       __syncthreads();
       const uint32_t *ellp = pt.ell[p];
       const uint64_t *offp = pt.off[p];
@@ -496,7 +498,14 @@ __global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int L, int 
     const int slot = tid + t * 1024;
     if (slot < nslots) {
       const uint32_t r = perm[slot];
-      if (r != 0xffffffffu) y[(size_t)I * nb + r] += acc[t];
+      if (SYM) {
+        if (r != 0xffffffffu && slot >= pos) {
+          y[(size_t)I * nb + r] += acc[t];
+          if (slot > pos) y[(size_t)r * nb + I] += acc[t];
+        }
+      } else {
+        if (r != 0xffffffffu) y[(size_t)I * nb + r] += acc[t];
+      }
     }
   }
 }
@@ -840,9 +849,9 @@ static void launch_mixed_lds(Context &c, const T *eri, const T *x, T *y, size_t 
                      c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.s_off, c.B.s_ent, eri, x, y);
 }
 
-template <typename T, int NT>
+template <typename T, int NT, bool SYM = false>
 static void launch_mixed_parts(Context &c, const T *eri, const T *x, T *y, size_t lds) {
-  auto fn = k_mixed_parts<T, NT>;
+  auto fn = k_mixed_parts<T, NT, SYM>;
   SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   PartTables pt{};
   for (int p = 0; p < c.B.nparts; p++) {
@@ -856,7 +865,7 @@ static void launch_mixed_parts(Context &c, const T *eri, const T *x, T *y, size_
 }
 
 // Ket row staged in parts: smallest part count (<= 8) whose piece and ERI row fit.
-template <typename T>
+template <typename T, bool SYM = false>
 static bool mixed_parts_dispatch(Context &c, const T *eri, const T *x, T *y) {
   const int nt = (c.B.nslice * 64 + 1023) / 1024;
   if (nt > 32) return false;
@@ -867,12 +876,12 @@ static bool mixed_parts_dispatch(Context &c, const T *eri, const T *x, T *y) {
     const size_t lds = sizeof(T) * (h + c.K + 1);
     if (lds > c.lds_limit) continue;
     build_singles_ell_parts(c, c.B, P);
-    if (nt <= 1) launch_mixed_parts<T, 1>(c, eri, x, y, lds);
-    else if (nt <= 2) launch_mixed_parts<T, 2>(c, eri, x, y, lds);
-    else if (nt <= 4) launch_mixed_parts<T, 4>(c, eri, x, y, lds);
-    else if (nt <= 8) launch_mixed_parts<T, 8>(c, eri, x, y, lds);
-    else if (nt <= 16) launch_mixed_parts<T, 16>(c, eri, x, y, lds);
-    else launch_mixed_parts<T, 32>(c, eri, x, y, lds);
+    if (nt <= 1) launch_mixed_parts<T, 1, SYM>(c, eri, x, y, lds);
+    else if (nt <= 2) launch_mixed_parts<T, 2, SYM>(c, eri, x, y, lds);
+    else if (nt <= 4) launch_mixed_parts<T, 4, SYM>(c, eri, x, y, lds);
+    else if (nt <= 8) launch_mixed_parts<T, 8, SYM>(c, eri, x, y, lds);
+    else if (nt <= 16) launch_mixed_parts<T, 16, SYM>(c, eri, x, y, lds);
+    else launch_mixed_parts<T, 32, SYM>(c, eri, x, y, lds);
     return true;
   }
   return false;
@@ -1381,15 +1390,21 @@ static void sigma_sym_impl(Context &c, const T *x, T *y) {
   const SigmaOperands<T> o = operands(c, (T *)nullptr);
   const bool f32 = sizeof(T) == 4;
   const int n = (int)c.na;
-  if (sizeof(T) * ((size_t)c.nb + c.K + 1) > c.lds_limit) {
-    // one staged pair does not fit in LDS (e.g. FP64 at 20000 strings): the
-    // general form (row parts) gives the same product for symmetric x
-    sigma_impl<T>(c, x, y);
-    return;
-  }
   {
     ProfScope ps(c, f32 ? "sym_same_f32" : "sym_same");
     sym_same(c, o, x, y);
+  }
+  if (sizeof(T) * ((size_t)c.nb + c.K + 1) > c.lds_limit) {
+    // one staged pair does not fit in LDS (e.g. FP64 at 20000 strings): the
+    // ket rows staged in parts, same triangle of slots; if even that does not
+    // fit, the general product (equal for symmetric x)
+    ProfScope ps(c, f32 ? "sym_mixed_f32" : "sym_mixed");
+    if (!mixed_parts_dispatch<T, true>(c, o.eri, x, y)) {
+      sigma_impl<T>(c, x, y);
+      return;
+    }
+    check_launch("k_mixed_parts<sym>");
+    return;
   }
   {
     ProfScope ps(c, f32 ? "sym_mixed_f32" : "sym_mixed");

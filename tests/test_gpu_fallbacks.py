@@ -65,3 +65,22 @@ def test_same_spin_forms_match_oracle(form, case, oracle_mod):
     assert np.linalg.norm(s32 - r) / np.linalg.norm(r) < 2e-6
     op.close()
     orc.close()
+
+
+# The spin-flip symmetric product when one staged pair does not fit in LDS:
+# the row-parts kernel over the same triangle of slots (lowered LDS budget on
+# h2o-1em3, equal string sets), FP64 and FP32, against the general product.
+def test_symmetric_parts_match_general():
+    import sbdx
+    from test_gpu_symmetry import _env, symmetric_vector
+    m = molecule("h2o", "1em3")
+    print("[fallback] symmetric row parts, LDS limit 6000", flush=True)
+    with _env("SBDX_LDS_LIMIT", "6000"):
+        op = sbdx.TPBOperator.from_instance(m)
+    v = symmetric_vector(m, 12)
+    s = op.sigma(v)
+    ss = op.sigma_sym(v)
+    assert np.max(np.abs(ss - s)) < 1e-12 * np.max(np.abs(s))
+    s32 = op.sigma_sym(v, fp32=True)
+    assert np.linalg.norm(s32 - s) / np.linalg.norm(s) < 2e-6
+    op.close()
