@@ -8,6 +8,10 @@
 //   load_strings     chemistry/basic/makedeterminants.h:160-185 with
 //                    framework/bit_manipulation.h:215-229 (rightmost char is
 //                    orbital 0) and the sort/unique of :149-168, 381-395
+//   write_rdm_text   the 1pRDM.txt / 2pRDM.txt files and
+//   energy_decomposition  the zero-/one-/two-body energies of the driver
+//                    apps/chemistry_tpb_selected_basis_diagonalization/
+//                    main.cc:264-318
 #ifndef SBDX_IO_HPP
 #define SBDX_IO_HPP
 
@@ -127,6 +131,55 @@ inline std::vector<std::vector<size_t>> load_strings(const std::string &path, si
   });
   out.erase(std::unique(out.begin(), out.end()), out.end());
   return out;
+}
+
+// 1pRDM.txt and 2pRDM.txt in directory `dir`: "i j value" for i, j (outer
+// to inner) with one[0][i+L*j] + one[1][i+L*j], and "i j k l value" with
+// the four spin blocks of two[.][i+L*j+L^2*(k+L*l)] summed in block order,
+// through streams of precision 16 (the driver's text format).
+inline void write_rdm_text(const std::string &dir, size_t L, const std::vector<std::vector<double>> &one,
+                           const std::vector<std::vector<double>> &two) {
+  std::ofstream f1(dir + "/1pRDM.txt");
+  f1.precision(16);
+  for (size_t i = 0; i < L; i++)
+    for (size_t j = 0; j < L; j++) f1 << i << " " << j << " " << one[0][i + L * j] + one[1][i + L * j] << "\n";
+  std::ofstream f2(dir + "/2pRDM.txt");
+  f2.precision(16);
+  for (size_t i = 0; i < L; i++)
+    for (size_t j = 0; j < L; j++)
+      for (size_t k = 0; k < L; k++)
+        for (size_t l = 0; l < L; l++) {
+          const size_t x = i + L * j + L * L * (k + L * l);
+          f2 << i << " " << j << " " << k << " " << l << " " << two[0][x] + two[1][x] + two[2][x] + two[3][x]
+             << "\n";
+        }
+  if (!f1 || !f2) throw std::runtime_error("cannot write the RDM text files in " + dir);
+}
+
+struct EnergyParts {
+  double zero = 0.0, one = 0.0, two = 0.0;
+};
+
+// I0; sum_ij h_ij (one[0] + one[1])[i+L*j]; 1/2 sum_ijkl (ik|jl)
+// sum_s two[s][i+L*j+L^2*(k+L*l)], with h and (ik|jl) read from the
+// integral stores (spin-orbitals 2i, 2k, 2j, 2l).
+inline EnergyParts energy_decomposition(const Integrals &g, const std::vector<std::vector<double>> &one,
+                                        const std::vector<std::vector<double>> &two) {
+  const size_t L = (size_t)g.L, L2 = 2 * L;
+  auto tri = [](size_t a, size_t b) { return a > b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; };
+  EnergyParts e;
+  e.zero = g.I0;
+  for (size_t i = 0; i < L; i++)
+    for (size_t j = 0; j < L; j++) e.one += g.I1.store[(2 * i) * L2 + 2 * j] * (one[0][i + L * j] + one[1][i + L * j]);
+  for (size_t i = 0; i < L; i++)
+    for (size_t j = 0; j < L; j++)
+      for (size_t k = 0; k < L; k++)
+        for (size_t l = 0; l < L; l++) {
+          const size_t x = i + L * j + L * L * (k + L * l);
+          const double v = g.I2.store[tri(tri(i, k), tri(j, l))];
+          for (int s = 0; s < 4; s++) e.two += 0.5 * v * two[s][x];
+        }
+  return e;
 }
 
 }  // namespace sbdx

@@ -466,9 +466,12 @@ def davidson(op: TPBOperator, W=None, max_it=100, nb=12, eps=1e-8, max_time=1e30
 
 
 def diag(fcidump_path, adet_path, bdet_path=None, max_it=100, nb=12, eps=1e-8, do_rdm=True,
-         bit_length=20):
+         bit_length=20, rdm_dir=None):
     """FCIDUMP + string files -> energy <W|HW>, density and RDMs, like
-    sbd::tpb::diag (chemistry/tpb/sbdiag.h:139-763) on one GPU."""
+    sbd::tpb::diag (chemistry/tpb/sbdiag.h:139-763) on one GPU.  With
+    rdm_dir, also the driver's 1pRDM.txt / 2pRDM.txt there and its zero-,
+    one- and two-body energies in out["energy_parts"]
+    (apps/chemistry_tpb_selected_basis_diagonalization/main.cc:264-318)."""
     fd = io.read_fcidump(fcidump_path)
     I0, I1, I2 = io.setup_integrals(fd)
     L = fd["norb"]
@@ -482,5 +485,9 @@ def diag(fcidump_path, adet_path, bdet_path=None, max_it=100, nb=12, eps=1e-8, d
         one, two = op.correlation(W)
         out["one"], out["two"] = one, two
         out["density"] = np.array([one[s * L * L + i + L * i] for i in range(L) for s in range(2)])
+        if rdm_dir is not None:
+            io.write_rdm_text(rdm_dir, L, one, two)
+            out["energy_parts"] = io.energy_decomposition(I0, io.spatial_h(L, I1), io.dense_eri(L, I2),
+                                                          one, two)
     op.close()
     return out

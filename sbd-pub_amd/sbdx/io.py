@@ -22,6 +22,7 @@ Formats (reference citations relative to the sbd-pub tree):
 from __future__ import annotations
 
 import gzip
+import os
 
 import numpy as np
 
@@ -216,3 +217,61 @@ def words_to_masks(words, norb, bit_length=20):
     for w in range(words.shape[1]):
         out |= words[:, w] << np.uint64(w * bit_length)
     return out
+
+
+def _g16(v: float) -> str:
+    # std::ostream with precision(16) and the default float field prints as
+    # printf("%.16g")
+    return f"{v:.16g}"
+
+
+def write_rdm_text(directory, norb, one, two):
+    """1pRDM.txt and 2pRDM.txt in the reference driver's text format
+    (apps/chemistry_tpb_selected_basis_diagonalization/main.cc:275-303):
+    "i j value" for i, j (outer to inner) with value one[0][i+L*j] +
+    one[1][i+L*j], and "i j k l value" for i, j, k, l with the four spin
+    blocks of two[.][i+L*j+L^2*(k+L*l)] summed in block order; 16
+    significant digits.  one: 2*L^2 values, two: 4*L^4 (the layouts of
+    TPBOperator.correlation)."""
+    L = int(norb)
+    o = np.asarray(one, dtype=np.float64).reshape(2, L * L)
+    t = np.asarray(two, dtype=np.float64).reshape(4, L ** 4)
+    g1 = o[0] + o[1]
+    g2 = ((t[0] + t[1]) + t[2]) + t[3]
+    with open(os.path.join(directory, "1pRDM.txt"), "w") as f:
+        f.write("".join(f"{i} {j} {_g16(g1[i + L * j])}\n" for i in range(L) for j in range(L)))
+    with open(os.path.join(directory, "2pRDM.txt"), "w") as f:
+        for i in range(L):
+            f.write("".join(f"{i} {j} {k} {l} {_g16(g2[i + L * j + L * L * (k + L * l)])}\n"
+                            for j in range(L) for k in range(L) for l in range(L)))
+
+
+def read_rdm_text(directory, norb):
+    """The spin-summed 1- and 2-RDM from 1pRDM.txt / 2pRDM.txt, as flat
+    arrays indexed i+L*j and i+L*j+L^2*(k+L*l)."""
+    L = int(norb)
+    g1 = np.zeros(L * L)
+    for line in open(os.path.join(directory, "1pRDM.txt")):
+        i, j, v = line.split()
+        g1[int(i) + L * int(j)] = float(v)
+    g2 = np.zeros(L ** 4)
+    for line in open(os.path.join(directory, "2pRDM.txt")):
+        i, j, k, l, v = line.split()
+        g2[int(i) + L * int(j) + L * L * (int(k) + L * int(l))] = float(v)
+    return g1, g2
+
+
+def energy_decomposition(I0, h, eri, one, two):
+    """Zero-, one- and two-body energies from the RDMs, as the reference
+    driver prints them (main.cc:279-281, 296-299): I0; sum_ij h_ij (one[0] +
+    one[1])[i+L*j]; 1/2 sum_ijkl (ik|jl) sum_s two[s][i+L*j+L^2*(k+L*l)].
+    h: L x L, eri: L^4 array indexed [p, q, r, s] = (pq|rs)."""
+    h = np.asarray(h, dtype=np.float64)
+    L = h.shape[0]
+    o = np.asarray(one, dtype=np.float64).reshape(2, L, L)       # [s, j, i]
+    t = np.asarray(two, dtype=np.float64).reshape(4, L, L, L, L)  # [s, l, k, j, i]
+    g1 = (o[0] + o[1]).T                                          # [i, j]
+    g2 = t.sum(axis=0).transpose(3, 2, 1, 0)                      # [i, j, k, l]
+    e1 = float(np.einsum("ij,ij->", h, g1))
+    e2 = 0.5 * float(np.einsum("ikjl,ijkl->", np.asarray(eri).reshape(L, L, L, L), g2))
+    return dict(zero=float(I0), one=e1, two=e2, total=float(I0) + e1 + e2)
