@@ -8,6 +8,8 @@
 //   load_strings     chemistry/basic/makedeterminants.h:160-185 with
 //                    framework/bit_manipulation.h:215-229 (rightmost char is
 //                    orbital 0) and the sort/unique of :149-168, 381-395
+//   save_wavefunction / load_wavefunction  the one-rank wavefunction file of
+//                    chemistry/tpb/restart.h:20-67 and :414-660
 //   write_rdm_text   the 1pRDM.txt / 2pRDM.txt files and
 //   energy_decomposition  the zero-/one-/two-body energies of the driver
 //                    apps/chemistry_tpb_selected_basis_diagonalization/
@@ -180,6 +182,52 @@ inline EnergyParts energy_decomposition(const Integrals &g, const std::vector<st
           for (int s = 0; s < 4; s++) e.two += 0.5 * v * two[s][x];
         }
   return e;
+}
+
+// Wavefunction file of one rank (file name + "000000"): size_t numbers of
+// alpha strings, beta strings and words per string; the strings' words (the
+// reference's determinant words), alpha then beta; W (alpha-major).
+template <typename ElemT>
+inline void save_wavefunction(const std::string &file, const std::vector<std::vector<size_t>> &adet,
+                              const std::vector<std::vector<size_t>> &bdet, const std::vector<ElemT> &W) {
+  if (W.size() != adet.size() * bdet.size()) throw std::invalid_argument("W does not match the string sets");
+  std::ofstream f(file + "000000", std::ios::binary);
+  size_t head[3] = {adet.size(), bdet.size(), adet.empty() ? 0 : adet[0].size()};
+  f.write(reinterpret_cast<const char *>(head), sizeof head);
+  for (const auto &d : adet) f.write(reinterpret_cast<const char *>(d.data()), sizeof(size_t) * head[2]);
+  for (const auto &d : bdet) f.write(reinterpret_cast<const char *>(d.data()), sizeof(size_t) * head[2]);
+  f.write(reinterpret_cast<const char *>(W.data()), sizeof(ElemT) * W.size());
+  if (!f) throw std::runtime_error("cannot write " + file + "000000");
+}
+
+// W over (adet, bdet) from a wavefunction file: zero, then each stored
+// (alpha, beta) coefficient whose strings both occur in the sets at their
+// position (restart from the intersection of the two determinant sets).
+template <typename ElemT>
+inline void load_wavefunction(const std::string &file, const std::vector<std::vector<size_t>> &adet,
+                              const std::vector<std::vector<size_t>> &bdet, std::vector<ElemT> &W) {
+  std::ifstream f(file + "000000", std::ios::binary);
+  if (!f) throw std::runtime_error("failed to open input file " + file + "000000");
+  size_t head[3];
+  f.read(reinterpret_cast<char *>(head), sizeof head);
+  std::vector<std::vector<size_t>> la(head[0], std::vector<size_t>(head[2])), lb(head[1], std::vector<size_t>(head[2]));
+  for (auto &d : la) f.read(reinterpret_cast<char *>(d.data()), sizeof(size_t) * head[2]);
+  for (auto &d : lb) f.read(reinterpret_cast<char *>(d.data()), sizeof(size_t) * head[2]);
+  std::vector<ElemT> lw(head[0] * head[1]);
+  f.read(reinterpret_cast<char *>(lw.data()), sizeof(ElemT) * lw.size());
+  if (!f) throw std::runtime_error(file + "000000: truncated");
+  std::map<std::vector<size_t>, size_t> ia, ib;
+  for (size_t k = 0; k < adet.size(); k++) ia[adet[k]] = k;
+  for (size_t k = 0; k < bdet.size(); k++) ib[bdet[k]] = k;
+  W.assign(adet.size() * bdet.size(), ElemT(0));
+  for (size_t a = 0; a < head[0]; a++) {
+    const auto pa = ia.find(la[a]);
+    if (pa == ia.end()) continue;
+    for (size_t b = 0; b < head[1]; b++) {
+      const auto pb = ib.find(lb[b]);
+      if (pb != ib.end()) W[pa->second * bdet.size() + pb->second] += lw[a * head[1] + b];
+    }
+  }
 }
 
 }  // namespace sbdx

@@ -466,19 +466,31 @@ def davidson(op: TPBOperator, W=None, max_it=100, nb=12, eps=1e-8, max_time=1e30
 
 
 def diag(fcidump_path, adet_path, bdet_path=None, max_it=100, nb=12, eps=1e-8, do_rdm=True,
-         bit_length=20, rdm_dir=None):
+         bit_length=20, rdm_dir=None, loadname="", savename=""):
     """FCIDUMP + string files -> energy <W|HW>, density and RDMs, like
     sbd::tpb::diag (chemistry/tpb/sbdiag.h:139-763) on one GPU.  With
     rdm_dir, also the driver's 1pRDM.txt / 2pRDM.txt there and its zero-,
     one- and two-body energies in out["energy_parts"]
-    (apps/chemistry_tpb_selected_basis_diagonalization/main.cc:264-318)."""
+    (apps/chemistry_tpb_selected_basis_diagonalization/main.cc:264-318).
+    loadname / savename: start from a saved wavefunction (its coefficients
+    on the determinants both sets share) instead of the HF vector, and save
+    the result, as sbdiag.h:285-291 and :742-746 do."""
     fd = io.read_fcidump(fcidump_path)
     I0, I1, I2 = io.setup_integrals(fd)
     L = fd["norb"]
     a = io.read_strings(adet_path, L)
     b = a if bdet_path is None else io.read_strings(bdet_path, L)
     op = TPBOperator(L, I0, I1, I2, a, b, bit_length)
-    _, W, steps, conv = davidson(op, max_it=max_it, nb=nb, eps=eps)
+    W0 = None
+    if loadname:
+        W0 = io.load_wavefunction(loadname, L, a, b, bit_length)
+        nrm = np.linalg.norm(W0)
+        if nrm == 0.0:
+            raise ValueError(f"{loadname}: no determinant in common with the string sets")
+        W0 /= nrm
+    _, W, steps, conv = davidson(op, W=W0, max_it=max_it, nb=nb, eps=eps)
+    if savename:
+        io.save_wavefunction(savename, L, a, b, W, bit_length)
     E = op.energy(W)
     out = dict(energy=E, W=W, steps=steps, converged=conv, norb=L)
     if do_rdm:

@@ -275,3 +275,55 @@ def energy_decomposition(I0, h, eri, one, two):
     e1 = float(np.einsum("ij,ij->", h, g1))
     e2 = 0.5 * float(np.einsum("ikjl,ijkl->", np.asarray(eri).reshape(L, L, L, L), g2))
     return dict(zero=float(I0), one=e1, two=e2, total=float(I0) + e1 + e2)
+
+
+def _wavefunction_path(file):
+    # one-rank file name: the basename plus the six-digit rank 0
+    return f"{file}000000"
+
+
+def save_wavefunction(file, norb, adet, bdet, W, bit_length=20):
+    """The reference's wavefunction file for one rank
+    (chemistry/tpb/restart.h:20-67, file name + "000000"): size_t number of
+    alpha strings, of beta strings and words per string; the alpha strings'
+    words, the beta strings' words (bit_length bits per word, orbital i in
+    word i/bl, bit i%bl); then W (double, alpha-major)."""
+    aw = masks_to_words(adet, norb, bit_length)
+    bw = masks_to_words(bdet, norb, bit_length)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    if W.size != aw.shape[0] * bw.shape[0]:
+        raise ValueError("W does not match the string sets")
+    with open(_wavefunction_path(file), "wb") as f:
+        np.array([aw.shape[0], bw.shape[0], aw.shape[1]], dtype=np.uint64).tofile(f)
+        aw.tofile(f)
+        bw.tofile(f)
+        W.tofile(f)
+
+
+def load_wavefunction(file, norb, adet, bdet, bit_length=20):
+    """W over the given string sets from a wavefunction file
+    (chemistry/tpb/restart.h:414-660 for one rank): zero, then for every
+    stored (alpha, beta) pair whose strings both occur in the sets the stored
+    coefficient at their position (restart from the intersection of the old
+    and new determinant sets)."""
+    with open(_wavefunction_path(file), "rb") as f:
+        na, nb, nw = (int(v) for v in np.fromfile(f, dtype=np.uint64, count=3))
+        aw = np.fromfile(f, dtype=np.uint64, count=na * nw).reshape(na, nw)
+        bw = np.fromfile(f, dtype=np.uint64, count=nb * nw).reshape(nb, nw)
+        Wl = np.fromfile(f, dtype=np.float64, count=na * nb)
+    if Wl.size != na * nb:
+        raise ValueError(f"{_wavefunction_path(file)}: truncated")
+    cur_a = masks_to_words(adet, norb, bit_length)
+    cur_b = masks_to_words(bdet, norb, bit_length)
+    if cur_a.shape[1] != nw:
+        raise ValueError("word count of the file does not match norb / bit_length")
+    ia = {row.tobytes(): k for k, row in enumerate(cur_a)}
+    ib = {row.tobytes(): k for k, row in enumerate(cur_b)}
+    ja = np.array([ia.get(row.tobytes(), -1) for row in aw], dtype=np.int64)
+    jb = np.array([ib.get(row.tobytes(), -1) for row in bw], dtype=np.int64)
+    W = np.zeros(cur_a.shape[0] * cur_b.shape[0])
+    sa, sb = np.nonzero(ja >= 0)[0], np.nonzero(jb >= 0)[0]
+    if sa.size and sb.size:
+        W.reshape(cur_a.shape[0], cur_b.shape[0])[np.ix_(ja[sa], jb[sb])] += \
+            Wl.reshape(na, nb)[np.ix_(sa, sb)]
+    return W
