@@ -204,8 +204,14 @@ int sbdx_symmetric_eig(int n, const double *A, double *V, double *E);
  * G = H X and y = hii*x + G + G^T, and the mixed term for the upper
  * triangle only, mirrored -- about half the work.  The start vector must be
  * symmetric (the Hartree-Fock vector of BasisInitVector is); the solve
- * then stays in the symmetric sector and returns its lowest state, which is
- * what the reference's Davidson reaches from that start.  Errors with
+ * then stays in the symmetric sector and returns its lowest state.  That is
+ * the ground state only when the ground state is even under the exchange
+ * (closed-shell singlets, e.g. the molecular data sets and the norb-36
+ * workload, where it matches the general solve to 1e-12).  When the ground
+ * state is odd (found for 3 electrons per spin in 64 orbitals), the
+ * reference's Davidson from the Hartree-Fock vector reaches it only through
+ * rounding noise after hundreds of steps, and this mode returns the lowest
+ * even state instead: use the general form there.  Errors with
  * SBDX_ERR_ARG when the string sets differ or the start vector is not
  * symmetric. */
 int sbdx_set_symmetry(sbdx_handle_t h, int on);
