@@ -38,7 +38,7 @@ def test_edge_sigma_and_energy(case, oracle_mod):
     inst = CASES[case]()
     print(f"[edge] {case}", flush=True)
     op = sbdx.TPBOperator.from_instance(inst)
-    orc = oracle_mod.Oracle.from_instance(inst, 4)
+    orc = oracle_mod.Oracle.from_instance(inst, 16)
     v = random_vector(op.ndet, 3)
     r = orc.sigma(v)
     s = op.sigma(v)
@@ -59,7 +59,7 @@ def test_edge_mixed_solve_no_singles(oracle_mod):
     inst = _no_singles()
     print("[edge] mixed no_singles", flush=True)
     op = sbdx.TPBOperator.from_instance(inst)
-    orc = oracle_mod.Oracle.from_instance(inst, 4)
+    orc = oracle_mod.Oracle.from_instance(inst, 16)
     ref = orc.davidson(nb=8, eps=1e-9, max_it=500)
     op.configure("mixed")
     op.set_symmetry(True)
@@ -82,7 +82,7 @@ def test_edge_norb64_exchange_sectors(oracle_mod):
     inst = _norb64()
     print("[edge] norb64 exchange sectors", flush=True)
     op = sbdx.TPBOperator.from_instance(inst)
-    orc = oracle_mod.Oracle.from_instance(inst, 4)
+    orc = oracle_mod.Oracle.from_instance(inst, 16)
     ref = orc.davidson(nb=8, eps=1e-9, max_it=800)
     assert ref["converged"]
     op.configure("mixed")
