@@ -123,6 +123,44 @@ def cpu_baseline(inst, n_steps, seconds, nthreads):
                 sigma_s=t_sigma)
 
 
+def factored_leg(op, v, norb):
+    """One product per factored form of the mixed term (dense (pq|rs) GEMM
+    per batch of beta rows) next to the direct form: device times of the
+    mixed term and of its GEMMs, the GEMMs' rate against the MI355X dense
+    matrix peak of their input type, and the difference to the direct form."""
+    K = norb * norb
+    flops = 2.0 * K * K * op.na * op.nb  # per product, one GEMM pass (logical)
+    peaks = {"fp64_dgemm": 78.6, "fp32_sgemm": 157.3, "fp32_split_f16x3": 2500.0}
+    out = {"flops_per_product": flops}
+    for name, form, fp32 in (("direct_fp64", 0, False), ("direct_fp32", 0, True), ("fp64_dgemm", 1, False),
+                             ("fp32_sgemm", 1, True), ("fp32_split_f16x3", 2, True)):
+        op.set_mixed_form(form)
+        f = op.sigma_fp32 if fp32 else op.sigma
+        f(v)  # untimed: workspace, FP16 halves of the ERI matrix
+        op.profile(True)
+        s = f(v)
+        st = op.profile_stats()
+        op.profile(False)
+        if form == 0:
+            out[name] = {"mixed_ms": round(st["mixed_f32" if fp32 else "mixed"][1], 3)}
+            out["_ref"] = s if not fp32 else out.get("_ref")
+            continue
+        g = st["fac_gemm_f32" if fp32 else "fac_gemm"]
+        mflops = flops * (3 if form == 2 else 1)  # MFMA flops actually issued
+        ref = out["_ref"]
+        out[name] = {
+            "mixed_ms": round(st["mixed_f32" if fp32 else "mixed"][1], 3),
+            "gemm_ms": round(g[1], 3), "gemm_calls": g[0],
+            "gemm_tflops": round(mflops / (g[1] * 1e-3) / 1e12, 1),
+            "mfma_peak_frac": round(mflops / (g[1] * 1e-3) / 1e12 / peaks[name], 3),
+            "max_rel_diff_vs_direct_fp64": float(np.max(np.abs(s - ref)) / np.max(np.abs(ref))),
+        }
+        log(f"factored {name}: {out[name]}")
+    op.set_mixed_form(0)
+    del out["_ref"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +194,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--factored", action="store_true",
+                    help="also time one product per factored (pq|rs)-GEMM form of the mixed term")
     ap.add_argument("--quiet", action="store_true", help="no per-step solver lines on stderr")
     ap.add_argument("--precond", choices=["diag", "mlp"], default="diag",
                     help="preconditioner of the timed solver (mlp: learned, weights below)")
@@ -390,6 +430,9 @@ def main():
         "same_spin_form": op.same_spin_forms(),
         "build_s": round(t_build, 3),
     }
+    if args.factored:
+        out["factored_mixed"] = factored_leg(op, W0 if args.init == "random" else
+                                             np.random.default_rng(2).standard_normal(op.ndet), inst["norb"])
     if not args.no_cpu_baseline:
         ref_steps = plain["steps"] if plain is not None else res["steps"]
         log(f"cpu baseline: ~{args.cpu_seconds:.0f} s oracle sample on {args.cpu_threads} threads")

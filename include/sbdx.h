@@ -187,6 +187,20 @@ int sbdx_fp64_switch_step(sbdx_handle_t h, int *step);
  * (0 when SBDX_SPLIT_GEMM=0|1 in the environment fixed the mode). */
 int sbdx_split_gemm(sbdx_handle_t h, int *mode, float *ms_sgemm, float *ms_split);
 
+/* Form of the mixed alpha-beta term in every later product (no counterpart
+ * in the reference, which evaluates each element directly,
+ * chemistry/tpb/mult.h:625-881): 0 (default) the direct sliced-ELL kernels;
+ * 1 the factored form, a dense (pq|rs) contraction on the matrix cores:
+ * per batch of beta rows D[rs][(r,J)] = sum over the extended beta singles
+ * of r (singles plus the occupied (kk) identity entries) of s_b x[J,j],
+ * G = (pq|rs) D as one DGEMM or SGEMM, y[I,r] += the alpha side over G
+ * minus the double-counted diagonal Coulomb term; 2 the same with the
+ * split-FP16x3 GEMM for FP32 products (FP64 products use the DGEMM).  The
+ * factored form costs 2 K^2 na nb flops per product (K = norb^2) and holds
+ * up to 2 GiB of D and G plus x^T; products in it use the general form
+ * even when the symmetric form is set, and no graph is captured. */
+int sbdx_set_mixed_form(sbdx_handle_t h, int form);
+
 /* The subspace eigensolver of the Davidson solves, on the host: eigenvalues
  * E[n] ascending and orthonormal eigenvectors (columns of V, n x n
  * column-major) of the symmetric n x n column-major A (not modified), by

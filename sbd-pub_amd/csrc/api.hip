@@ -230,6 +230,7 @@ Context *make_sub_context(const Context &c, size_t m) {
   std::vector<uint64_t> bs(c.h_bstr.begin(), c.h_bstr.begin() + m);
   Context *s = create_core(c.L, std::move(as), std::move(bs), c.I0, c.h_h1, c.h_eri);
   s->lds_limit = c.lds_limit;
+  s->mixed_form = c.mixed_form;
   return s;
 }
 
@@ -511,6 +512,15 @@ int sbdx_split_gemm(sbdx_handle_t h, int *mode, float *ms_sgemm, float *ms_split
   if (mode) *mode = c.split_mode;
   if (ms_sgemm) *ms_sgemm = c.split_ms[0];
   if (ms_split) *ms_split = c.split_ms[1];
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_set_mixed_form(sbdx_handle_t h, int form) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (form < 0 || form > 2) throw Error(SBDX_ERR_ARG, "mixed form must be 0, 1 or 2");
+  c.mixed_form = form;
   return SBDX_OK;
   SBDX_GUARD_END
 }

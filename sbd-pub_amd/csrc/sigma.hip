@@ -969,7 +969,11 @@ static int mixed_cfg(Context &c, const T *eri, const T *x) {
 }
 
 template <typename T>
+static void mixed_factored(Context &c, const T *eri, const T *x, T *y);
+
+template <typename T>
 static void sigma_mixed(Context &c, const T *eri, const T *x, T *y) {
+  if (c.mixed_form != 0) return mixed_factored<T>(c, eri, x, y);
   if (c.B.has_ell) {
     // tuned staging width and slots per thread, then row parts
     const int pi = sizeof(T) == 4 ? 1 : 0;
@@ -1025,6 +1029,10 @@ template <typename T>
 static void sigma_mixed_block(Context &c, const T *eri, int k, const T *x, T *y) {
   const size_t N = c.N;
   const int pi = sizeof(T) == 4 ? 1 : 0;
+  if (c.mixed_form != 0) {
+    for (int v = 0; v < k; v++) sigma_mixed<T>(c, eri, x + v * N, y + v * N);
+    return;
+  }
   if (k >= 2 && c.multi_mode[pi] < 0) {
     // first block product of this precision: two vectors through the
     // multi-vector kernel vs two single-vector launches, into a scratch pair
@@ -1300,7 +1308,7 @@ static void sym_same_split(Context &c, const float *hii, const float *x, float *
   const int n = (int)c.na;
   const size_t nn = (size_t)n * n;
   if (!c.Hh) {
-    c.split_part = (float *)dalloc(c, sizeof(float) * kSplitParts);
+    if (!c.split_part) c.split_part = (float *)dalloc(c, sizeof(float) * kSplitParts);
     c.split_inv = (float *)dalloc(c, sizeof(float) * 2);
     c.Hh = dalloc(c, 2 * nn);
     c.Hl = dalloc(c, 2 * nn);
@@ -1370,6 +1378,170 @@ static void sym_same(Context &c, const SigmaOperands<float> &o, const float *x, 
     sym_same_gemm<float>(c, o, x, y);
 }
 
+// ---- mixed alpha-beta term as a dense (pq|rs) contraction on the matrix cores
+// The extended singles of a string are its singles (ket, pq, sign) plus, for
+// each occupied orbital k, the identity entry (itself, kk, +1).  Over them the
+// mixed term with both spectator dressings is
+//   y[I,r] += sum_{(J,pq,sa) in ext_a(I)} sum_{(j,rs,sb) in ext_b(r)} sa sb (pq|rs) x[J,j]
+//             - x[I,r] sum_{k in alpha(I)} sum_{k' in beta(r)} (kk|k'k')
+// (identity-identity pairs add the diagonal alpha-beta Coulomb term, which
+// hii already holds; the correction removes it).  For a batch of B beta bra
+// rows r0.. and column index col = (r - r0) na + J (M = B na columns):
+//   D[rs][col] = sum_{(j,rs,sb) in ext_b(r)} sb x[J,j]    K x M, mostly zero
+//   G = E D with E[pq][rs] = (pq|rs)                      2 K^2 M flops, one GEMM
+//   y[I,r] += sum_{(J,pq,sa) in ext_a(I)} sa G[pq][col(r,J)] - correction
+// Per vector that is 2 K^2 na nb flops whatever the sparsity of D.
+constexpr size_t kFacBytes = size_t(1) << 30;  // D and G, each (FP64 sizing)
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fac_zero(size_t n, T *__restrict__ v) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) v[i] = T(0);
+}
+
+// grid (ceil(na/256), B): thread J of batch row b writes column (b, J) of D.
+// Within one bra r the singles' rs are distinct and off the (kk) diagonal, so
+// every element is written at most once.  xT = x^T (nb x na).
+template <typename T>
+__global__ void __launch_bounds__(256) k_fac_build(int na, int L, int r0, size_t M,
+                                                   const uint64_t *__restrict__ b_off,
+                                                   const uint64_t *__restrict__ b_ent,
+                                                   const uint64_t *__restrict__ b_str,
+                                                   const T *__restrict__ xT, T *__restrict__ D) {
+  const int J = blockIdx.x * 256 + threadIdx.x;
+  if (J >= na) return;
+  const int b = blockIdx.y, r = r0 + b;
+  const size_t col = (size_t)b * na + J;
+  for (uint64_t f = b_off[r]; f < b_off[r + 1]; f++) {
+    const uint64_t be = b_ent[f];
+    const T v = xT[(size_t)ent_ket(be) * na + J];
+    D[(size_t)ent_pq(be) * M + col] = ent_neg(be) ? -v : v;
+  }
+  const T xr = xT[(size_t)r * na + J];
+  for (uint64_t occ = b_str[r]; occ; occ &= occ - 1) D[(size_t)(__builtin_ctzll(occ) * (L + 1)) * M + col] = xr;
+}
+
+// thread (I, b), b fastest: y[I, r0+b] += the alpha side over G.  gscale
+// (device, may be null): G holds 2^(ex+eh) times the product.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fac_apply(int na, int nb, int L, int r0, int B, size_t M,
+                                                   const uint64_t *__restrict__ a_off,
+                                                   const uint64_t *__restrict__ a_ent,
+                                                   const uint64_t *__restrict__ a_str,
+                                                   const double *__restrict__ jsum_b,
+                                                   const T *__restrict__ G, const float *__restrict__ gscale,
+                                                   const T *__restrict__ x, T *__restrict__ y) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (size_t)na * B) return;
+  const int b = (int)(t % B), I = (int)(t / B), r = r0 + b;
+  const size_t cb = (size_t)b * na;
+  T g = T(0);
+  for (uint64_t e = a_off[I]; e < a_off[I + 1]; e++) {
+    const uint64_t ae = a_ent[e];
+    const T v = G[(size_t)ent_pq(ae) * M + cb + ent_ket(ae)];
+    g += ent_neg(ae) ? -v : v;
+  }
+  double corr = 0.0;
+  for (uint64_t occ = a_str[I]; occ; occ &= occ - 1) {
+    const int k = __builtin_ctzll(occ);
+    g += G[(size_t)(k * (L + 1)) * M + cb + I];
+    corr += jsum_b[(size_t)r * L + k];  // sum_{k' in beta(r)} (k'k'|kk)
+  }
+  if (gscale) g *= (T)(gscale[0] * gscale[1]);
+  const size_t o = (size_t)I * nb + r;
+  y[o] += g - x[o] * (T)corr;
+}
+
+// C = C*beta + A B, m x n x k column-major FP16 operands, FP32 accumulation and output.
+static rocblas_status gemm_f16(rocblas_handle h, int m, int n, int k, const void *A, int lda, const void *B,
+                               int ldb, float *C, int ldc, float beta) {
+  const float one = 1.0f;
+  return rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A,
+                         rocblas_datatype_f16_r, lda, B, rocblas_datatype_f16_r, ldb, &beta, C,
+                         rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc, rocblas_datatype_f32_r,
+                         rocblas_gemm_algo_standard, 0, 0);
+}
+
+static void fac_workspace(Context &c) {
+  if (c.fac_D) return;
+  const size_t row = sizeof(double) * (size_t)c.K * c.na;  // one beta row of D
+  c.fac_cap = std::max(row, std::min(kFacBytes, row * c.nb));
+  c.fac_D = dalloc(c, c.fac_cap);
+  c.fac_G = dalloc(c, c.fac_cap);
+  c.fac_xT = dalloc(c, sizeof(double) * c.N);
+}
+
+// G = E D for the batch: the product's own precision (DGEMM / SGEMM), or for
+// FP32 with form 2 the split-FP16x3 GEMM (D and E as 2^-e (hi + lo), see
+// k_split_f16) whose scales are returned for the apply kernel.
+static const float *fac_gemm(Context &c, int M, const double *eri, const double *D, double *G) {
+  if (gemm(blas(c), M, c.K, c.K, D, M, eri, c.K, G, M, 0.0) != rocblas_status_success)
+    throw Error(3, "factored mixed-term DGEMM failed");
+  return nullptr;
+}
+static const float *fac_gemm(Context &c, int M, const float *eri, const float *D, float *G) {
+  rocblas_handle hb = blas(c);
+  if (c.mixed_form != 2) {
+    if (gemm(hb, M, c.K, c.K, D, M, eri, c.K, G, M, 0.f) != rocblas_status_success)
+      throw Error(3, "factored mixed-term SGEMM failed");
+    return nullptr;
+  }
+  const size_t KK = (size_t)c.K * c.K, n = (size_t)M * c.K;
+  if (!c.fac_Eh) {
+    if (!c.split_part) c.split_part = (float *)dalloc(c, sizeof(float) * kSplitParts);
+    c.fac_inv = (float *)dalloc(c, sizeof(float) * 2);
+    c.fac_Eh = dalloc(c, 2 * KK);
+    c.fac_El = dalloc(c, 2 * KK);
+    c.fac_Dh = dalloc(c, c.fac_cap / 2);  // FP16 halves of an FP32 D of at most fac_cap/2 bytes
+    c.fac_Dl = dalloc(c, c.fac_cap / 2);
+    split_f16(c, KK, eri, c.fac_Eh, c.fac_El, c.fac_inv + 1);
+  }
+  split_f16(c, n, D, c.fac_Dh, c.fac_Dl, c.fac_inv);
+  const int K = c.K;
+  if (gemm_f16(hb, M, K, K, c.fac_Dh, M, c.fac_Eh, K, G, M, 0.f) != rocblas_status_success ||
+      gemm_f16(hb, M, K, K, c.fac_Dh, M, c.fac_El, K, G, M, 1.f) != rocblas_status_success ||
+      gemm_f16(hb, M, K, K, c.fac_Dl, M, c.fac_Eh, K, G, M, 1.f) != rocblas_status_success)
+    throw Error(3, "factored mixed-term split-FP16 GEMM failed");
+  return c.fac_inv;
+}
+
+template <typename T>
+static void mixed_factored(Context &c, const T *eri, const T *x, T *y) {
+  fac_workspace(c);
+  const int na = (int)c.na, nb = (int)c.nb, K = c.K;
+  const size_t row = (size_t)K * na;
+  // beta rows per batch: as many as fit in the workspace, at most 32768
+  // (a grid dimension), or SBDX_FACTORED_ROWS when set
+  size_t rows = std::min<size_t>({(size_t)nb, c.fac_cap / (sizeof(T) * row), 32768});
+  if (const char *env = std::getenv("SBDX_FACTORED_ROWS")) {
+    const long v = std::atol(env);
+    if (v < 1) throw Error(1, "SBDX_FACTORED_ROWS must be a positive integer");
+    rows = std::min<size_t>(rows, (size_t)v);
+  }
+  const int B = (int)std::max<size_t>(1, rows);
+  if ((size_t)B * na > (size_t)INT32_MAX) throw Error(3, "factored mixed term: batch too wide for the GEMM");
+  T *xT = (T *)c.fac_xT, *D = (T *)c.fac_D, *G = (T *)c.fac_G;
+  transpose<T>(c, x, xT, na, nb, false);
+  const bool f32 = sizeof(T) == 4;
+  for (int r0 = 0; r0 < nb; r0 += B) {
+    const int Bc = std::min(B, nb - r0);
+    const size_t M = (size_t)Bc * na, n = M * K;
+    hipLaunchKernelGGL((k_fac_zero<T>), dim3((unsigned)std::min<size_t>(cdiv(n, 256), 65536)), dim3(256), 0,
+                       c.stream, n, D);
+    check_launch("k_fac_zero");
+    hipLaunchKernelGGL((k_fac_build<T>), dim3(cdiv(na, 256), (unsigned)Bc), dim3(256), 0, c.stream, na, c.L, r0,
+                       M, c.B.s_off, c.B.s_ent, c.B.str, (const T *)xT, D);
+    check_launch("k_fac_build");
+    const float *gs;
+    {
+      ProfScope ps(c, f32 ? "fac_gemm_f32" : "fac_gemm");
+      gs = fac_gemm(c, (int)M, eri, D, G);
+    }
+    hipLaunchKernelGGL((k_fac_apply<T>), dim3(cdiv(M, 256)), dim3(256), 0, c.stream, na, nb, c.L, r0, Bc, M,
+                       c.A.s_off, c.A.s_ent, c.A.str, (const double *)c.B.jsum, (const T *)G, gs, x, y);
+    check_launch("k_fac_apply");
+  }
+}
+
 // Whether the symmetric form applies: equal string sets, the dense blocks
 // fit (same-spin autotuner chose or can hold them), the sliced-ELL table.
 static bool sym_ok(Context &c) {
@@ -1387,9 +1559,14 @@ static bool sym_ok(Context &c) {
 // such x (H commutes with the transpose when the string sets are equal).
 template <typename T>
 static void sigma_sym_impl(Context &c, const T *x, T *y) {
+  if (c.mixed_form != 0) {
+    // the factored mixed term has no triangle form: the general product,
+    // equal for symmetric x
+    sigma_impl<T>(c, x, y);
+    return;
+  }
   const SigmaOperands<T> o = operands(c, (T *)nullptr);
   const bool f32 = sizeof(T) == 4;
-  const int n = (int)c.na;
   {
     ProfScope ps(c, f32 ? "sym_same_f32" : "sym_same");
     sym_same(c, o, x, y);
@@ -1429,7 +1606,7 @@ void sigma_sym(Context &c, const double *x, double *y) { sigma_sym_impl<double>(
 // eagerly (autotuning, dense blocks and rocBLAS workspace are settled then)
 // and captures the same sequence for later calls.
 static void sym_core_f32(Context &c) {
-  if (!c.use_graphs || sizeof(float) * ((size_t)c.nb + c.K + 1) > c.lds_limit) {
+  if (!c.use_graphs || c.mixed_form != 0 || sizeof(float) * ((size_t)c.nb + c.K + 1) > c.lds_limit) {
     sigma_sym_impl<float>(c, c.f32.x, c.f32.y);
     return;
   }

@@ -44,7 +44,7 @@ ABI_SYMBOLS = [
     "sbdx_mlp_forward", "sbdx_mlp_gradient", "sbdx_mlp_train", "sbdx_mlp_record", "sbdx_mlp_collect",
     "sbdx_set_preconditioner", "sbdx_precond_stats", "sbdx_set_verbose",
     "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
-    "sbdx_mixed_staging", "sbdx_split_gemm", "sbdx_fp64_switch_step", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry",
+    "sbdx_mixed_staging", "sbdx_split_gemm", "sbdx_set_mixed_form", "sbdx_fp64_switch_step", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry",
     "sbdx_sigma_sym", "sbdx_mlp_apply", "sbdx_set_coarse_start", "sbdx_coarse_stats",
     "sbdx_set_graphs",
 ]
@@ -111,6 +111,7 @@ def lib():
                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]
     _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.sbdx_fp64_switch_step.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.sbdx_set_mixed_form.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_split_gemm.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.sbdx_mixed_staging.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_float), C.POINTER(C.c_int)]
@@ -279,6 +280,13 @@ class TPBOperator:
         """Refine the start vector on the first m strings of each set first
         (thick-restart solves); m = 0 switches it off."""
         _check(lib().sbdx_set_coarse_start(self.h, int(m), float(eps)))
+
+    def set_mixed_form(self, form):
+        """Mixed alpha-beta term of later products: 0 direct sliced-ELL
+        kernels (default), 1 the dense (pq|rs) contraction as one GEMM per
+        batch of beta rows, 2 the same with the split-FP16x3 GEMM for FP32
+        products."""
+        _check(lib().sbdx_set_mixed_form(self.h, int(form)))
 
     def fp64_switch_step(self):
         """Step of the last thick solve from which its products were FP64
