@@ -140,11 +140,12 @@ struct Context {
   // Mixed alpha-beta term: 0 direct sliced-ELL kernels; 1 factored dense
   // (pq|rs) GEMM in the product's precision; 2 factored with the
   // split-FP16x3 GEMM for FP32 products.  Workspace of the factored form
-  // (built on first use): D and G (fac_cap bytes each), x^T, FP16 halves of
-  // D and of the ERI matrix, and their two inverse scales (D's, ERI's).
+  // (built on first use): D and G (fac_cap bytes each), x^T, the FP16
+  // operands [Dh | Dh | Dl] and [Eh | El | Eh] of the split GEMM, and their
+  // two inverse scales (D's, ERI's).
   int mixed_form = 0;
   void *fac_D = nullptr, *fac_G = nullptr, *fac_xT = nullptr;
-  void *fac_Dh = nullptr, *fac_Dl = nullptr, *fac_Eh = nullptr, *fac_El = nullptr;
+  void *fac_D3 = nullptr, *fac_E3 = nullptr;
   float *fac_inv = nullptr;
   size_t fac_cap = 0;
   int mixed_eb[4] = {0, 0, 0, 0};

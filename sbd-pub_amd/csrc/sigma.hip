@@ -691,11 +691,12 @@ __global__ void __launch_bounds__(256) k_absmax_part(size_t n, const float *__re
 
 // hi/lo halves of v with the scale taken from the np block maxima; block 0
 // stores 2^-e at *inv.  e is capped at 100 (vectors below 2^-85 in max
-// norm keep the absolute floor 2^-125).
+// norm keep the absolute floor 2^-125).  hi2 (may be null) receives a second
+// copy of hi.
 __global__ void __launch_bounds__(256) k_split_f16(size_t n, const float *__restrict__ v,
                                                    const float *__restrict__ part, int np,
                                                    _Float16 *__restrict__ hi, _Float16 *__restrict__ lo,
-                                                   float *__restrict__ inv) {
+                                                   _Float16 *__restrict__ hi2, float *__restrict__ inv) {
   float m = 0.f;
   for (int i = threadIdx.x; i < np; i += 256) m = fmaxf(m, part[i]);
   m = block_max256(m);
@@ -709,6 +710,7 @@ __global__ void __launch_bounds__(256) k_split_f16(size_t n, const float *__rest
     const _Float16 h = (_Float16)t;
     hi[i] = h;
     lo[i] = (_Float16)(t - (float)h);
+    if (hi2) hi2[i] = h;
   }
 }
 
@@ -1293,12 +1295,13 @@ static rocblas_status gemm_f16(rocblas_handle h, int n, const void *A, const voi
                          rocblas_gemm_algo_standard, 0, 0);
 }
 
-static void split_f16(Context &c, size_t n, const float *v, void *hi, void *lo, float *inv) {
+static void split_f16(Context &c, size_t n, const float *v, void *hi, void *lo, float *inv,
+                      void *hi2 = nullptr) {
   const unsigned g = (unsigned)std::min<size_t>(cdiv(n, 256), 8192);
   hipLaunchKernelGGL(k_absmax_part, dim3(kSplitParts), dim3(256), 0, c.stream, n, v, c.split_part);
   check_launch("k_absmax_part");
   hipLaunchKernelGGL(k_split_f16, dim3(g), dim3(256), 0, c.stream, n, v, (const float *)c.split_part,
-                     kSplitParts, (_Float16 *)hi, (_Float16 *)lo, inv);
+                     kSplitParts, (_Float16 *)hi, (_Float16 *)lo, (_Float16 *)hi2, inv);
   check_launch("k_split_f16");
 }
 
@@ -1451,16 +1454,6 @@ __global__ void __launch_bounds__(256) k_fac_apply(int na, int nb, int L, int r0
   y[o] += g - x[o] * (T)corr;
 }
 
-// C = C*beta + A B, m x n x k column-major FP16 operands, FP32 accumulation and output.
-static rocblas_status gemm_f16(rocblas_handle h, int m, int n, int k, const void *A, int lda, const void *B,
-                               int ldb, float *C, int ldc, float beta) {
-  const float one = 1.0f;
-  return rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A,
-                         rocblas_datatype_f16_r, lda, B, rocblas_datatype_f16_r, ldb, &beta, C,
-                         rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc, rocblas_datatype_f32_r,
-                         rocblas_gemm_algo_standard, 0, 0);
-}
-
 static void fac_workspace(Context &c) {
   if (c.fac_D) return;
   const size_t row = sizeof(double) * (size_t)c.K * c.na;  // one beta row of D
@@ -1472,7 +1465,12 @@ static void fac_workspace(Context &c) {
 
 // G = E D for the batch: the product's own precision (DGEMM / SGEMM), or for
 // FP32 with form 2 the split-FP16x3 GEMM (D and E as 2^-e (hi + lo), see
-// k_split_f16) whose scales are returned for the apply kernel.
+// k_split_f16) whose scales are returned for the apply kernel.  The three
+// FP16 products run as one GEMM over k = 3K,
+//   G = [Dh | Dh | Dl] [Eh; El; Eh],
+// so they accumulate in the matrix cores' FP32 accumulators and G is
+// written once.  The ERI operand is used transposed: its three K x K blocks
+// are stored one after another (Eh, El and Eh are symmetric).
 static const float *fac_gemm(Context &c, int M, const double *eri, const double *D, double *G) {
   if (gemm(blas(c), M, c.K, c.K, D, M, eri, c.K, G, M, 0.0) != rocblas_status_success)
     throw Error(3, "factored mixed-term DGEMM failed");
@@ -1486,20 +1484,23 @@ static const float *fac_gemm(Context &c, int M, const float *eri, const float *D
     return nullptr;
   }
   const size_t KK = (size_t)c.K * c.K, n = (size_t)M * c.K;
-  if (!c.fac_Eh) {
+  if (!c.fac_E3) {
     if (!c.split_part) c.split_part = (float *)dalloc(c, sizeof(float) * kSplitParts);
     c.fac_inv = (float *)dalloc(c, sizeof(float) * 2);
-    c.fac_Eh = dalloc(c, 2 * KK);
-    c.fac_El = dalloc(c, 2 * KK);
-    c.fac_Dh = dalloc(c, c.fac_cap / 2);  // FP16 halves of an FP32 D of at most fac_cap/2 bytes
-    c.fac_Dl = dalloc(c, c.fac_cap / 2);
-    split_f16(c, KK, eri, c.fac_Eh, c.fac_El, c.fac_inv + 1);
+    _Float16 *E3 = (_Float16 *)dalloc(c, 2 * 3 * KK);
+    c.fac_E3 = E3;
+    // three FP16 copies of an FP32 D of at most fac_cap/4 elements
+    c.fac_D3 = dalloc(c, 3 * 2 * (c.fac_cap / 4));
+    split_f16(c, KK, eri, E3, E3 + KK, c.fac_inv + 1, E3 + 2 * KK);
   }
-  split_f16(c, n, D, c.fac_Dh, c.fac_Dl, c.fac_inv);
+  _Float16 *D3 = (_Float16 *)c.fac_D3;
+  split_f16(c, n, D, D3, D3 + 2 * n, c.fac_inv, D3 + n);
   const int K = c.K;
-  if (gemm_f16(hb, M, K, K, c.fac_Dh, M, c.fac_Eh, K, G, M, 0.f) != rocblas_status_success ||
-      gemm_f16(hb, M, K, K, c.fac_Dh, M, c.fac_El, K, G, M, 1.f) != rocblas_status_success ||
-      gemm_f16(hb, M, K, K, c.fac_Dl, M, c.fac_Eh, K, G, M, 1.f) != rocblas_status_success)
+  const float one = 1.0f, zero = 0.0f;
+  if (rocblas_gemm_ex(hb, rocblas_operation_none, rocblas_operation_transpose, M, K, 3 * K, &one, D3,
+                      rocblas_datatype_f16_r, M, c.fac_E3, rocblas_datatype_f16_r, K, &zero, G,
+                      rocblas_datatype_f32_r, M, G, rocblas_datatype_f32_r, M, rocblas_datatype_f32_r,
+                      rocblas_gemm_algo_standard, 0, 0) != rocblas_status_success)
     throw Error(3, "factored mixed-term split-FP16 GEMM failed");
   return c.fac_inv;
 }
