@@ -49,7 +49,9 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return r;
 }
 
-// part[b*nv + k] = sum over this block's slice of X_k . y
+// part[b*nv + k] = sum over this block's slice of X_k . y.  The vectors go
+// in groups of 8 sharing one read of y (y is read ceil(nv/8) times, each X_k
+// once); every thread sums its elements in index order, then a fixed tree.
 // gate (may be null): the kernel does nothing when *gate == 0 (the
 // conditional second Gram-Schmidt pass, decided on the device).
 __global__ void __launch_bounds__(kRedThreads) k_dots(VecList X, int nv, const double *__restrict__ y,
@@ -59,12 +61,25 @@ __global__ void __launch_bounds__(kRedThreads) k_dots(VecList X, int nv, const d
   __shared__ double sh[kRedThreads];
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (int k = 0; k < nv; k++) {
-    const double *xk = X.p[k];
-    double s = 0.0;
-    for (size_t i = i0; i < n; i += stride) s += xk[i] * y[i];
-    const double r = block_sum(s, sh);
-    if (threadIdx.x == 0) part[(size_t)blockIdx.x * nv + k] = r;
+  for (int k0 = 0; k0 < nv; k0 += 8) {
+    const int m = nv - k0 < 8 ? nv - k0 : 8;
+    const double *xk[8];
+    double s[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      xk[j] = X.p[j < m ? k0 + j : k0];
+      s[j] = 0.0;
+    }
+    for (size_t i = i0; i < n; i += stride) {
+      const double yi = y[i];
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (j < m) s[j] += xk[j][i] * yi;
+    }
+    for (int j = 0; j < m; j++) {
+      const double r = block_sum(s[j], sh);
+      if (threadIdx.x == 0) part[(size_t)blockIdx.x * nv + k0 + j] = r;
+    }
   }
 }
 
@@ -211,11 +226,24 @@ __global__ void __launch_bounds__(kRedThreads) k_ritz(VecList Cv, VecList HCv, i
   const double E0 = E[0];
   double sw = 0.0, sr = 0.0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    double w = Cv.p[0][i] * U[0];
-    double r = HCv.p[0][i] * (-U[0]);
-    for (int k = 1; k < nv; k++) {
-      w += Cv.p[k][i] * U[k];
-      r += HCv.p[k][i] * (-U[k]);
+    // vectors in groups of 8 whose loads issue together (index order of
+    // the sums kept); lanes j >= m reload vector k0 and are not summed
+    double w = 0.0, r = 0.0;
+    for (int k0 = 0; k0 < nv; k0 += 8) {
+      const int m = nv - k0 < 8 ? nv - k0 : 8;
+      double cv[8], hv[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int k = j < m ? k0 + j : k0;
+        cv[j] = Cv.p[k][i];
+        hv[j] = HCv.p[k][i];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (j < m) {
+          w += cv[j] * U[k0 + j];
+          r += hv[j] * (-U[k0 + j]);
+        }
     }
     r += E0 * w;
     W[i] = w;
@@ -347,7 +375,15 @@ __global__ void __launch_bounds__(kRedThreads) k_sub_combo(VecList X, int nv, co
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     double acc = t[i];
-    for (int k = 0; k < nv; k++) acc -= cf[k] * X.p[k][i];
+    for (int k0 = 0; k0 < nv; k0 += 8) {  // loads of 8 vectors issue together
+      const int m = nv - k0 < 8 ? nv - k0 : 8;
+      double v[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) v[j] = X.p[j < m ? k0 + j : k0][i];
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (j < m) acc -= cf[k0 + j] * v[j];
+    }
     t[i] = acc;
   }
 }
@@ -385,11 +421,18 @@ __global__ void __launch_bounds__(kRedThreads) k_combine(VecList in, int nin, co
     double acc[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) acc[j] = 0.0;
-    for (int k = 0; k < nin; k++) {
-      const double v = in.p[k][i];
+    for (int k0 = 0; k0 < nin; k0 += 8) {  // loads of 8 inputs issue together
+      const int m = nin - k0 < 8 ? nin - k0 : 8;
+      double v[8];
 #pragma unroll
-      for (int j = 0; j < 8; j++)
-        if (j < nout) acc[j] += U[k + (size_t)ld * (j0 + j)] * v;
+      for (int q = 0; q < 8; q++) v[q] = in.p[q < m ? k0 + q : k0][i];
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (q < m) {
+#pragma unroll
+          for (int j = 0; j < 8; j++)
+            if (j < nout) acc[j] += U[k0 + q + (size_t)ld * (j0 + j)] * v[q];
+        }
     }
 #pragma unroll
     for (int j = 0; j < 8; j++)
