@@ -130,8 +130,10 @@ struct Context {
   // tuned time (ms).
   // split-FP16x3 GEMM of the symmetric FP32 same-spin term: mode (-1 not
   // decided, 0 FP32 SGEMM, 1 split), the deciding timings (ms: SGEMM,
-  // split), FP16 hi/lo halves of H_alpha and of the current x, per-block
-  // maxima, and the two inverse power-of-two scales (x's, H's)
+  // split), the FP16 operands (Hh: [Hh; Hl; Hh] of H_alpha stacked per
+  // column, 3n x n; Xh: [Xh | Xh | Xl] of the current x, n x 3n; Hl, Xl
+  // unused), per-block maxima, and the two inverse power-of-two scales
+  // (x's, H's)
   int fp64_switch_step = -1;  // last thick solve: step from which products were FP64 (-1 never)
   int split_mode = -1;
   float split_ms[2] = {0.f, 0.f};

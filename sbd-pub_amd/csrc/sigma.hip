@@ -1286,13 +1286,15 @@ static void sigma_impl(Context &c, const T *x, T *y) {
 
 void sigma_alpha_first(Context &c, const double *x, double *y) { sigma_impl<double>(c, x, y); }
 
-// C = C*beta + A B for n x n FP16 operands, FP32 accumulation and output.
-static rocblas_status gemm_f16(rocblas_handle h, int n, const void *A, const void *B, float *C, float beta) {
-  const float one = 1.0f;
-  return rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, A,
-                         rocblas_datatype_f16_r, n, B, rocblas_datatype_f16_r, n, &beta, C,
-                         rocblas_datatype_f32_r, n, C, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
-                         rocblas_gemm_algo_standard, 0, 0);
+// out (3n x n, column-major): column j = [hi_j; lo_j; hi_j] for the n x n
+// column-major FP16 matrices hi, lo.
+__global__ void __launch_bounds__(256) k_stack3(int n, const _Float16 *__restrict__ hi,
+                                                const _Float16 *__restrict__ lo, _Float16 *__restrict__ out) {
+  const size_t n3 = (size_t)3 * n, m = n3 * n;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < m; idx += (size_t)gridDim.x * 256) {
+    const size_t j = idx / n3, t3 = idx % n3, b = t3 / n, t = t3 % n;
+    out[idx] = (b == 1 ? lo : hi)[t + (size_t)n * j];
+  }
 }
 
 static void split_f16(Context &c, size_t n, const float *v, void *hi, void *lo, float *inv,
@@ -1306,26 +1308,36 @@ static void split_f16(Context &c, size_t n, const float *v, void *hi, void *lo, 
 }
 
 // Symmetric FP32 same-spin term with the split-FP16x3 GEMM:
-// y = hii*x + (G + G^T), G = X H_alpha from three FP16 MFMA GEMMs.
+// y = hii*x + (G + G^T), G = X H_alpha as one FP16 MFMA GEMM over k = 3n,
+//   G = [Xh | Xh | Xl] [Hh; Hl; Hh]
+// (the three products accumulate in the matrix cores' FP32 accumulators and
+// G is written once).  c.Hh holds the stacked H operand (3n x n), built
+// once; c.Xh the three x blocks (n x 3n).
 static void sym_same_split(Context &c, const float *hii, const float *x, float *G, float *y) {
   const int n = (int)c.na;
   const size_t nn = (size_t)n * n;
   if (!c.Hh) {
     if (!c.split_part) c.split_part = (float *)dalloc(c, sizeof(float) * kSplitParts);
     c.split_inv = (float *)dalloc(c, sizeof(float) * 2);
-    c.Hh = dalloc(c, 2 * nn);
-    c.Hl = dalloc(c, 2 * nn);
-    c.Xh = dalloc(c, 2 * nn);
-    c.Xl = dalloc(c, 2 * nn);
+    c.Hh = dalloc(c, 2 * 3 * nn);
+    c.Xh = dalloc(c, 2 * 3 * nn);
     float *Ha, *Hb;
     dense_blocks(c, Ha, Hb);
-    split_f16(c, nn, Ha, c.Hh, c.Hl, c.split_inv + 1);
+    TmpBuf hl(2 * 2 * nn);
+    _Float16 *h = (_Float16 *)hl.p;
+    split_f16(c, nn, Ha, h, h + nn, c.split_inv + 1);
+    hipLaunchKernelGGL(k_stack3, dim3((unsigned)std::min<size_t>(cdiv(3 * nn, 256), 65536)), dim3(256), 0,
+                       c.stream, n, (const _Float16 *)h, (const _Float16 *)(h + nn), (_Float16 *)c.Hh);
+    check_launch("k_stack3");
+    SBDX_HIP(hipStreamSynchronize(c.stream));  // before the halves are freed
   }
-  split_f16(c, nn, x, c.Xh, c.Xl, c.split_inv);
-  rocblas_handle hb = blas(c);
-  if (gemm_f16(hb, n, c.Xh, c.Hh, G, 0.f) != rocblas_status_success ||
-      gemm_f16(hb, n, c.Xh, c.Hl, G, 1.f) != rocblas_status_success ||
-      gemm_f16(hb, n, c.Xl, c.Hh, G, 1.f) != rocblas_status_success)
+  _Float16 *X3 = (_Float16 *)c.Xh;
+  split_f16(c, nn, x, X3, X3 + 2 * nn, c.split_inv, X3 + nn);
+  const float one = 1.0f, zero = 0.0f;
+  if (rocblas_gemm_ex(blas(c), rocblas_operation_none, rocblas_operation_none, n, n, 3 * n, &one, X3,
+                      rocblas_datatype_f16_r, n, c.Hh, rocblas_datatype_f16_r, 3 * n, &zero, G,
+                      rocblas_datatype_f32_r, n, G, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
+                      rocblas_gemm_algo_standard, 0, 0) != rocblas_status_success)
     throw Error(3, "split-FP16 same-spin GEMM failed");
   hipLaunchKernelGGL((k_sym_combine<float>), dim3(cdiv(n, 64), cdiv(n, 64)), dim3(256), 0, c.stream, n, hii, x,
                      (const float *)G, y, (const float *)c.split_inv);
