@@ -181,8 +181,9 @@ int sbdx_fp64_switch_step(sbdx_handle_t h, int *step);
 /* Form of the GEMM in the spin-flip symmetric FP32 same-spin term (no
  * counterpart in the reference): *mode -1 before the first such product, 0
  * FP32 SGEMM, 1 split-FP16x3 (x and H_alpha each written as 2^-e (hi + lo)
- * in FP16 with a power-of-two scale, about 22 significant bits; three FP16
- * MFMA GEMMs with FP32 accumulation, the lo x lo product dropped). Decided
+ * in FP16 with a power-of-two scale, about 22 significant bits; the three
+ * FP16 products as one MFMA GEMM over k = 3n with FP32 accumulation, the
+ * lo x lo product dropped). Decided
  * on the first product by timing both; *ms_sgemm, *ms_split those timings
  * (0 when SBDX_SPLIT_GEMM=0|1 in the environment fixed the mode). */
 int sbdx_split_gemm(sbdx_handle_t h, int *mode, float *ms_sgemm, float *ms_split);
