@@ -256,6 +256,7 @@ def main():
         if solver in ("thick", "mixed"):
             op.set_restart(args.keep)
             op.set_symmetry(args.symmetry and same_sets)
+        if solver in ("thick", "mixed", "block"):
             op.set_coarse_start(args.coarse if args.coarse < min(na, nb) else 0, args.coarse_eps)
         op.set_preconditioner(1 if precond == "mlp" else 0)
         op.set_verbose(0 if args.quiet else 1)
@@ -281,8 +282,8 @@ def main():
             nsteps.append(n)
             conv.append(ok)
             tr = op.trace()
-            extra = (f" coarse={op.coarse_stats()} fp64-from-step={op.fp64_switch_step()}"
-                     if solver in ("thick", "mixed") else "")
+            extra = (f" coarse={op.coarse_stats()}" if solver in ("thick", "mixed", "block") else "") + \
+                (f" fp64-from-step={op.fp64_switch_step()}" if solver in ("thick", "mixed") else "")
             log(f"{solver}/{precond} solve {k}: E0={E:.12f} |r|={tr[-1, 1]:.2e} sigma-products={n} "
                 f"converged={ok} time={dt:.3f} s precond-steps={op.precond_stats()}{extra}")
         stats = op.profile_stats()
@@ -290,7 +291,7 @@ def main():
         return dict(time=float(np.median(times)), E=energies[-1], steps=int(np.median(nsteps)),
                     converged=all(conv), stats=stats, precond_steps=op.precond_stats(), roots=roots,
                     fp64_switch_step=op.fp64_switch_step() if solver in ("thick", "mixed") else None,
-                    coarse=op.coarse_stats() if solver in ("thick", "mixed") else None)
+                    coarse=op.coarse_stats() if solver in ("thick", "mixed", "block") else None)
 
     best = args.solver
     plain = run("plain", 1 if best != "plain" else args.steps) if args.compare or best == "plain" else None

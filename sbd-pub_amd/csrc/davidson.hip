@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <memory>
 #include <string>
 
 #include "common.h"
@@ -26,6 +27,7 @@ constexpr int kRedBlocks = 1024;
 // kFp32Floor |E0| a stalled mixed-precision solve switches to FP64 products
 constexpr double kFp32Floor = 1000.0 * 5.9604644775390625e-8;
 constexpr int kRedThreads = 256;
+constexpr int kBlockCoarseIt = 25;  // iteration cap of the block solver's coarse stage
 
 struct OutList {
   double *p[8];
@@ -1102,9 +1104,93 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
     for (int j = 0; j < nv; j++) hR[j] = std::sqrt(hR[j]);
   };
 
-  // initial block: the given vectors, then unit vectors on the lowest
-  // diagonal elements, orthonormalized
-  SBDX_HIP(hipMemcpyAsync(C, W0, sizeof(double) * N * nroots, hipMemcpyDeviceToDevice, c.stream));
+  // coarse start, as in the thick solver: the lowest roots of the problem
+  // on the first coarse_m strings of each set are found by this solver
+  // (same settings, nested levels) to coarse_eps and lifted back as the
+  // start vectors (zero elsewhere; the basis sign depends only on the two
+  // strings).  The sub-problem starts from the given vectors restricted to
+  // it when each keeps at least 1e-3 of its squared norm there, else from
+  // unit vectors on its lowest diagonal elements.  When the sub-problem
+  // cannot start, the given vectors are used as they are.
+  const double *start = W0;
+  std::unique_ptr<TmpBuf> lifted;
+  c.coarse_products = 0;
+  c.coarse_energy = 0.0;
+  c.coarse_seconds = 0.0;
+  const size_t mc = c.coarse_m;
+  if (mc > 0 && mc < c.na && mc < c.nb) {
+    ProfScope ps(c, "coarse_start");
+    const auto tc = std::chrono::steady_clock::now();
+    Context *sub = make_sub_context(c, mc);
+    struct Owner {
+      Context *p;
+      ~Owner() { destroy_context(p); }
+    } owner{sub};
+    sub->precision = c.precision;
+    sub->coarse_m = mc / 4 >= 256 ? mc / 4 : 0;
+    sub->coarse_eps = c.coarse_eps;
+    const size_t Ns = mc * mc;
+    TmpBuf sw0(sizeof(double) * Ns * nroots), sw1(sizeof(double) * Ns * nroots);
+    double *s0 = (double *)sw0.p, *s1 = (double *)sw1.p;
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+    for (int j = 0; j < nroots; j++)
+      SBDX_HIP(hipMemcpy2DAsync(s0 + j * Ns, sizeof(double) * mc, W0 + j * N, sizeof(double) * c.nb,
+                                sizeof(double) * mc, mc, hipMemcpyDeviceToDevice, sub->stream));
+    bool restricted_ok = true;
+    for (int j = 0; j < nroots && restricted_ok; j++) {
+      const double full = device_dot(c, W0 + j * N, W0 + j * N, N);
+      const double part = device_dot(*sub, s0 + j * Ns, s0 + j * Ns, Ns);
+      restricted_ok = part > 1e-3 * full && part > 0.0;
+    }
+    if (!restricted_ok) {
+      std::vector<double> hh(Ns);
+      SBDX_HIP(hipMemcpyAsync(hh.data(), sub->hii, sizeof(double) * Ns, hipMemcpyDeviceToHost, sub->stream));
+      SBDX_HIP(hipStreamSynchronize(sub->stream));
+      std::vector<size_t> idx(Ns);
+      for (size_t i = 0; i < Ns; i++) idx[i] = i;
+      const size_t kk = std::min<size_t>(Ns, (size_t)nroots);
+      std::partial_sort(idx.begin(), idx.begin() + kk, idx.end(),
+                        [&](size_t a, size_t b) { return hh[a] < hh[b] || (hh[a] == hh[b] && a < b); });
+      std::vector<double> h0(Ns * nroots, 0.0);
+      for (size_t k = 0; k < kk; k++) h0[k * Ns + idx[k]] = 1.0;
+      SBDX_HIP(hipMemcpyAsync(s0, h0.data(), sizeof(double) * Ns * nroots, hipMemcpyHostToDevice, sub->stream));
+      SBDX_HIP(hipStreamSynchronize(sub->stream));
+    }
+    std::vector<double> es(nroots);
+    int np = 0;
+    bool ok = true;
+    try {
+      // at most kBlockCoarseIt block iterations: excited roots of the
+      // sub-problem can take hundreds, and partly converged Ritz vectors
+      // already make far better starts than unit vectors
+      (void)davidson_block(*sub, nroots, block, std::min(max_it, kBlockCoarseIt), c.coarse_eps, max_time, mixed,
+                           s0, s1, es.data(), &np);
+    } catch (const Error &) {
+      ok = false;
+    }
+    SBDX_HIP(hipStreamSynchronize(sub->stream));
+    if (ok) {
+      lifted.reset(new TmpBuf(sizeof(double) * N * nroots));
+      double *l = (double *)lifted->p;
+      SBDX_HIP(hipMemsetAsync(l, 0, sizeof(double) * N * nroots, c.stream));
+      for (int j = 0; j < nroots; j++)
+        SBDX_HIP(hipMemcpy2DAsync(l + j * N, sizeof(double) * c.nb, s1 + j * Ns, sizeof(double) * mc,
+                                  sizeof(double) * mc, mc, hipMemcpyDeviceToDevice, c.stream));
+      SBDX_HIP(hipStreamSynchronize(c.stream));
+      start = l;
+      c.coarse_products = np;
+      c.coarse_energy = es[0];
+    }
+    c.coarse_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
+    if (c.verbose) {
+      fprintf(stderr, "[sbdx] block coarse start on %zu x %zu strings: %s, E0 = %.12f, %d products, %.3f s\n", mc,
+              mc, ok ? "used" : "skipped", c.coarse_energy, c.coarse_products, c.coarse_seconds);
+      fflush(stderr);
+    }
+  }
+  // initial block: the given (or coarse-refined) vectors, then unit vectors
+  // on the lowest diagonal elements, orthonormalized
+  SBDX_HIP(hipMemcpyAsync(C, start, sizeof(double) * N * nroots, hipMemcpyDeviceToDevice, c.stream));
   if (block > nroots) {
     std::vector<double> hh(N);
     SBDX_HIP(hipMemcpyAsync(hh.data(), c.hii, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
