@@ -242,8 +242,13 @@ int sbdx_sigma_sym(sbdx_handle_t h, const double *Wk, double *Wb, int fp32);
  * residual is below eps, then lifted back (zero on the other determinants)
  * as the full solve's start.  The full solve still converges to eps on
  * exact FP64 products.  Levels nest: the sub-problem itself starts from its
- * first m/4 strings while m/4 >= 256.  m = 0 (default) switches it off.  Statistics of the
- * last solve's coarse stage: sub-problem products, energy, wall seconds. */
+ * first m/4 strings while m/4 >= 256.  m = 0 (default) switches it off.
+ * sbdx_davidson_block uses it too: the lowest nroots roots of the
+ * sub-problem (at most 25 block iterations, from the given vectors
+ * restricted to it, or from its lowest diagonal elements when they keep
+ * less than 1e-3 of their weight there) become the start vectors.
+ * Statistics of the last solve's coarse stage: sub-problem products,
+ * energy, wall seconds. */
 int sbdx_set_coarse_start(sbdx_handle_t h, size_t m, double eps);
 int sbdx_coarse_stats(sbdx_handle_t h, int *products, double *energy, double *seconds);
 
