@@ -143,12 +143,14 @@ struct Context {
   // (pq|rs) GEMM in the product's precision; 2 factored with the
   // split-FP16x3 GEMM for FP32 products.  Workspace of the factored form
   // (built on first use): D and G (fac_cap bytes each), x^T, the FP16
-  // operands [Dh | Dh | Dl] and [Eh | El | Eh] of the split GEMM, and their
+  // operands [Dh | Dh | Dl] and [Eh; El; Eh] (stacked per column) of the
+  // split GEMM, and their
   // two inverse scales (D's, ERI's).
   int mixed_form = 0;
   void *fac_D = nullptr, *fac_G = nullptr, *fac_xT = nullptr;
   void *fac_D3 = nullptr, *fac_E3 = nullptr;
   float *fac_inv = nullptr;
+  float *fac_part = nullptr;  // block maxima of |x| of the current factored FP32 product
   size_t fac_cap = 0;
   int mixed_eb[4] = {0, 0, 0, 0};
   int mixed_nt[4] = {0, 0, 0, 0};

@@ -1481,8 +1481,8 @@ static void fac_workspace(Context &c) {
 // FP16 products run as one GEMM over k = 3K,
 //   G = [Dh | Dh | Dl] [Eh; El; Eh],
 // so they accumulate in the matrix cores' FP32 accumulators and G is
-// written once.  The ERI operand is used transposed: its three K x K blocks
-// are stored one after another (Eh, El and Eh are symmetric).
+// written once.  The ERI operand is stacked per column (3K x K, column pq =
+// [Eh; El; Eh] of ERI row pq), built once.
 static const float *fac_gemm(Context &c, int M, const double *eri, const double *D, double *G) {
   if (gemm(blas(c), M, c.K, c.K, D, M, eri, c.K, G, M, 0.0) != rocblas_status_success)
     throw Error(3, "factored mixed-term DGEMM failed");
@@ -1503,14 +1503,24 @@ static const float *fac_gemm(Context &c, int M, const float *eri, const float *D
     c.fac_E3 = E3;
     // three FP16 copies of an FP32 D of at most fac_cap/4 elements
     c.fac_D3 = dalloc(c, 3 * 2 * (c.fac_cap / 4));
-    split_f16(c, KK, eri, E3, E3 + KK, c.fac_inv + 1, E3 + 2 * KK);
+    TmpBuf hl(2 * 2 * KK);
+    _Float16 *h = (_Float16 *)hl.p;
+    split_f16(c, KK, eri, h, h + KK, c.fac_inv + 1);
+    hipLaunchKernelGGL(k_stack3, dim3((unsigned)std::min<size_t>(cdiv(3 * KK, 256), 65536)), dim3(256), 0,
+                       c.stream, c.K, (const _Float16 *)h, (const _Float16 *)(h + KK), E3);
+    check_launch("k_stack3");
+    SBDX_HIP(hipStreamSynchronize(c.stream));  // before the halves are freed
   }
   _Float16 *D3 = (_Float16 *)c.fac_D3;
-  split_f16(c, n, D, D3, D3 + 2 * n, c.fac_inv, D3 + n);
+  // D's entries are +-x values or zero, so the block maxima of x taken once
+  // per product (fac_part) bound every batch: one scale for all batches
+  hipLaunchKernelGGL(k_split_f16, dim3((unsigned)std::min<size_t>(cdiv(n, 256), 8192)), dim3(256), 0, c.stream, n,
+                     D, (const float *)c.fac_part, kSplitParts, D3, D3 + 2 * n, D3 + n, c.fac_inv);
+  check_launch("k_split_f16");
   const int K = c.K;
   const float one = 1.0f, zero = 0.0f;
-  if (rocblas_gemm_ex(hb, rocblas_operation_none, rocblas_operation_transpose, M, K, 3 * K, &one, D3,
-                      rocblas_datatype_f16_r, M, c.fac_E3, rocblas_datatype_f16_r, K, &zero, G,
+  if (rocblas_gemm_ex(hb, rocblas_operation_none, rocblas_operation_none, M, K, 3 * K, &one, D3,
+                      rocblas_datatype_f16_r, M, c.fac_E3, rocblas_datatype_f16_r, 3 * K, &zero, G,
                       rocblas_datatype_f32_r, M, G, rocblas_datatype_f32_r, M, rocblas_datatype_f32_r,
                       rocblas_gemm_algo_standard, 0, 0) != rocblas_status_success)
     throw Error(3, "factored mixed-term split-FP16 GEMM failed");
@@ -1535,6 +1545,13 @@ static void mixed_factored(Context &c, const T *eri, const T *x, T *y) {
   T *xT = (T *)c.fac_xT, *D = (T *)c.fac_D, *G = (T *)c.fac_G;
   transpose<T>(c, x, xT, na, nb, false);
   const bool f32 = sizeof(T) == 4;
+  if constexpr (sizeof(T) == 4) {
+    if (c.mixed_form == 2) {  // block maxima of |x| for the split GEMM's scale
+      if (!c.fac_part) c.fac_part = (float *)dalloc(c, sizeof(float) * kSplitParts);
+      hipLaunchKernelGGL(k_absmax_part, dim3(kSplitParts), dim3(256), 0, c.stream, c.N, x, c.fac_part);
+      check_launch("k_absmax_part");
+    }
+  }
   for (int r0 = 0; r0 < nb; r0 += B) {
     const int Bc = std::min(B, nb - r0);
     const size_t M = (size_t)Bc * na, n = M * K;
