@@ -1416,12 +1416,40 @@ __global__ void __launch_bounds__(256) k_fac_zero(size_t n, T *__restrict__ v) {
 // grid (ceil(na/256), B): thread J of batch row b writes column (b, J) of D.
 // Within one bra r the singles' rs are distinct and off the (kk) diagonal, so
 // every element is written at most once.  xT = x^T (nb x na).
-template <typename T>
-__global__ void __launch_bounds__(256) k_fac_build(int na, int L, int r0, size_t M,
+// SPLIT: instead of D (n = M K entries), the FP16 operand [Dh | Dh | Dl] of
+// the split GEMM into D3 (3n halves, zero-filled), with the power-of-two
+// scale of k_split_f16 taken from the block maxima `part` of |x| (D holds
+// +-x values or zeros); block (0, 0) stores the inverse scale at *inv.
+template <typename T, bool SPLIT>
+__global__ void __launch_bounds__(256) k_fac_build(int na, int L, int r0, size_t M, size_t n,
                                                    const uint64_t *__restrict__ b_off,
                                                    const uint64_t *__restrict__ b_ent,
                                                    const uint64_t *__restrict__ b_str,
-                                                   const T *__restrict__ xT, T *__restrict__ D) {
+                                                   const T *__restrict__ xT, T *__restrict__ D,
+                                                   _Float16 *__restrict__ D3, const float *__restrict__ part,
+                                                   float *__restrict__ inv) {
+  float sc = 1.f;
+  if (SPLIT) {  // every thread of the block takes part, before any return
+    float m = 0.f;
+    for (int i = threadIdx.x; i < kSplitParts; i += 256) m = fmaxf(m, part[i]);
+    m = block_max256(m);
+    int k = 0;
+    if (m > 0.f) (void)frexpf(m, &k);  // m = f 2^k, f in [0.5, 1)
+    const int e = m > 0.f ? std::min(15 - k, 100) : 0;
+    sc = ldexpf(1.f, e);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *inv = ldexpf(1.f, -e);
+  }
+  auto put = [&](size_t idx, T v) {
+    if (SPLIT) {
+      const float t = (float)v * sc;
+      const _Float16 h = (_Float16)t;
+      D3[idx] = h;
+      D3[n + idx] = h;
+      D3[2 * n + idx] = (_Float16)(t - (float)h);
+    } else {
+      D[idx] = v;
+    }
+  };
   const int J = blockIdx.x * 256 + threadIdx.x;
   if (J >= na) return;
   const int b = blockIdx.y, r = r0 + b;
@@ -1429,10 +1457,10 @@ __global__ void __launch_bounds__(256) k_fac_build(int na, int L, int r0, size_t
   for (uint64_t f = b_off[r]; f < b_off[r + 1]; f++) {
     const uint64_t be = b_ent[f];
     const T v = xT[(size_t)ent_ket(be) * na + J];
-    D[(size_t)ent_pq(be) * M + col] = ent_neg(be) ? -v : v;
+    put((size_t)ent_pq(be) * M + col, ent_neg(be) ? -v : v);
   }
   const T xr = xT[(size_t)r * na + J];
-  for (uint64_t occ = b_str[r]; occ; occ &= occ - 1) D[(size_t)(__builtin_ctzll(occ) * (L + 1)) * M + col] = xr;
+  for (uint64_t occ = b_str[r]; occ; occ &= occ - 1) put((size_t)(__builtin_ctzll(occ) * (L + 1)) * M + col, xr);
 }
 
 // thread (I, b), b fastest: y[I, r0+b] += the alpha side over G.  gscale
@@ -1488,6 +1516,28 @@ static const float *fac_gemm(Context &c, int M, const double *eri, const double 
     throw Error(3, "factored mixed-term DGEMM failed");
   return nullptr;
 }
+// FP16 operands of the split form, built on first use: the ERI matrix
+// stacked per column ([Eh; El; Eh], 3K x K), room for [Dh | Dh | Dl] of an
+// FP32 D of at most fac_cap/4 entries, the block maxima of |x| and the two
+// inverse scales (D's, ERI's).
+static void fac_split_init(Context &c, const float *eri) {
+  if (c.fac_E3) return;
+  const size_t KK = (size_t)c.K * c.K;
+  if (!c.split_part) c.split_part = (float *)dalloc(c, sizeof(float) * kSplitParts);
+  c.fac_inv = (float *)dalloc(c, sizeof(float) * 2);
+  c.fac_part = (float *)dalloc(c, sizeof(float) * kSplitParts);
+  c.fac_D3 = dalloc(c, 3 * 2 * (c.fac_cap / 4));
+  _Float16 *E3 = (_Float16 *)dalloc(c, 2 * 3 * KK);
+  TmpBuf hl(2 * 2 * KK);
+  _Float16 *h = (_Float16 *)hl.p;
+  split_f16(c, KK, eri, h, h + KK, c.fac_inv + 1);
+  hipLaunchKernelGGL(k_stack3, dim3((unsigned)std::min<size_t>(cdiv(3 * KK, 256), 65536)), dim3(256), 0, c.stream,
+                     c.K, (const _Float16 *)h, (const _Float16 *)(h + KK), E3);
+  check_launch("k_stack3");
+  SBDX_HIP(hipStreamSynchronize(c.stream));  // before the halves are freed
+  c.fac_E3 = E3;
+}
+
 static const float *fac_gemm(Context &c, int M, const float *eri, const float *D, float *G) {
   rocblas_handle hb = blas(c);
   if (c.mixed_form != 2) {
@@ -1495,31 +1545,10 @@ static const float *fac_gemm(Context &c, int M, const float *eri, const float *D
       throw Error(3, "factored mixed-term SGEMM failed");
     return nullptr;
   }
-  const size_t KK = (size_t)c.K * c.K, n = (size_t)M * c.K;
-  if (!c.fac_E3) {
-    if (!c.split_part) c.split_part = (float *)dalloc(c, sizeof(float) * kSplitParts);
-    c.fac_inv = (float *)dalloc(c, sizeof(float) * 2);
-    _Float16 *E3 = (_Float16 *)dalloc(c, 2 * 3 * KK);
-    c.fac_E3 = E3;
-    // three FP16 copies of an FP32 D of at most fac_cap/4 elements
-    c.fac_D3 = dalloc(c, 3 * 2 * (c.fac_cap / 4));
-    TmpBuf hl(2 * 2 * KK);
-    _Float16 *h = (_Float16 *)hl.p;
-    split_f16(c, KK, eri, h, h + KK, c.fac_inv + 1);
-    hipLaunchKernelGGL(k_stack3, dim3((unsigned)std::min<size_t>(cdiv(3 * KK, 256), 65536)), dim3(256), 0,
-                       c.stream, c.K, (const _Float16 *)h, (const _Float16 *)(h + KK), E3);
-    check_launch("k_stack3");
-    SBDX_HIP(hipStreamSynchronize(c.stream));  // before the halves are freed
-  }
-  _Float16 *D3 = (_Float16 *)c.fac_D3;
-  // D's entries are +-x values or zero, so the block maxima of x taken once
-  // per product (fac_part) bound every batch: one scale for all batches
-  hipLaunchKernelGGL(k_split_f16, dim3((unsigned)std::min<size_t>(cdiv(n, 256), 8192)), dim3(256), 0, c.stream, n,
-                     D, (const float *)c.fac_part, kSplitParts, D3, D3 + 2 * n, D3 + n, c.fac_inv);
-  check_launch("k_split_f16");
+  // [Dh | Dh | Dl] was written by k_fac_build<float, true>
   const int K = c.K;
   const float one = 1.0f, zero = 0.0f;
-  if (rocblas_gemm_ex(hb, rocblas_operation_none, rocblas_operation_none, M, K, 3 * K, &one, D3,
+  if (rocblas_gemm_ex(hb, rocblas_operation_none, rocblas_operation_none, M, K, 3 * K, &one, c.fac_D3,
                       rocblas_datatype_f16_r, M, c.fac_E3, rocblas_datatype_f16_r, 3 * K, &zero, G,
                       rocblas_datatype_f32_r, M, G, rocblas_datatype_f32_r, M, rocblas_datatype_f32_r,
                       rocblas_gemm_algo_standard, 0, 0) != rocblas_status_success)
@@ -1545,9 +1574,11 @@ static void mixed_factored(Context &c, const T *eri, const T *x, T *y) {
   T *xT = (T *)c.fac_xT, *D = (T *)c.fac_D, *G = (T *)c.fac_G;
   transpose<T>(c, x, xT, na, nb, false);
   const bool f32 = sizeof(T) == 4;
+  bool split = false;
   if constexpr (sizeof(T) == 4) {
-    if (c.mixed_form == 2) {  // block maxima of |x| for the split GEMM's scale
-      if (!c.fac_part) c.fac_part = (float *)dalloc(c, sizeof(float) * kSplitParts);
+    if (c.mixed_form == 2) {  // FP16 operands, and the block maxima of |x| for D's scale
+      split = true;
+      fac_split_init(c, eri);
       hipLaunchKernelGGL(k_absmax_part, dim3(kSplitParts), dim3(256), 0, c.stream, c.N, x, c.fac_part);
       check_launch("k_absmax_part");
     }
@@ -1555,11 +1586,22 @@ static void mixed_factored(Context &c, const T *eri, const T *x, T *y) {
   for (int r0 = 0; r0 < nb; r0 += B) {
     const int Bc = std::min(B, nb - r0);
     const size_t M = (size_t)Bc * na, n = M * K;
-    hipLaunchKernelGGL((k_fac_zero<T>), dim3((unsigned)std::min<size_t>(cdiv(n, 256), 65536)), dim3(256), 0,
-                       c.stream, n, D);
-    check_launch("k_fac_zero");
-    hipLaunchKernelGGL((k_fac_build<T>), dim3(cdiv(na, 256), (unsigned)Bc), dim3(256), 0, c.stream, na, c.L, r0,
-                       M, c.B.s_off, c.B.s_ent, c.B.str, (const T *)xT, D);
+    const dim3 gb(cdiv(na, 256), (unsigned)Bc);
+    if (split) {
+      _Float16 *D3 = (_Float16 *)c.fac_D3;
+      hipLaunchKernelGGL((k_fac_zero<_Float16>), dim3((unsigned)std::min<size_t>(cdiv(3 * n, 256), 65536)),
+                         dim3(256), 0, c.stream, 3 * n, D3);
+      check_launch("k_fac_zero");
+      hipLaunchKernelGGL((k_fac_build<T, true>), gb, dim3(256), 0, c.stream, na, c.L, r0, M, n, c.B.s_off,
+                         c.B.s_ent, c.B.str, (const T *)xT, D, D3, (const float *)c.fac_part, c.fac_inv);
+    } else {
+      hipLaunchKernelGGL((k_fac_zero<T>), dim3((unsigned)std::min<size_t>(cdiv(n, 256), 65536)), dim3(256), 0,
+                         c.stream, n, D);
+      check_launch("k_fac_zero");
+      hipLaunchKernelGGL((k_fac_build<T, false>), gb, dim3(256), 0, c.stream, na, c.L, r0, M, n, c.B.s_off,
+                         c.B.s_ent, c.B.str, (const T *)xT, D, (_Float16 *)nullptr, (const float *)nullptr,
+                         (float *)nullptr);
+    }
     check_launch("k_fac_build");
     const float *gs;
     {
