@@ -179,9 +179,13 @@ def main():
                          "with FP32 correction products")
     ap.add_argument("--block", type=int, default=4)
     ap.add_argument("--roots", type=int, default=1, help="roots of the block solver")
-    ap.add_argument("--no-symmetry", dest="symmetry", action="store_false",
-                    help="general sigma in the optimized solver (default: spin-flip symmetric "
-                         "form when the string sets are equal)")
+    ap.add_argument("--sectors", choices=["lowest", "even", "odd", "off"], default="lowest",
+                    help="spin-flip sectors of the optimized solver when the string sets are equal: "
+                         "lowest (default) solves the even sector from the start vector, then the "
+                         "odd one, and keeps the lower state; even/odd one sector; off the general "
+                         "sigma")
+    ap.add_argument("--no-symmetry", dest="sectors", action="store_const", const="off",
+                    help="same as --sectors off")
     ap.add_argument("--coarse", type=int, default=500,
                     help="coarse start on the first M strings of each set (optimized solver; 0: off)")
     ap.add_argument("--coarse-eps", type=float, default=1e-2)
@@ -221,6 +225,7 @@ def main():
     log(f"operator built in {t_build:.2f} s: ndet={op.ndet} counts={wc}")
 
     same_sets = bool(np.array_equal(inst["astr"], inst["bstr"]))
+    symmetric = args.sectors != "off" and same_sets
     if args.init == "hf":
         W0 = sbdx.basis_init_vector(op.ndet)
     else:
@@ -234,7 +239,7 @@ def main():
         op.sigma(W0)
         if args.solver == "mixed":
             op.sigma_fp32(W0)
-        if args.symmetry and same_sets and args.solver in ("thick", "mixed"):
+        if symmetric and args.solver in ("thick", "mixed"):
             e0 = sbdx.basis_init_vector(op.ndet)
             op.sigma_sym(e0)
             if args.solver == "mixed":
@@ -255,14 +260,14 @@ def main():
         op.configure(solver)
         if solver in ("thick", "mixed"):
             op.set_restart(args.keep)
-            op.set_symmetry(args.symmetry and same_sets)
+            op.set_symmetry(args.sectors if symmetric else "off")
         if solver in ("thick", "mixed", "block"):
             op.set_coarse_start(args.coarse if args.coarse < min(na, nb) else 0, args.coarse_eps)
         op.set_preconditioner(1 if precond == "mlp" else 0)
         op.set_verbose(0 if args.quiet else 1)
         op.profile(True)
         times, energies, nsteps, conv = [], [], [], []
-        roots = None
+        roots = sectors = None
         for k in range(nsolve):
             op.set_vector(W0)
             t0 = time.perf_counter()
@@ -281,6 +286,9 @@ def main():
             energies.append(E)
             nsteps.append(n)
             conv.append(ok)
+            if solver in ("thick", "mixed") and symmetric:
+                sectors = op.sector_stats()
+                log(f"spin-flip sectors: {sectors}")
             tr = op.trace()
             extra = (f" coarse={op.coarse_stats()}" if solver in ("thick", "mixed", "block") else "") + \
                 (f" fp64-from-step={op.fp64_switch_step()}" if solver in ("thick", "mixed") else "")
@@ -290,6 +298,7 @@ def main():
         op.profile(False)
         return dict(time=float(np.median(times)), E=energies[-1], steps=int(np.median(nsteps)),
                     converged=all(conv), stats=stats, precond_steps=op.precond_stats(), roots=roots,
+                    sectors=sectors,
                     fp64_switch_step=op.fp64_switch_step() if solver in ("thick", "mixed") else None,
                     coarse=op.coarse_stats() if solver in ("thick", "mixed", "block") else None)
 
@@ -297,13 +306,13 @@ def main():
     plain = run("plain", 1 if best != "plain" else args.steps) if args.compare or best == "plain" else None
     res = run(best, args.steps, args.precond) if best != "plain" else plain
     graph_res = None
-    if args.graph_run and best == "mixed" and args.symmetry and same_sets:
+    if args.graph_run and best == "mixed" and symmetric:
         # the same solve with the FP32 symmetric sigma core replayed as a
         # captured hipGraph (profiling off: one timed solve)
         op.set_graphs(True)
         op.configure(best)
         op.set_restart(args.keep)
-        op.set_symmetry(True)
+        op.set_symmetry(args.sectors)
         op.set_coarse_start(args.coarse if args.coarse < min(na, nb) else 0, args.coarse_eps)
         op.set_preconditioner(1 if args.precond == "mlp" else 0)
         op.set_verbose(0)
@@ -404,7 +413,8 @@ def main():
             "solver": best,
             "subspace": args.nb,
             "restart_keep": args.keep if best in ("thick", "mixed") else None,
-            "spin_flip_symmetric": bool(args.symmetry and same_sets and best in ("thick", "mixed")),
+            "spin_flip_sectors": args.sectors if symmetric and best in ("thick", "mixed") else "off",
+            "sector_results": res.get("sectors"),
             "coarse_start": None if res["coarse"] is None or not args.coarse else
             dict(strings=args.coarse, eps=args.coarse_eps, **res["coarse"]),
             "preconditioner": args.precond,

@@ -90,7 +90,9 @@ int sbdx_sigma_device(sbdx_handle_t h, const double *d_Wk, double *d_Wb, int acc
  * Single-root Davidson, chemistry/tpb/davidson.h:667-930 and
  * chemistry/basic/davidson_thrust.h:85-373: restarts every nb vectors,
  * preconditioner R/(E0-d) with the 1e-12 guard, modified Gram-Schmidt,
- * Rayleigh-Ritz on the device, stop when |E0 W - H W| < eps, or when the
+ * Rayleigh-Ritz (subspace matrix built on the device, its eigenproblem of
+ * at most 64 x 64 solved on the host as the reference's dsyev call is),
+ * stop when |E0 W - H W| < eps, or when the
  * predicted end of the next step passes max_time seconds.
  * W (host, n_det) holds the initial vector on entry and the normalized
  * solution on exit.  *energy receives the lowest Ritz value of the last
@@ -211,27 +213,47 @@ int sbdx_set_mixed_form(sbdx_handle_t h, int form);
 int sbdx_symmetric_eig(int n, const double *A, double *V, double *E);
 
 /* Spin-flip symmetry (no counterpart in the reference): with equal alpha and
- * beta string sets, H commutes with the exchange of the two strings, and in
- * the alpha-first basis of the device a vector of the symmetric sector is a
- * symmetric na x na matrix (W[ia*nb+ib] = W[ib*nb+ia] after the basis sign
- * of the interleaved convention).  With on != 0 the thick-restart solves
- * (sbdx_set_restart > 0) compute sigma in that form: one same-spin GEMM
- * G = H X and y = hii*x + G + G^T, and the mixed term for the upper
- * triangle only, mirrored -- about half the work.  The start vector must be
- * symmetric (the Hartree-Fock vector of BasisInitVector is); the solve
- * then stays in the symmetric sector and returns its lowest state.  That is
- * the ground state only when the ground state is even under the exchange
- * (closed-shell singlets, e.g. the molecular data sets and the norb-36
- * workload, where it matches the general solve to 1e-12).  When the ground
- * state is odd (found for 3 electrons per spin in 64 orbitals), the
- * reference's Davidson from the Hartree-Fock vector reaches it only through
- * rounding noise after hundreds of steps, and this mode returns the lowest
- * even state instead: use the general form there.  Errors with
- * SBDX_ERR_ARG when the string sets differ or the start vector is not
- * symmetric. */
-int sbdx_set_symmetry(sbdx_handle_t h, int on);
-/* sigma of a spin-flip symmetric vector (reference convention) in the
- * symmetric form, FP64 (fp32 = 0) or FP32 arithmetic; Wb is overwritten. */
+ * beta string sets, H commutes with the exchange of the two strings, so its
+ * eigenvectors split into two sectors.  In the alpha-first basis of the
+ * device a vector of the even sector is a symmetric na x na matrix
+ * (W[ia*nb+ib] = W[ib*nb+ia] after the basis sign of the interleaved
+ * convention), one of the odd sector an antisymmetric one.  In either sector
+ * sigma needs one same-spin GEMM G = H X, y = hii*x + (G +- G^T), and the
+ * mixed term for the upper triangle only, mirrored with the sector's sign --
+ * about half the work of the general form.  The mode applies to the
+ * thick-restart solves (sbdx_set_restart > 0):
+ *   SBDX_SYM_OFF     the general form;
+ *   SBDX_SYM_EVEN    the even sector: the start vector must be symmetric
+ *                    (the Hartree-Fock vector of BasisInitVector is), the
+ *                    solve returns the sector's lowest state;
+ *   SBDX_SYM_ODD     the odd sector likewise (antisymmetric start vector);
+ *   SBDX_SYM_LOWEST  both: the even sector from the start vector's even
+ *                    part, then the odd sector from its odd part (or, when
+ *                    that is negligible, from (e_ab - e_ba)/sqrt 2 on the
+ *                    lowest off-diagonal diagonal element hii[a,b], a < b);
+ *                    the odd stage stops once its Ritz value E0 lies above
+ *                    the even result by more than 10 |r| (near convergence
+ *                    E0 exceeds the eigenvalue it approaches by ~|r|^2/gap),
+ *                    and the lower of the two results is returned.
+ * The reference's Davidson from the Hartree-Fock vector stays in the even
+ * sector up to rounding, so it can only find an odd ground state (found for
+ * synthetic instances of 3 and 5 electrons per spin) through rounding noise
+ * after many steps; SBDX_SYM_LOWEST returns the ground state of either
+ * sector.  Errors with SBDX_ERR_ARG when the string sets differ, and the
+ * solve errors when the start vector is outside the chosen sector. */
+#define SBDX_SYM_OFF 0
+#define SBDX_SYM_EVEN 1
+#define SBDX_SYM_ODD 2
+#define SBDX_SYM_LOWEST 3
+int sbdx_set_symmetry(sbdx_handle_t h, int mode);
+/* Per sector (0 even, 1 odd) of the last thick solve: its final energy,
+ * sigma products and state (0 not run, 1 converged, 2 not converged, 3 odd
+ * stage stopped once above the even result); *chosen the sector returned
+ * (1 even, 2 odd, 0 when the solve used the general form). */
+int sbdx_sector_stats(sbdx_handle_t h, double *energies, int *products, int *states, int *chosen);
+/* sigma of a vector of either spin-flip sector (reference convention) in
+ * the symmetric form of that sector, FP64 (fp32 = 0) or FP32 arithmetic; Wb
+ * is overwritten.  SBDX_ERR_ARG when the vector lies in neither sector. */
 int sbdx_sigma_sym(sbdx_handle_t h, const double *Wk, double *Wb, int fp32);
 
 /* Coarse start of the thick-restart solves (no counterpart in the

@@ -449,8 +449,8 @@ int sbdx_solve(sbdx_handle_t h, int max_it, int nb, double eps, double max_time,
   if (max_it < 1) throw Error(SBDX_ERR_ARG, "max_it must be >= 1");
   const int rc = c.restart_keep == 0
                      ? davidson_solve(c, max_it, nb, eps, max_time, energy, n_steps)
-                     : davidson_solve_thick(c, max_it * nb, nb, c.restart_keep, eps, max_time,
-                                            c.precision == SBDX_PREC_MIXED, energy, n_steps);
+                     : davidson_solve_sectors(c, max_it * nb, nb, c.restart_keep, eps, max_time,
+                                              c.precision == SBDX_PREC_MIXED, energy, n_steps);
   if (rc != 0) g_err = "Davidson did not reach eps";
   return rc;
   SBDX_GUARD_END
@@ -548,13 +548,28 @@ int sbdx_symmetric_eig(int n, const double *A, double *V, double *E) {
   SBDX_GUARD_END
 }
 
-int sbdx_set_symmetry(sbdx_handle_t h, int on) {
+int sbdx_set_symmetry(sbdx_handle_t h, int mode) {
   SBDX_GUARD_BEGIN
   Context &c = ctx_of(h);
-  if (on && !sigma_sym_available(c))
+  if (mode < SBDX_SYM_OFF || mode > SBDX_SYM_LOWEST)
+    throw Error(SBDX_ERR_ARG, "spin-flip symmetry mode must be 0 (off), 1 (even), 2 (odd) or 3 (lowest of both)");
+  if (mode && !sigma_sym_available(c))
     throw Error(SBDX_ERR_ARG, "spin-flip symmetry needs equal alpha and beta string sets (and memory for the "
                               "dense same-spin block)");
-  c.symmetric = on ? 1 : 0;
+  c.symmetric = mode;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_sector_stats(sbdx_handle_t h, double *energies, int *products, int *states, int *chosen) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  for (int k = 0; k < 2; k++) {
+    if (energies) energies[k] = c.sector_E[k];
+    if (products) products[k] = c.sector_products[k];
+    if (states) states[k] = c.sector_state[k];
+  }
+  if (chosen) *chosen = c.sector_chosen;
   return SBDX_OK;
   SBDX_GUARD_END
 }
@@ -566,11 +581,25 @@ int sbdx_sigma_sym(sbdx_handle_t h, const double *Wk, double *Wb, int fp32) {
   if (!sigma_sym_available(c)) throw Error(SBDX_ERR_ARG, "spin-flip symmetry not available for this handle");
   upload(c, c.vout, Wk);
   sign_convert(c, c.vout, c.vin);
-  if (sym_defect(c, c.vin) > 1e-6) throw Error(SBDX_ERR_ARG, "vector is not spin-flip symmetric");
-  if (fp32)
-    sigma_sym_f32(c, c.vin, c.vout);
-  else
-    sigma_sym(c, c.vin, c.vout);
+  // the sector is the vector's own: even (x = x^T) or odd (x = -x^T)
+  int par = 1;
+  if (sym_defect(c, c.vin, 1) > 1e-6) {
+    if (sym_defect(c, c.vin, -1) > 1e-6)
+      throw Error(SBDX_ERR_ARG, "vector lies in neither spin-flip sector (x = x^T or x = -x^T)");
+    par = -1;
+  }
+  const int saved = c.sym_par;
+  c.sym_par = par;
+  try {
+    if (fp32)
+      sigma_sym_f32(c, c.vin, c.vout);
+    else
+      sigma_sym(c, c.vin, c.vout);
+  } catch (...) {
+    c.sym_par = saved;
+    throw;
+  }
+  c.sym_par = saved;
   sign_convert(c, c.vout, c.vin);
   download(c, Wb, c.vin);
   return SBDX_OK;

@@ -160,12 +160,28 @@ struct Context {
   int multi_mode[2] = {-1, -1};
   int precision = 0;       // 0: FP64 throughout; 1: FP32 sigma on corrections, FP64 restarts
   int verbose = 0;         // 1: one stderr line per Davidson step (E0, |r|)
-  int symmetric = 0;       // 1: thick-restart solves use the spin-flip symmetric sigma
+  // spin-flip sectors of the thick-restart solves: 0 general form; 1 the
+  // even sector (x = x^T), 2 the odd sector (x = -x^T), 3 both, the lower
+  // result kept.  sym_par (+1 / -1) is the sector of the symmetric products
+  // being computed.
+  int symmetric = 0;
+  int sym_par = 1;
+  // results of the last two-sector solve: energies, products, converged
+  // flags, abandoned (odd stage stopped once it lay above the even result),
+  // the sector returned (1 even, 2 odd)
+  double sector_E[2] = {0.0, 0.0};
+  int sector_products[2] = {0, 0};
+  int sector_state[2] = {0, 0};  // 0 not run, 1 converged, 2 not converged, 3 abandoned
+  int sector_chosen = 0;
+  // thick solves stop early (status 6) once E0 - |r| exceeds this value
+  // (the odd stage of a two-sector solve); +inf: never
+  double abandon_above = 1e300;
   // hipGraph replay of the FP32 symmetric sigma core (fixed FP32 staging
   // buffers): captured once after an eager warm-up when use_graphs is set
   int use_graphs = 0;
   bool capturing = false;  // no profiling events while a stream capture runs
   hipGraphExec_t g_sym32 = nullptr;
+  int g_sym32_par = 1;     // the sector the graph was captured for
   // Coarse start of the thick-restart solves: the problem on the first
   // coarse_m strings of each set solved to coarse_eps first (0: off), and
   // what it cost in the last solve.
@@ -263,8 +279,8 @@ void sigma_block(Context &c, int k, const double *x, double *y);
 bool sigma_sym_available(Context &c);
 void sigma_sym(Context &c, const double *x, double *y);
 void sigma_sym_f32(Context &c, const double *x, double *y);
-double sym_defect(Context &c, const double *v);  // max|v - v^T| / max|v|
-void symmetrize(Context &c, double *v);         // v = (v + v^T)/2
+double sym_defect(Context &c, const double *v, int p);  // max|v - p v^T| / max|v|, p = +-1
+void symmetrize(Context &c, double *v, int p);         // v = (v + p v^T)/2
 void sigma_block_f32(Context &c, int k, const double *x, double *y);
 void same_dense_release(Context &c);  // frees the rocBLAS handle
 void graphs_release(Context &c);      // frees captured graphs
@@ -280,6 +296,10 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
                    bool mixed, const double *W0, double *Wout, double *Eout, int *n_products);
 int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps, double max_time,
                          bool mixed, double *energy, int *n_steps);
+// the thick solve in the spin-flip sector(s) of c.symmetric (mode 3: both,
+// the lower kept)
+int davidson_solve_sectors(Context &c, int max_steps, int nb, int keep, double eps, double max_time,
+                           bool mixed, double *energy, int *n_steps);
 double device_dot(Context &c, const double *x, const double *y, size_t n);
 
 // rdm.hip

@@ -632,21 +632,23 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
                        H, ld, col);
     check_launch("k_finish_hcol");
   };
-  // spin-flip symmetric products when asked for: the start vector must be
-  // symmetric (x = x^T in the alpha-first layout); the diagonal step keeps
-  // every later vector symmetric, the learned step is symmetrized
+  // spin-flip symmetric products when asked for: the start vector must lie
+  // in the sector c.sym_par (x = x^T even, x = -x^T odd, in the alpha-first
+  // layout); the diagonal step keeps every later vector there (hii is
+  // symmetric), the learned step is projected back
   const bool sym = c.symmetric != 0;
   if (sym) {
     if (!sigma_sym_available(c))
       throw Error(1, "symmetric mode needs equal alpha and beta string sets and room for the dense blocks");
-    const double defect = sym_defect(c, c.W);
+    const double defect = sym_defect(c, c.W, c.sym_par);
     if (defect > 1e-6)
     {
-      char msg[128];
-      snprintf(msg, sizeof msg, "symmetric mode: the start vector is not spin-flip symmetric (defect %.3e)", defect);
+      char msg[160];
+      snprintf(msg, sizeof msg, "symmetric mode: the start vector is not in the spin-flip %s sector (defect %.3e)",
+               c.sym_par < 0 ? "odd" : "even", defect);
       throw Error(1, msg);
     }
-    symmetrize(c, c.W);  // rounding-level asymmetry of a vector from elsewhere
+    symmetrize(c, c.W, c.sym_par);  // rounding-level defect of a vector from elsewhere
   }
   auto apply = [&](const double *x, double *y, bool approx) {
     if (sym && approx)
@@ -679,7 +681,8 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
     // nested levels: the sub-problem starts from a quarter of its strings
     sub->coarse_m = mc / 4 >= 256 ? mc / 4 : 0;
     sub->coarse_eps = c.coarse_eps;
-    sub->symmetric = (sym && sigma_sym_available(*sub)) ? 1 : 0;
+    sub->symmetric = (sym && sigma_sym_available(*sub)) ? (c.sym_par < 0 ? 2 : 1) : 0;
+    sub->sym_par = c.sym_par;
     // copies between the two contexts run on the consumer's stream after the
     // producer's stream has drained (both are non-blocking streams)
     SBDX_HIP(hipStreamSynchronize(c.stream));
@@ -716,7 +719,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
   apply(c.C[0], c.HC[0], false);
   add_column(0);
   int d = 0, steps = 1;
-  bool done = false, fresh = true;
+  bool done = false, fresh = true, abandoned = false;
   double E0 = 0.0, nr = 0.0;
   std::vector<double> rhist;
   double prev_exact = -1.0;  // exact residual of the previous restart
@@ -752,6 +755,16 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
     const bool exact_now = !mixed;  // every product in the basis is FP64
     if (nr < eps && exact_now) {
       done = true;
+      break;
+    }
+    // The Ritz value E0 only decreases, and near convergence it lies above
+    // the eigenvalue it approaches by about |r|^2 / (gap to the next one):
+    // once E0 exceeds abandon_above by ten times |r|, the state this
+    // iteration converges to lies above that value and the caller has no
+    // use for it (the same premise as any Davidson solve: the iteration
+    // converges to the lowest state its start vector reaches)
+    if (E0 - c.abandon_above > 10.0 * nr) {
+      abandoned = true;
       break;
     }
     const bool timeout = elapsed() > max_time || steps >= max_steps;
@@ -863,7 +876,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       double *cn = c.C[d + 1];
       if (use_mlp) {
         mlp_precond(c, c.R, E, cn);
-        if (sym) symmetrize(c, cn);
+        if (sym) symmetrize(c, cn, c.sym_par);
         c.mlp.used_mlp++;
       } else {
         hipLaunchKernelGGL(k_precond, g, b, 0, c.stream, c.R, c.hii, E, N, eps_reg, cn);
@@ -910,7 +923,162 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
   SBDX_HIP(hipStreamSynchronize(c.stream));
   if (energy) *energy = E0;
   if (n_steps) *n_steps = steps;
-  return done ? 0 : 5;
+  return done ? 0 : abandoned ? 6 : 5;
+}
+
+// ------------------------------------------------------- spin-flip sectors
+// Smallest hii[a*n + b] over a < b (block partials: value, flat index).
+__global__ void __launch_bounds__(kRedThreads) k_argmin_offdiag(size_t n, const double *__restrict__ hii,
+                                                                double *__restrict__ part_v,
+                                                                unsigned long long *__restrict__ part_i) {
+  __shared__ double sv[kRedThreads];
+  __shared__ unsigned long long si[kRedThreads];
+  double best = 1e300;
+  unsigned long long bi = ~0ull;
+  const size_t N = n * n;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    if (i / n < i % n && hii[i] < best) {
+      best = hii[i];
+      bi = i;
+    }
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int st = kRedThreads / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      const double v = sv[threadIdx.x + st];
+      const unsigned long long j = si[threadIdx.x + st];
+      // ties go to the lower index, so the choice does not depend on the grid
+      if (v < sv[threadIdx.x] || (v == sv[threadIdx.x] && j < si[threadIdx.x])) {
+        sv[threadIdx.x] = v;
+        si[threadIdx.x] = j;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part_v[blockIdx.x] = sv[0];
+    part_i[blockIdx.x] = si[0];
+  }
+}
+
+// The thick solve in the spin-flip sector(s) c.symmetric asks for.  Mode 3
+// runs the even sector from the even part of the start vector, then the odd
+// sector from its odd part (or, when that part is negligible -- the
+// Hartree-Fock vector has none -- from the antisymmetric pair on the lowest
+// off-diagonal diagonal element, (e_ab - e_ba)/sqrt 2), and keeps the lower
+// of the two.  The odd stage stops once its Ritz value lies above the even
+// result by more than ten times its residual norm (near convergence the Ritz
+// value exceeds its eigenvalue by ~|r|^2 / gap), so when the even sector
+// holds the ground state the odd stage costs only the steps needed to show
+// that.
+int davidson_solve_sectors(Context &c, int max_steps, int nb, int keep, double eps, double max_time,
+                           bool mixed, double *energy, int *n_steps) {
+  c.sector_E[0] = c.sector_E[1] = 0.0;
+  c.sector_products[0] = c.sector_products[1] = 0;
+  c.sector_state[0] = c.sector_state[1] = 0;
+  c.sector_chosen = 0;
+  c.abandon_above = 1e300;
+  if (c.symmetric != 3) {
+    c.sym_par = c.symmetric == 2 ? -1 : 1;
+    const int rc = davidson_solve_thick(c, max_steps, nb, keep, eps, max_time, mixed, energy, n_steps);
+    if (c.symmetric != 0) {
+      const int k = c.symmetric == 2 ? 1 : 0;
+      c.sector_E[k] = energy ? *energy : 0.0;
+      c.sector_products[k] = n_steps ? *n_steps : 0;
+      c.sector_state[k] = rc == 0 ? 1 : 2;
+      c.sector_chosen = c.symmetric;
+    }
+    return rc;
+  }
+  if (!sigma_sym_available(c))
+    throw Error(1, "symmetric mode needs equal alpha and beta string sets and room for the dense blocks");
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t N = c.N, n = c.na;
+  TmpBuf start(sizeof(double) * N), even(sizeof(double) * N);
+  double *W0 = (double *)start.p, *We = (double *)even.p;
+  SBDX_HIP(hipMemcpyAsync(W0, c.W, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+  const double n0 = device_dot(c, W0, W0, N);
+  if (!(n0 > 0.0)) throw Error(1, "two-sector solve: zero start vector");
+  // ---- even sector
+  double Ee = 0.0;
+  int se = 0, rce = 5;
+  c.sym_par = 1;
+  symmetrize(c, c.W, 1);
+  const double ne = device_dot(c, c.W, c.W, N);
+  if (ne > 1e-12 * n0) {
+    rce = davidson_solve_thick(c, max_steps, nb, keep, eps, max_time, mixed, &Ee, &se);
+    SBDX_HIP(hipMemcpyAsync(We, c.W, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+    c.sector_E[0] = Ee;
+    c.sector_products[0] = se;
+    c.sector_state[0] = rce == 0 ? 1 : 2;
+  }
+  // ---- odd sector
+  SBDX_HIP(hipMemcpyAsync(c.W, W0, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+  symmetrize(c, c.W, -1);
+  const double no = device_dot(c, c.W, c.W, N);
+  if (!(no > 1e-6 * n0)) {
+    if (n < 2) {
+      // one string: the odd sector is empty
+      SBDX_HIP(hipMemcpyAsync(c.W, We, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+      SBDX_HIP(hipStreamSynchronize(c.stream));
+      c.sector_chosen = 1;
+      if (energy) *energy = Ee;
+      if (n_steps) *n_steps = se;
+      return rce;
+    }
+    TmpBuf pv(sizeof(double) * kRedBlocks), pi(sizeof(unsigned long long) * kRedBlocks);
+    hipLaunchKernelGGL(k_argmin_offdiag, dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, n, c.hii,
+                       (double *)pv.p, (unsigned long long *)pi.p);
+    check_launch("k_argmin_offdiag");
+    std::vector<double> hv(kRedBlocks);
+    std::vector<unsigned long long> hi(kRedBlocks);
+    SBDX_HIP(hipMemcpyAsync(hv.data(), pv.p, sizeof(double) * kRedBlocks, hipMemcpyDeviceToHost, c.stream));
+    SBDX_HIP(hipMemcpyAsync(hi.data(), pi.p, sizeof(unsigned long long) * kRedBlocks, hipMemcpyDeviceToHost,
+                            c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+    double bv = 1e300;
+    unsigned long long bi = ~0ull;
+    for (int b = 0; b < kRedBlocks; b++)
+      if (hv[b] < bv || (hv[b] == bv && hi[b] < bi)) {
+        bv = hv[b];
+        bi = hi[b];
+      }
+    const size_t a = bi / n, b = bi % n;
+    const double v[2] = {std::sqrt(0.5), -std::sqrt(0.5)};
+    SBDX_HIP(hipMemsetAsync(c.W, 0, sizeof(double) * N, c.stream));
+    SBDX_HIP(hipMemcpyAsync(c.W + a * n + b, &v[0], sizeof(double), hipMemcpyHostToDevice, c.stream));
+    SBDX_HIP(hipMemcpyAsync(c.W + b * n + a, &v[1], sizeof(double), hipMemcpyHostToDevice, c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+  }
+  const double left =
+      max_time - std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  c.sym_par = -1;
+  c.abandon_above = c.sector_state[0] ? Ee : 1e300;
+  double Eo = 0.0;
+  int so = 0, rco = 5;
+  try {
+    rco = davidson_solve_thick(c, max_steps, nb, keep, eps, std::max(left, 0.0), mixed, &Eo, &so);
+  } catch (...) {
+    c.abandon_above = 1e300;
+    c.sym_par = 1;
+    throw;
+  }
+  c.abandon_above = 1e300;
+  c.sym_par = 1;
+  c.sector_E[1] = Eo;
+  c.sector_products[1] = so;
+  c.sector_state[1] = rco == 0 ? 1 : rco == 6 ? 3 : 2;
+  // the odd result wins when its Ritz value (an upper bound of the odd
+  // sector's lowest eigenvalue) lies below the even result
+  const bool odd = rco != 6 && (!c.sector_state[0] || Eo < Ee);
+  if (!odd) SBDX_HIP(hipMemcpyAsync(c.W, We, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  c.sector_chosen = odd ? 2 : 1;
+  if (energy) *energy = odd ? Eo : Ee;
+  if (n_steps) *n_steps = se + so;
+  return odd ? rco : rce;
 }
 
 // ------------------------------------------------------------ block Davidson

@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
                                                     const uint64_t *__restrict__ sl_off,
                                                     const uint32_t *__restrict__ sl_len,
                                                     const T *__restrict__ eri,
-                                                    const T *__restrict__ x, T *__restrict__ y) {
+                                                    const T *__restrict__ x, T *__restrict__ y, int odd) {
   typedef typename Pack<T, EB>::V V;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   V *X = reinterpret_cast<V *>(lds_raw);  // nb ket values per pair
@@ -306,9 +306,11 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
       const T v = hsum<T, EB>(acc[t]);
       const uint32_t r = perm[slot];
       if (SYM) {
-        if (r != 0xffffffffu && slot >= pos) {
+        if (r != 0xffffffffu && slot > pos) {
           y[(size_t)I * nb + r] += v;
-          if (slot > pos) y[(size_t)r * nb + I] += v;
+          y[(size_t)r * nb + I] += odd ? -v : v;
+        } else if (r != 0xffffffffu && slot == pos && !odd) {
+          y[(size_t)I * nb + r] += v;  // odd sector: the diagonal stays 0
         }
       } else {
         if (r != 0xffffffffu) y[(size_t)I * nb + r] += v;
@@ -427,7 +429,8 @@ __global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int L, int 
                                                       const uint64_t *__restrict__ b_str,
                                                       PartTables pt, const uint32_t *__restrict__ perm,
                                                       const T *__restrict__ eri,
-                                                      const T *__restrict__ x, T *__restrict__ y) {
+                                                      const T *__restrict__ x, T *__restrict__ y,
+                                                      int odd) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *crow = reinterpret_cast<T *>(lds_raw);
   T *vrow = crow + h;
@@ -499,9 +502,11 @@ __global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int L, int 
     if (slot < nslots) {
       const uint32_t r = perm[slot];
       if (SYM) {
-        if (r != 0xffffffffu && slot >= pos) {
+        if (r != 0xffffffffu && slot > pos) {
           y[(size_t)I * nb + r] += acc[t];
-          if (slot > pos) y[(size_t)r * nb + I] += acc[t];
+          y[(size_t)r * nb + I] += odd ? -acc[t] : acc[t];
+        } else if (r != 0xffffffffu && slot == pos && !odd) {
+          y[(size_t)I * nb + r] += acc[t];
         }
       } else {
         if (r != 0xffffffffu) y[(size_t)I * nb + r] += acc[t];
@@ -637,13 +642,15 @@ __global__ void __launch_bounds__(256) k_densify(int n, const uint64_t *__restri
 
 // Spin-flip symmetric form of the same-spin terms for equal string sets:
 // y = hii*x + (G + G^T) with G = H X (n x n row-major), 64x64 tiles, G^T
-// through LDS; for symmetric x the result is bitwise symmetric.
+// through LDS; for symmetric x the result is bitwise symmetric.  For the
+// odd sector (x = -x^T, odd != 0) X H = -G^T, so y = hii*x + (G - G^T),
+// bitwise antisymmetric with a zero diagonal.
 // gscale (device, may be null): G holds 2^(ex+eh) times the product (the
 // split-FP16 GEMM), and gscale[0] * gscale[1] = 2^-(ex+eh) undoes it exactly.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sym_combine(int n, const T *__restrict__ hii, const T *__restrict__ x,
                                                      const T *__restrict__ G, T *__restrict__ y,
-                                                     const float *__restrict__ gscale) {
+                                                     const float *__restrict__ gscale, int odd) {
   const T s = gscale ? (T)(gscale[0] * gscale[1]) : T(1);
   __shared__ T tile[64][65];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -657,7 +664,8 @@ __global__ void __launch_bounds__(256) k_sym_combine(int n, const T *__restrict_
     const int r = r0 + k, cc = c0 + tx;
     if (r < n && cc < n) {
       const size_t o = (size_t)r * n + cc;
-      y[o] = hii[o] * x[o] + (G[o] + tile[tx][k]) * s;  // G + G^T commutes: bitwise symmetric
+      // G + G^T commutes (bitwise symmetric); G - G^T is bitwise antisymmetric
+      y[o] = hii[o] * x[o] + (odd ? G[o] - tile[tx][k] : G[o] + tile[tx][k]) * s;
     }
   }
 }
@@ -714,9 +722,10 @@ __global__ void __launch_bounds__(256) k_split_f16(size_t n, const float *__rest
   }
 }
 
-// v = (v + v^T)/2 on an n x n row-major matrix; block (bx, by), bx <= by,
-// owns the tile pair (by, bx) / (bx, by).
-__global__ void __launch_bounds__(256) k_symmetrize(int n, double *__restrict__ v) {
+// v = (v + p v^T)/2 on an n x n row-major matrix, p = +1 (even sector) or
+// -1 (odd sector: the diagonal becomes 0); block (bx, by), bx <= by, owns
+// the tile pair (by, bx) / (bx, by).
+__global__ void __launch_bounds__(256) k_symmetrize(int n, double *__restrict__ v, double p) {
   if (blockIdx.x > blockIdx.y) return;
   __shared__ double A[64][65], B[64][65];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -727,13 +736,14 @@ __global__ void __launch_bounds__(256) k_symmetrize(int n, double *__restrict__ 
   }
   __syncthreads();
   for (int k = ty; k < 64; k += 4) {
-    if (R0 + k < n && C0 + tx < n) v[(size_t)(R0 + k) * n + C0 + tx] = 0.5 * (A[k][tx] + B[tx][k]);
-    if (C0 + k < n && R0 + tx < n) v[(size_t)(C0 + k) * n + R0 + tx] = 0.5 * (B[k][tx] + A[tx][k]);
+    if (R0 + k < n && C0 + tx < n) v[(size_t)(R0 + k) * n + C0 + tx] = 0.5 * (A[k][tx] + p * B[tx][k]);
+    if (C0 + k < n && R0 + tx < n) v[(size_t)(C0 + k) * n + R0 + tx] = 0.5 * (B[k][tx] + p * A[tx][k]);
   }
 }
 
-// part[2*b] = max |v - v^T| and part[2*b+1] = max |v| over tile pair b.
-__global__ void __launch_bounds__(256) k_sym_defect(int n, const double *__restrict__ v, double *__restrict__ part) {
+// part[2*b] = max |v - p v^T| and part[2*b+1] = max |v| over tile pair b.
+__global__ void __launch_bounds__(256) k_sym_defect(int n, const double *__restrict__ v, double p,
+                                                    double *__restrict__ part) {
   __shared__ double A[64][65];
   __shared__ double red[2][256];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -745,7 +755,7 @@ __global__ void __launch_bounds__(256) k_sym_defect(int n, const double *__restr
   for (int k = ty; k < 64; k += 4)
     if (R0 + k < n && C0 + tx < n) {
       const double a = v[(size_t)(R0 + k) * n + C0 + tx];
-      d = fmax(d, fabs(a - A[tx][k]));
+      d = fmax(d, fabs(a - p * A[tx][k]));
       m = fmax(m, fabs(a));
     }
   red[0][threadIdx.x] = d;
@@ -827,7 +837,7 @@ static void launch_mixed_ell(Context &c, const T *eri, const T *x, T *y, size_t 
   const unsigned nsplit = cdiv(nslots, 1024 * NT);
   hipLaunchKernelGGL(fn, dim3((unsigned)c.na, nsplit), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
                      nslots, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell, c.B.perm, c.B.sl_off,
-                     c.B.sl_len, eri, x, y);
+                     c.B.sl_len, eri, x, y, (SYM && c.sym_par < 0) ? 1 : 0);
 }
 
 // EB pairs staged per round, NT slots per thread (1, 2, 4 or 8); false when
@@ -863,7 +873,7 @@ static void launch_mixed_parts(Context &c, const T *eri, const T *x, T *y, size_
   }
   hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
                      c.B.nslice * 64, c.B.nparts, c.B.part_h, c.A.s_off, c.A.s_ent, c.A.str, c.B.str,
-                     pt, c.B.perm, eri, x, y);
+                     pt, c.B.perm, eri, x, y, (SYM && c.sym_par < 0) ? 1 : 0);
 }
 
 // Ket row staged in parts: smallest part count (<= 8) whose piece and ERI row fit.
@@ -1340,7 +1350,7 @@ static void sym_same_split(Context &c, const float *hii, const float *x, float *
                       rocblas_gemm_algo_standard, 0, 0) != rocblas_status_success)
     throw Error(3, "split-FP16 same-spin GEMM failed");
   hipLaunchKernelGGL((k_sym_combine<float>), dim3(cdiv(n, 64), cdiv(n, 64)), dim3(256), 0, c.stream, n, hii, x,
-                     (const float *)G, y, (const float *)c.split_inv);
+                     (const float *)G, y, (const float *)c.split_inv, c.sym_par < 0 ? 1 : 0);
   check_launch("k_sym_combine");
 }
 
@@ -1353,7 +1363,7 @@ static void sym_same_gemm(Context &c, const SigmaOperands<T> &o, const T *x, T *
   if (gemm(blas(c), n, n, n, x, n, Ha, n, o.T1, n, T(0)) != rocblas_status_success)
     throw Error(3, "symmetric same-spin GEMM failed");
   hipLaunchKernelGGL((k_sym_combine<T>), dim3(cdiv(n, 64), cdiv(n, 64)), dim3(256), 0, c.stream, n, o.hii, x,
-                     (const T *)o.T1, y, (const float *)nullptr);
+                     (const T *)o.T1, y, (const float *)nullptr, c.sym_par < 0 ? 1 : 0);
   check_launch("k_sym_combine");
 }
 
@@ -1682,6 +1692,7 @@ static void sym_core_f32(Context &c) {
     sigma_sym_impl<float>(c, c.f32.x, c.f32.y);
     return;
   }
+  if (c.g_sym32 && c.g_sym32_par != c.sym_par) graphs_release(c);  // captured for the other sector
   if (c.g_sym32) {
     ProfScope ps(c, "sym_graph_f32");
     SBDX_HIP(hipGraphLaunch(c.g_sym32, c.stream));
@@ -1704,6 +1715,7 @@ static void sym_core_f32(Context &c) {
   SBDX_HIP(hipStreamEndCapture(c.stream, &graph));
   SBDX_HIP(hipGraphInstantiate(&c.g_sym32, graph, nullptr, nullptr, 0));
   SBDX_HIP(hipGraphDestroy(graph));
+  c.g_sym32_par = c.sym_par;
 }
 
 void graphs_release(Context &c) {
@@ -1721,12 +1733,14 @@ void sigma_sym_f32(Context &c, const double *x, double *y) {
   check_launch("k_cast");
 }
 
-// max |v - v^T| / max |v| of an na x na vector (0 for v = 0)
-double sym_defect(Context &c, const double *v) {
+// max |v - p v^T| / max |v| of an na x na vector (0 for v = 0); p = +1
+// measures the distance from the even sector, p = -1 from the odd one
+double sym_defect(Context &c, const double *v, int p) {
   const int n = (int)c.na;
   const unsigned t = cdiv(n, 64);
   TmpBuf part(sizeof(double) * 2 * t * t);
-  hipLaunchKernelGGL(k_sym_defect, dim3(t, t), dim3(256), 0, c.stream, n, v, (double *)part.p);
+  hipLaunchKernelGGL(k_sym_defect, dim3(t, t), dim3(256), 0, c.stream, n, v, p < 0 ? -1.0 : 1.0,
+                     (double *)part.p);
   check_launch("k_sym_defect");
   std::vector<double> h((size_t)2 * t * t);
   SBDX_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
@@ -1739,9 +1753,9 @@ double sym_defect(Context &c, const double *v) {
   return m > 0.0 ? d / m : 0.0;
 }
 
-void symmetrize(Context &c, double *v) {
+void symmetrize(Context &c, double *v, int p) {
   const unsigned t = cdiv(c.na, 64);
-  hipLaunchKernelGGL(k_symmetrize, dim3(t, t), dim3(256), 0, c.stream, (int)c.na, v);
+  hipLaunchKernelGGL(k_symmetrize, dim3(t, t), dim3(256), 0, c.stream, (int)c.na, v, p < 0 ? -1.0 : 1.0);
   check_launch("k_symmetrize");
 }
 

@@ -123,7 +123,10 @@ __global__ void __launch_bounds__(256) k_string_energy(const uint64_t *__restric
 
 // hii[ia*nb+ib] = I0 + e1a[ia] + e1b[ib] + sum_{r in beta(ib)} jsum_a[ia][r]
 // (ZeroExcite, chemistry/basic/determinants.h:434-457, split by spin).
-__global__ void __launch_bounds__(256) k_diag(int nb, int nchunk, int L, double I0,
+// canon != 0 (equal string sets, shared tables): element (ia, ib) with
+// ib < ia is evaluated as (ib, ia), so hii is bitwise symmetric and the
+// spin-flip sectors (x = +-x^T) stay exact under hii * x.
+__global__ void __launch_bounds__(256) k_diag(int nb, int nchunk, int L, double I0, int canon,
                                               const uint64_t *__restrict__ bstr,
                                               const double *__restrict__ e1a,
                                               const double *__restrict__ e1b,
@@ -132,15 +135,16 @@ __global__ void __launch_bounds__(256) k_diag(int nb, int nchunk, int L, double 
   const int ia = blockIdx.x / nchunk;
   const int ib = (blockIdx.x % nchunk) * blockDim.x + threadIdx.x;
   if (ib >= nb) return;
-  const double *j = jsa + (size_t)ia * L;
-  uint64_t occ = bstr[ib];
+  const bool swap = canon && ib < ia;
+  const double *j = jsa + (size_t)(swap ? ib : ia) * L;
+  uint64_t occ = bstr[swap ? ia : ib];
   double acc = 0.0;
   while (occ) {
     const int r = __ffsll((unsigned long long)occ) - 1;
     occ &= occ - 1;
     acc += j[r];
   }
-  hii[(size_t)ia * nb + ib] = I0 + e1a[ia] + e1b[ib] + acc;
+  hii[(size_t)ia * nb + ib] = I0 + (e1a[ia] + e1b[ib]) + acc;
 }
 
 // out = S * in, S(ia,ib) = (-1)^#{(p in alpha, r in beta): r < p}: the sign
@@ -435,7 +439,8 @@ void compute_diag(Context &c) {
   const unsigned nchunk = cdiv(c.nb, 256);
   ProfScope ps(c, "diag");
   hipLaunchKernelGGL(k_diag, dim3((unsigned)(c.na * nchunk)), dim3(256), 0, c.stream, (int)c.nb,
-                     (int)nchunk, c.L, c.I0, c.B.str, c.A.e1, c.B.e1, c.A.jsum, c.hii);
+                     (int)nchunk, c.L, c.I0, (c.B.alias && c.na == c.nb) ? 1 : 0, c.B.str, c.A.e1, c.B.e1,
+                     c.A.jsum, c.hii);
   check_launch("k_diag");
 }
 

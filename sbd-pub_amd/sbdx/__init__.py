@@ -44,7 +44,7 @@ ABI_SYMBOLS = [
     "sbdx_mlp_forward", "sbdx_mlp_gradient", "sbdx_mlp_train", "sbdx_mlp_record", "sbdx_mlp_collect",
     "sbdx_set_preconditioner", "sbdx_precond_stats", "sbdx_set_verbose",
     "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
-    "sbdx_mixed_staging", "sbdx_split_gemm", "sbdx_set_mixed_form", "sbdx_fp64_switch_step", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry",
+    "sbdx_mixed_staging", "sbdx_split_gemm", "sbdx_set_mixed_form", "sbdx_fp64_switch_step", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry", "sbdx_sector_stats",
     "sbdx_sigma_sym", "sbdx_mlp_apply", "sbdx_set_coarse_start", "sbdx_coarse_stats",
     "sbdx_set_graphs",
 ]
@@ -96,6 +96,7 @@ def lib():
     L.sbdx_sigma_block.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
     L.sbdx_symmetric_eig.argtypes = [C.c_int, _dp, _dp, _dp]
     L.sbdx_set_symmetry.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_sector_stats.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p, C.c_void_p]
     L.sbdx_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_set_coarse_start.argtypes = [C.c_void_p, C.c_size_t, C.c_double]
     L.sbdx_coarse_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double),
@@ -265,10 +266,25 @@ class TPBOperator:
         k Ritz vectors, exact FP64 residual at every restart."""
         _check(lib().sbdx_set_restart(self.h, int(keep)))
 
-    def set_symmetry(self, on):
-        """Spin-flip symmetric sigma in the thick-restart solves (equal string
-        sets, symmetric start vector)."""
-        _check(lib().sbdx_set_symmetry(self.h, 1 if on else 0))
+    def set_symmetry(self, mode):
+        """Spin-flip sectors of the thick-restart solves (equal string sets):
+        0/False general form, 1/True/"even" the even sector (symmetric start
+        vector), 2/"odd" the odd sector (antisymmetric start vector),
+        3/"lowest" both sectors, the lower result returned."""
+        names = {"off": 0, "even": 1, "odd": 2, "lowest": 3}
+        m = names[mode] if isinstance(mode, str) else (1 if mode is True else int(mode))
+        _check(lib().sbdx_set_symmetry(self.h, m))
+
+    def sector_stats(self):
+        """Per sector of the last thick solve: dict with energies [even, odd],
+        products, states (0 not run, 1 converged, 2 not converged, 3 odd stage
+        stopped above the even result) and chosen (1 even, 2 odd, 0 general)."""
+        e = np.zeros(2)
+        p = (C.c_int * 2)()
+        st = (C.c_int * 2)()
+        ch = C.c_int(0)
+        _check(lib().sbdx_sector_stats(self.h, e, p, st, C.byref(ch)))
+        return dict(energies=e.tolist(), products=list(p), states=list(st), chosen=ch.value)
 
     def sigma_sym(self, W, fp32=False):
         W = np.ascontiguousarray(W, dtype=np.float64)

@@ -38,7 +38,7 @@ CASES = {
     "n2_3em4": dict(fcidump=os.path.join(DATA, "n2_fcidump.txt.gz"),
                     strings=os.path.join(DATA, "n2-3em4-alpha.txt"), nb=10, rdm=0),
     "syn10": dict(workload=dict(norb=10, nelec_half=3, nstr=80, nchol=10, seed=77), nb=12),
-    "tiny16": dict(workload="tiny16", nb=12, max_it=400),
+    "tiny16": dict(workload="tiny16", nb=12, max_it=3),
 }
 
 SEED = 7

@@ -9,13 +9,18 @@ import pytest
 
 from conftest import GOLDEN, molecule
 
-CASES = ["h2o_1em3", "n2_1em3", "n2_3em4", "syn10"]
-# Cases whose Davidson result is compared.  syn10's ground state has exactly
-# zero Hartree-Fock weight (another spin-symmetry sector): from the HF start
-# vector the reference reaches it only through rounding noise after ~250
-# steps, so its solver result is not a reproducible target; its tables,
-# diagonal and sigma still are.
-SOLVE = {"h2o_1em3", "n2_1em3", "n2_3em4"}
+CASES = ["h2o_1em3", "n2_1em3", "n2_3em4", "syn10", "tiny16"]
+# Cases whose Davidson result is compared, with (nb, max_it).  syn10's
+# ground state has exactly zero Hartree-Fock weight (another spin-symmetry
+# sector): from the HF start vector the reference reaches it only through
+# rounding noise after ~250 steps, so its solver result is not a
+# reproducible target; its tables, diagonal and sigma still are.  tiny16
+# (norb 16, 5 electrons per spin, 500 x 500 strings) has its ground state in
+# the odd spin-flip sector too, and the HF vector is a poor start for the even
+# one (weight 3.5e-4): the reference's Davidson is still far from either
+# after 3 restarts of 12 steps, and that deterministic 36-step trajectory,
+# its energy <W|HW> and W are the fixture.
+SOLVE = {"h2o_1em3": (10, 100), "n2_1em3": (10, 100), "n2_3em4": (10, 100), "tiny16": (12, 3)}
 
 
 def load(name):
@@ -81,14 +86,15 @@ def test_oracle_matches_reference(oracle_mod, name):
     hii = orc.diag()
 
     def dav():
-        r = orc.davidson(nb=12 if name == "syn10" else 10, eps=1e-8, max_it=100, hii=hii)
+        nb, max_it = SOLVE[name]
+        r = orc.davidson(nb=nb, eps=1e-8, max_it=max_it, hii=hii)
         return r["E"], r["W"], r["steps"]
 
     def rdm(W):
         o1, o2 = orc.rdm(W)
         return o1.ravel(), o2.ravel()
 
-    solve = name in SOLVE and name != "n2_3em4"  # keep the CPU suite short
+    solve = name in ("h2o_1em3", "n2_1em3")  # keep the CPU suite short
     check_all(g, orc.table, lambda: hii, lambda v: orc.sigma(v, hii), dav if solve else None,
               rdm if solve else None)
 
@@ -103,8 +109,9 @@ def test_gpu_matches_reference(name):
     op = sbdx.TPBOperator.from_instance(inst)
 
     def dav():
-        E, W, steps, conv = sbdx.davidson(op, nb=12 if name == "syn10" else 10, eps=1e-8, max_it=100)
-        assert conv
+        nb, max_it = SOLVE[name]
+        E, W, steps, conv = sbdx.davidson(op, nb=nb, eps=1e-8, max_it=max_it)
+        assert conv or name == "tiny16"
         return op.energy(W), W, steps
 
     check_all(g, op.table, op.make_diag, op.sigma, dav if name in SOLVE else None,

@@ -77,7 +77,8 @@ def test_edge_norb64_exchange_sectors(oracle_mod):
     rounding noise.  The general mixed solve from a random start reaches the
     oracle's ground energy; the symmetric mode, restricted to the even sector
     that holds the HF vector, finds that sector's lowest state, the same in
-    mixed precision and in FP64, above the ground state."""
+    mixed precision and in FP64, above the ground state; with both sectors
+    the symmetric mode returns the ground state."""
     import sbdx
     inst = _norb64()
     print("[edge] norb64 exchange sectors", flush=True)
@@ -99,5 +100,16 @@ def test_edge_norb64_exchange_sectors(oracle_mod):
         assert conv
     assert abs(e_sym["mixed"] - e_sym["thick"]) < 1e-9
     assert e_sym["mixed"] > ref["E"] + 1e-3
+    # both sectors: the even solve from the HF vector, then the odd one from
+    # the antisymmetric pair on the lowest off-diagonal diagonal element; the
+    # odd result is the ground state
+    for solver in ("mixed", "thick"):
+        op.configure(solver)
+        op.set_symmetry("lowest")
+        op.set_vector(sbdx.basis_init_vector(op.ndet))
+        E, steps, conv = op.solve(max_it=800, nb=8, eps=1e-9)
+        st = op.sector_stats()
+        assert conv and abs(E - ref["E"]) < 1e-9, (solver, E, ref["E"], st)
+        assert st["chosen"] == 2 and abs(st["energies"][0] - e_sym["mixed"]) < 1e-9
     op.close()
     orc.close()
