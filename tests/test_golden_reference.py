@@ -17,10 +17,15 @@ CASES = ["h2o_1em3", "n2_1em3", "n2_3em4", "syn10", "tiny16"]
 # reproducible target; its tables, diagonal and sigma still are.  tiny16
 # (norb 16, 5 electrons per spin, 500 x 500 strings) has its ground state in
 # the odd spin-flip sector too, and the HF vector is a poor start for the even
-# one (weight 3.5e-4): the reference's Davidson is still far from either
-# after 3 restarts of 12 steps, and that deterministic 36-step trajectory,
-# its energy <W|HW> and W are the fixture.
-SOLVE = {"h2o_1em3": (10, 100), "n2_1em3": (10, 100), "n2_3em4": (10, 100), "tiny16": (12, 3)}
+# one (weight 3.5e-4): the reference's Davidson from it is chaotic -- the
+# preconditioner 1/(E0 - h_ii) meets diagonal elements close to E0, and
+# rounding differences change the trajectory (the oracle's departs from the
+# reference's at step 8, the GPU's summation order at step 5).  Its first
+# 4 steps, as the reference prints them (6 significant digits), are the
+# fixture; its ground state is pinned
+# by tests/golden/sectors_tiny16.json instead.
+SOLVE = {"h2o_1em3": (10, 100), "n2_1em3": (10, 100), "n2_3em4": (10, 100), "tiny16": (12, 1)}
+TRAJ_PREFIX = {"tiny16": 4}
 
 
 def load(name):
@@ -63,6 +68,16 @@ def check_all(g, tables, hii, sigma, davidson, rdm, tol_e=1e-10):
     assert abs(v @ s - g["sigma_vhv"]) < 1e-11 * max(1.0, abs(g["sigma_vhv"]))
     if davidson is None:
         return
+    if "prefix" in g:
+        trace = davidson()
+        k = int(g["prefix"])
+        ref = g["trajectory"][:k]
+        # the reference prints E0 and |r| with 6 significant digits: half a
+        # unit of the 6th digit, plus the float rounding of the parse
+        for col in (0, 1):
+            half = 0.5 * 10.0 ** (np.floor(np.log10(np.abs(ref[:, col]))) - 5)
+            assert np.all(np.abs(trace[:k, col] - ref[:, col]) <= half * (1 + 1e-9)), (col, trace[:k], ref)
+        return
     E, W, steps = davidson()
     assert abs(E - float(g["energy"])) < tol_e, (E, float(g["energy"]))
     traj = g["trajectory"]
@@ -88,13 +103,17 @@ def test_oracle_matches_reference(oracle_mod, name):
     def dav():
         nb, max_it = SOLVE[name]
         r = orc.davidson(nb=nb, eps=1e-8, max_it=max_it, hii=hii)
+        if name in TRAJ_PREFIX:
+            return r["trace"]
         return r["E"], r["W"], r["steps"]
 
     def rdm(W):
         o1, o2 = orc.rdm(W)
         return o1.ravel(), o2.ravel()
 
-    solve = name in ("h2o_1em3", "n2_1em3")  # keep the CPU suite short
+    solve = name in ("h2o_1em3", "n2_1em3", "tiny16")  # keep the CPU suite short
+    if name in TRAJ_PREFIX:
+        g["prefix"] = TRAJ_PREFIX[name]
     check_all(g, orc.table, lambda: hii, lambda v: orc.sigma(v, hii), dav if solve else None,
               rdm if solve else None)
 
@@ -111,8 +130,13 @@ def test_gpu_matches_reference(name):
     def dav():
         nb, max_it = SOLVE[name]
         E, W, steps, conv = sbdx.davidson(op, nb=nb, eps=1e-8, max_it=max_it)
-        assert conv or name == "tiny16"
+        if name in TRAJ_PREFIX:
+            return op.trace()
+        assert conv
         return op.energy(W), W, steps
+
+    if name in TRAJ_PREFIX:
+        g["prefix"] = TRAJ_PREFIX[name]
 
     check_all(g, op.table, op.make_diag, op.sigma, dav if name in SOLVE else None,
               op.correlation)
