@@ -346,6 +346,20 @@ int sbdx_energy(sbdx_handle_t h, const double *W, double *energy);
  * overwritten.
  */
 int sbdx_rdm(sbdx_handle_t h, const double *W, double *one, double *two);
+/* The RDMs of sbdx_rdm with device pointers, allocating nothing (no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory: code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  d_W (n_det doubles, reference convention), d_one (2*L*L) and
+ * d_two (4*L^4) are device buffers of the handle's device, which must be
+ * current; the outputs are overwritten and the call returns when they are
+ * complete.  Working memory comes from a static device array of 128 MiB per
+ * device (held from the first call on that device until the process ends)
+ * and the handle's own buffers: n_alpha + n_beta + n_alpha*L + L^4 doubles
+ * must fit in it with room for 64 pair-table entries (the pair tables are
+ * processed in chunks), else SBDX_ERR_ARG names the sizes (norb 58 at
+ * 20000 x 20000 strings fits).  Calls on one device share the array: calls
+ * that overlap on one device, from any handle or thread, race. */
+int sbdx_rdm_device_noalloc(sbdx_handle_t h, const double *d_W, double *d_one, double *d_two);
 
 /*
  * Excitation tables as the reference's TaskHelpers lists

@@ -823,6 +823,17 @@ int sbdx_rdm(sbdx_handle_t h, const double *W, double *one, double *two) {
   SBDX_GUARD_END
 }
 
+int sbdx_rdm_device_noalloc(sbdx_handle_t h, const double *d_W, double *d_one, double *d_two) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (!d_W || !d_one || !d_two) throw Error(SBDX_ERR_ARG, "null pointer");
+  // c.vout and c.T1 are the handle's own buffers, allocated by sbdx_create
+  sign_convert_noprof(c, d_W, c.vout);
+  compute_rdm_noalloc(c, c.vout, d_one, d_two);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
 int sbdx_table_count(sbdx_handle_t h, int spin, int kind, size_t *count) {
   SBDX_GUARD_BEGIN
   Context &c = ctx_of(h);

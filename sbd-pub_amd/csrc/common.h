@@ -267,6 +267,7 @@ void build_singles_ell(Context &c, SpinTables &t);
 void build_singles_ell_parts(Context &c, SpinTables &t, int nparts);
 void compute_diag(Context &c);
 void sign_convert(Context &c, const double *in, double *out);      // out = S*in
+void sign_convert_noprof(Context &c, const double *in, double *out);  // out = S*in, no profiling events
 void sign_convert_add(Context &c, const double *in, double *out);  // out += S*in
 
 // sigma.hip
@@ -304,6 +305,9 @@ double device_dot(Context &c, const double *x, const double *y, size_t n);
 
 // rdm.hip
 void compute_rdm(Context &c, const double *Wp, double *one, double *two);
+// the same with working memory from the static scratch array of rdm.hip
+// (no allocation; errors when the instance needs more than it holds)
+void compute_rdm_noalloc(Context &c, const double *Wp, double *one, double *two);
 
 // mlp.hip
 int mlp_nparam();

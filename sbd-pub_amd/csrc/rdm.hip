@@ -12,6 +12,9 @@
 //   TwoDiff   -> per same-spin double pair: T; mixed: T[pq][rs] summed over
 //                all (alpha single, beta single) pairs, then placed with the
 //                reference's spin-orbital ordering rules.
+#include <mutex>
+#include <string>
+
 #include "common.h"
 
 namespace sbdx {
@@ -123,7 +126,8 @@ __global__ void k_rdm_diag_scatter(int L, int na, int nb, const uint64_t *__rest
 // Single pairs of one spin: block per pair e; rows of w are this spin's strings.
 //   out[e*(L+1)]     = s * sum_m w[I,m] w[J,m]
 //   out[e*(L+1)+1+k] = s * sum_m w[I,m] w[J,m] n_k(m)    (other-spin strings)
-__global__ void __launch_bounds__(256) k_rdm_single_pairs(int n, int M, int L,
+// Pairs e0 <= e < e1 only; out holds pair e at (e - e0).
+__global__ void __launch_bounds__(256) k_rdm_single_pairs(int n, int M, int L, uint64_t e0, uint64_t e1,
                                                           const uint64_t *__restrict__ s_off,
                                                           const uint64_t *__restrict__ s_ent,
                                                           const uint64_t *__restrict__ ostr,
@@ -131,8 +135,7 @@ __global__ void __launch_bounds__(256) k_rdm_single_pairs(int n, int M, int L,
                                                           double *__restrict__ out) {
   __shared__ double sh[256];
   __shared__ double acc[64];
-  const uint64_t ns = s_off[n];
-  for (uint64_t e = blockIdx.x; e < ns; e += gridDim.x) {
+  for (uint64_t e = e0 + blockIdx.x; e < e1; e += gridDim.x) {
   const int I = row_of(s_off, n, e);
   const uint64_t ent = s_ent[e];
   const size_t J = (uint32_t)(ent & 0xffffffffull);
@@ -153,15 +156,15 @@ __global__ void __launch_bounds__(256) k_rdm_single_pairs(int n, int M, int L,
     }
   }
   const double t = blk_sum256(s, sh);
-  double *o = out + e * (size_t)(L + 1);
+  double *o = out + (e - e0) * (size_t)(L + 1);
   if (threadIdx.x == 0) o[0] = sg * t;
   if (threadIdx.x < L) o[1 + threadIdx.x] = sg * acc[threadIdx.x];
   __syncthreads();
   }
 }
 
-// Double pairs of one spin: out[e] = sign * sum_m w[I,m] w[J,m]
-__global__ void __launch_bounds__(256) k_rdm_double_pairs(int n, int M,
+// Double pairs of one spin, e0 <= e < e1: out[e - e0] = sign * sum_m w[I,m] w[J,m]
+__global__ void __launch_bounds__(256) k_rdm_double_pairs(int n, int M, uint64_t e0, uint64_t e1,
                                                           const uint64_t *__restrict__ d_off,
                                                           const uint32_t *__restrict__ d_ket,
                                                           const uint32_t *__restrict__ d_orb,
@@ -169,8 +172,7 @@ __global__ void __launch_bounds__(256) k_rdm_double_pairs(int n, int M,
                                                           const double *__restrict__ w,
                                                           double *__restrict__ out) {
   __shared__ double sh[256];
-  const uint64_t nd = d_off[n];
-  for (uint64_t e = blockIdx.x; e < nd; e += gridDim.x) {
+  for (uint64_t e = e0 + blockIdx.x; e < e1; e += gridDim.x) {
   const int I = row_of(d_off, n, e);
   const size_t J = d_ket[e];
   double s = 0.0;
@@ -182,25 +184,26 @@ __global__ void __launch_bounds__(256) k_rdm_double_pairs(int n, int M,
     const uint64_t bra = str[I];
     double sg = parity1r(bra, i, a) * parity1r(bra, j, b);
     if (a > j || b < i) sg = -sg;
-    out[e] = sg * t;
+    out[e - e0] = sg * t;
   }
   }
 }
 
-// OneDiff scatter for the single pairs of one spin (si = spin).
-__global__ void k_rdm_single_scatter(int n, int L, int spin, size_t ns,
+// OneDiff scatter for the single pairs e0 <= e < e1 of one spin (si =
+// spin); T holds pair e at (e - e0).
+__global__ void k_rdm_single_scatter(int n, int L, int spin, uint64_t e0, uint64_t e1,
                                      const uint64_t *__restrict__ s_off,
                                      const uint64_t *__restrict__ s_ent,
                                      const uint64_t *__restrict__ str,
                                      const double *__restrict__ T, double *__restrict__ one,
                                      double *__restrict__ two) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ns) return;
+  const uint64_t e = e0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= e1) return;
   const size_t L2 = (size_t)L * L, L4 = L2 * L2, L3 = L2 * L;
   const int I = row_of(s_off, n, e);
   const uint32_t pq = (uint32_t)((s_ent[e] >> 32) & 0x7fffffffull);
   const int p = pq / L, q = pq % L;  // p = bra-only (cr), q = ket-only (an)
-  const double *t = T + e * (size_t)(L + 1);
+  const double *t = T + (e - e0) * (size_t)(L + 1);
   const double t0 = t[0];
   atomicAdd(&one[spin * L2 + p + (size_t)L * q], t0);
   const size_t bs = (size_t)(3 * spin) * L4;  // same-spin block
@@ -224,15 +227,17 @@ __global__ void k_rdm_single_scatter(int n, int L, int spin, size_t ns,
   }
 }
 
-// TwoDiff scatter for same-spin double pairs (block 0 alpha, 3 beta).
-__global__ void k_rdm_double_scatter(int L, int spin, size_t nd, const uint32_t *__restrict__ d_orb,
+// TwoDiff scatter for same-spin double pairs e0 <= e < e1 (block 0 alpha,
+// 3 beta); T holds pair e at (e - e0).
+__global__ void k_rdm_double_scatter(int L, int spin, uint64_t e0, uint64_t e1,
+                                     const uint32_t *__restrict__ d_orb,
                                      const double *__restrict__ T, double *__restrict__ two) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nd) return;
+  const uint64_t e = e0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= e1) return;
   const size_t L2 = (size_t)L * L, L4 = L2 * L2;
   const uint32_t o = d_orb[e];
   const size_t i = o & 255, j = (o >> 8) & 255, a = (o >> 16) & 255, b = (o >> 24) & 255;
-  const double t = T[e];
+  const double t = T[e - e0];
   const size_t bs = (size_t)(3 * spin) * L4;
   atomicAdd(&two[bs + a + L * b + L2 * (i + L * j)], t);
   atomicAdd(&two[bs + b + L * a + L2 * (j + L * i)], t);
@@ -323,74 +328,170 @@ static void launch_rdm_mixed(Context &c, const double *w, double *Tm, size_t lds
   check_launch("k_rdm_mixed");
 }
 
-void compute_rdm(Context &c, const double *Wp, double *one, double *two) {
+// p[0, n) = 0 (a fill kernel rather than hipMemsetAsync, so the work is a
+// plain kernel sequence in every path)
+__global__ void __launch_bounds__(256) k_rdm_zero(size_t n, double *__restrict__ p) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0.0;
+}
+
+static void zero(Context &c, double *p, size_t n) {
+  hipLaunchKernelGGL(k_rdm_zero, dim3((unsigned)std::min<size_t>(cdiv(n, 256), 4096)), dim3(256), 0, c.stream,
+                     n, p);
+  check_launch("k_rdm_zero");
+}
+
+// Working memory of one RDM evaluation: per-string sums, the mixed-term
+// accumulator Tm[K][K], and room for ts_cap single pairs ((L+1) values each)
+// and td_cap double pairs (the pair tables are processed in chunks of that
+// many entries).
+struct RdmWork {
+  double *rho_a, *rho_b, *mrow, *Tm, *Ts, *Td;
+  size_t ts_cap, td_cap;
+};
+
+// Everything after the sign conversion: W' (alpha-first, device) to one and
+// two (device, overwritten).  Kernels only, on c.stream; synchronizes once.
+static void rdm_run(Context &c, const double *Wp, double *one, double *two, const RdmWork &wk) {
   const int L = c.L;
-  const size_t L2 = (size_t)L * L, L4 = L2 * L2;
+  const size_t L2 = (size_t)L * L, L4 = L2 * L2, K = (size_t)c.K;
   const int na = (int)c.na, nb = (int)c.nb;
-  ProfScope ps(c, "rdm");
-  SBDX_HIP(hipMemsetAsync(one, 0, sizeof(double) * 2 * L2, c.stream));
-  SBDX_HIP(hipMemsetAsync(two, 0, sizeof(double) * 4 * L4, c.stream));
+  const size_t lds = sizeof(double) * ((size_t)nb + K);
+  const int nt = (nb + 1023) / 1024;
+  if (lds > c.lds_limit || nt > 32)
+    throw Error(3, "rdm: beta string count too large for the LDS-staged mixed RDM kernel");
+  zero(c, one, 2 * L2);
+  zero(c, two, 4 * L4);
+  zero(c, wk.Tm, K * K);
   // W'^T (beta-major) for the beta-side sums
   transpose_into(c, Wp, c.T1, na, nb);
   // ZeroDiff
-  TmpBuf b_rho_a(sizeof(double) * na), b_rho_b(sizeof(double) * nb),
-      b_mrow(sizeof(double) * (size_t)na * L);
-  double *rho_a = (double *)b_rho_a.p, *rho_b = (double *)b_rho_b.p, *mrow = (double *)b_mrow.p;
-  hipLaunchKernelGGL(k_rdm_diag_rows, dim3(na), dim3(256), 0, c.stream, nb, L, c.B.str, Wp, rho_a, mrow);
+  hipLaunchKernelGGL(k_rdm_diag_rows, dim3(na), dim3(256), 0, c.stream, nb, L, c.B.str, Wp, wk.rho_a, wk.mrow);
   check_launch("k_rdm_diag_rows");
-  hipLaunchKernelGGL(k_rdm_diag_rows, dim3(nb), dim3(256), 0, c.stream, na, L, c.A.str, c.T1, rho_b,
+  hipLaunchKernelGGL(k_rdm_diag_rows, dim3(nb), dim3(256), 0, c.stream, na, L, c.A.str, c.T1, wk.rho_b,
                      (double *)nullptr);
   check_launch("k_rdm_diag_rows");
   hipLaunchKernelGGL(k_rdm_diag_scatter, dim3(L), dim3(64), 0, c.stream, L, na, nb, c.A.str, c.B.str,
-                     rho_a, rho_b, mrow, one, two);
+                     wk.rho_a, wk.rho_b, wk.mrow, one, two);
   check_launch("k_rdm_diag_scatter");
-  // same-spin singles and doubles, alpha (rows of W') then beta (rows of W'^T)
+  // same-spin singles and doubles, alpha (rows of W') then beta (rows of
+  // W'^T), in chunks of the pair tables
   for (int spin = 0; spin < 2; spin++) {
     const SpinTables &t = spin ? c.B : c.A;
     const SpinTables &o = spin ? c.A : c.B;
     const double *w = spin ? c.T1 : Wp;
     const int M = spin ? na : nb;
-    if (t.ns > 0) {
-      TmpBuf bts(sizeof(double) * t.ns * (L + 1));
-      double *Ts = (double *)bts.p;
-      hipLaunchKernelGGL(k_rdm_single_pairs, dim3((unsigned)std::min<size_t>(t.ns, 1u << 20)), dim3(256), 0, c.stream, t.n, M, L,
-                         t.s_off, t.s_ent, o.str, w, Ts);
+    for (size_t e0 = 0; e0 < t.ns; e0 += wk.ts_cap) {
+      const size_t e1 = std::min(t.ns, e0 + wk.ts_cap);
+      hipLaunchKernelGGL(k_rdm_single_pairs, dim3((unsigned)std::min<size_t>(e1 - e0, 1u << 20)), dim3(256), 0,
+                         c.stream, t.n, M, L, (uint64_t)e0, (uint64_t)e1, t.s_off, t.s_ent, o.str, w, wk.Ts);
       check_launch("k_rdm_single_pairs");
-      hipLaunchKernelGGL(k_rdm_single_scatter, dim3(cdiv(t.ns, 256)), dim3(256), 0, c.stream, t.n, L,
-                         spin, t.ns, t.s_off, t.s_ent, t.str, Ts, one, two);
+      hipLaunchKernelGGL(k_rdm_single_scatter, dim3(cdiv(e1 - e0, 256)), dim3(256), 0, c.stream, t.n, L, spin,
+                         (uint64_t)e0, (uint64_t)e1, t.s_off, t.s_ent, t.str, wk.Ts, one, two);
       check_launch("k_rdm_single_scatter");
-      SBDX_HIP(hipStreamSynchronize(c.stream));
     }
-    if (t.nd > 0) {
-      TmpBuf btd(sizeof(double) * t.nd);
-      double *Td = (double *)btd.p;
-      hipLaunchKernelGGL(k_rdm_double_pairs, dim3((unsigned)std::min<size_t>(t.nd, 1u << 20)), dim3(256), 0, c.stream, t.n, M,
-                         t.d_off, t.d_ket, t.d_orb, t.str, w, Td);
+    for (size_t e0 = 0; e0 < t.nd; e0 += wk.td_cap) {
+      const size_t e1 = std::min(t.nd, e0 + wk.td_cap);
+      hipLaunchKernelGGL(k_rdm_double_pairs, dim3((unsigned)std::min<size_t>(e1 - e0, 1u << 20)), dim3(256), 0,
+                         c.stream, t.n, M, (uint64_t)e0, (uint64_t)e1, t.d_off, t.d_ket, t.d_orb, t.str, w,
+                         wk.Td);
       check_launch("k_rdm_double_pairs");
-      hipLaunchKernelGGL(k_rdm_double_scatter, dim3(cdiv(t.nd, 256)), dim3(256), 0, c.stream, L, spin,
-                         t.nd, t.d_orb, Td, two);
+      hipLaunchKernelGGL(k_rdm_double_scatter, dim3(cdiv(e1 - e0, 256)), dim3(256), 0, c.stream, L, spin,
+                         (uint64_t)e0, (uint64_t)e1, t.d_orb, wk.Td, two);
       check_launch("k_rdm_double_scatter");
-      SBDX_HIP(hipStreamSynchronize(c.stream));
     }
   }
   // mixed alpha-single x beta-single
-  const size_t K = (size_t)c.K;
-  TmpBuf btm(sizeof(double) * K * K);
-  double *Tm = (double *)btm.p;
-  SBDX_HIP(hipMemsetAsync(Tm, 0, sizeof(double) * K * K, c.stream));
-  const size_t lds = sizeof(double) * ((size_t)nb + K);
-  const int nt = (nb + 1023) / 1024;
-  if (lds > c.lds_limit || nt > 32)
-    throw Error(3, "rdm: beta string count too large for the LDS-staged mixed RDM kernel");
-  if (nt <= 1) launch_rdm_mixed<1>(c, Wp, Tm, lds);
-  else if (nt <= 2) launch_rdm_mixed<2>(c, Wp, Tm, lds);
-  else if (nt <= 4) launch_rdm_mixed<4>(c, Wp, Tm, lds);
-  else if (nt <= 8) launch_rdm_mixed<8>(c, Wp, Tm, lds);
-  else if (nt <= 16) launch_rdm_mixed<16>(c, Wp, Tm, lds);
-  else launch_rdm_mixed<32>(c, Wp, Tm, lds);
-  hipLaunchKernelGGL(k_rdm_mixed_place, dim3(cdiv(K * K, 256)), dim3(256), 0, c.stream, L, Tm, two);
+  if (nt <= 1) launch_rdm_mixed<1>(c, Wp, wk.Tm, lds);
+  else if (nt <= 2) launch_rdm_mixed<2>(c, Wp, wk.Tm, lds);
+  else if (nt <= 4) launch_rdm_mixed<4>(c, Wp, wk.Tm, lds);
+  else if (nt <= 8) launch_rdm_mixed<8>(c, Wp, wk.Tm, lds);
+  else if (nt <= 16) launch_rdm_mixed<16>(c, Wp, wk.Tm, lds);
+  else launch_rdm_mixed<32>(c, Wp, wk.Tm, lds);
+  hipLaunchKernelGGL(k_rdm_mixed_place, dim3(cdiv(K * K, 256)), dim3(256), 0, c.stream, L, wk.Tm, two);
   check_launch("k_rdm_mixed_place");
   SBDX_HIP(hipStreamSynchronize(c.stream));
+}
+
+void compute_rdm(Context &c, const double *Wp, double *one, double *two) {
+  ProfScope ps(c, "rdm");
+  const size_t L = (size_t)c.L, K = (size_t)c.K;
+  const size_t ns = std::max<size_t>(std::max(c.A.ns, c.B.ns), 1);
+  const size_t nd = std::max<size_t>(std::max(c.A.nd, c.B.nd), 1);
+  TmpBuf b_rho_a(sizeof(double) * c.na), b_rho_b(sizeof(double) * c.nb), b_mrow(sizeof(double) * c.na * L),
+      b_tm(sizeof(double) * K * K), b_ts(sizeof(double) * ns * (L + 1)), b_td(sizeof(double) * nd);
+  const RdmWork wk{(double *)b_rho_a.p, (double *)b_rho_b.p, (double *)b_mrow.p, (double *)b_tm.p,
+                   (double *)b_ts.p,    (double *)b_td.p,    ns,                  nd};
+  rdm_run(c, Wp, one, two, wk);
+}
+
+// ---------------------------------------------------------------------------
+// The same evaluation with working memory from a fixed static device array,
+// for callers that cannot allocate (sbdx_rdm_device_noalloc).  Each device
+// holds its own copy of `scratch` (kRdmScratchDoubles doubles = 128 MiB) from
+// the first use of this file's kernels until the process ends.  In host code
+// the array's name is only a stand-in: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is passed on.
+// Every call on one device shares the array, so calls that overlap on one
+// device (other threads, other handles) race; the array keeps the last
+// call's contents, and rdm_run clears what must start at zero.
+constexpr size_t kRdmScratchDoubles = size_t(16) << 20;
+static __device__ __align__(256) double scratch[kRdmScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+static double *scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) throw Error(3, "rdm (no-alloc): device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    SBDX_HIP(hipGetDevice(&cur));
+    if (cur != device) throw Error(1, "rdm (no-alloc): call with the handle's device current");
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error(2, std::string("rdm (no-alloc): hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (double *)p;
+  }
+  return g_scratch_ptr[device];
+}
+
+size_t rdm_noalloc_fixed_doubles(const Context &c) {
+  auto up = [](size_t n) { return (n + 31) & ~size_t(31); };  // 256-byte aligned regions
+  return up(c.na) + up(c.nb) + up(c.na * (size_t)c.L) + up((size_t)c.K * c.K);
+}
+
+void compute_rdm_noalloc(Context &c, const double *Wp, double *one, double *two) {
+  auto up = [](size_t n) { return (n + 31) & ~size_t(31); };
+  const size_t L = (size_t)c.L, K = (size_t)c.K;
+  const size_t fixed = rdm_noalloc_fixed_doubles(c);
+  // at least 64 single pairs and 64 double pairs per chunk
+  if (fixed + up(64 * (L + 1)) + up(64) > kRdmScratchDoubles) {
+    char msg[256];
+    snprintf(msg, sizeof msg,
+             "rdm (no-alloc): the instance needs %zu + %zu doubles of working memory, more than the %zu "
+             "(128 MiB) of the static scratch array",
+             fixed, up(64 * (L + 1)) + up(64), kRdmScratchDoubles);
+    throw Error(1, msg);
+  }
+  const int device = c.device;
+  double *base = scratch_address(device);
+  double *p = base;
+  RdmWork wk{};
+  wk.rho_a = p, p += up(c.na);
+  wk.rho_b = p, p += up(c.nb);
+  wk.mrow = p, p += up(c.na * L);
+  wk.Tm = p, p += up(K * K);
+  const size_t rem = kRdmScratchDoubles - (size_t)(p - base);
+  const size_t ns = std::max<size_t>(std::max(c.A.ns, c.B.ns), 1);
+  const size_t nd = std::max<size_t>(std::max(c.A.nd, c.B.nd), 1);
+  // half the rest for single pairs, the remainder for double pairs
+  wk.ts_cap = std::min(ns, std::max<size_t>(64, (rem / 2) / (L + 1)));
+  wk.Ts = p, p += up(wk.ts_cap * (L + 1));
+  wk.td_cap = std::min(nd, kRdmScratchDoubles - (size_t)(p - base));
+  wk.Td = p;
+  rdm_run(c, Wp, one, two, wk);
 }
 
 }  // namespace sbdx

@@ -452,6 +452,13 @@ void sign_convert(Context &c, const double *in, double *out) {
   check_launch("k_sign");
 }
 
+void sign_convert_noprof(Context &c, const double *in, double *out) {
+  const unsigned nchunk = cdiv(c.nb, 256);
+  hipLaunchKernelGGL(k_sign<false>, dim3((unsigned)(c.na * nchunk)), dim3(256), 0, c.stream,
+                     (int)c.nb, (int)nchunk, c.A.str, c.B.str, in, out);
+  check_launch("k_sign");
+}
+
 void sign_convert_add(Context &c, const double *in, double *out) {
   const unsigned nchunk = cdiv(c.nb, 256);
   ProfScope ps(c, "sign");

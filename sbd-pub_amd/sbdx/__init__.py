@@ -46,7 +46,7 @@ ABI_SYMBOLS = [
     "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
     "sbdx_mixed_staging", "sbdx_split_gemm", "sbdx_set_mixed_form", "sbdx_fp64_switch_step", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry", "sbdx_sector_stats",
     "sbdx_sigma_sym", "sbdx_mlp_apply", "sbdx_set_coarse_start", "sbdx_coarse_stats",
-    "sbdx_set_graphs",
+    "sbdx_set_graphs", "sbdx_rdm_device_noalloc",
 ]
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -96,6 +96,7 @@ def lib():
     L.sbdx_sigma_block.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
     L.sbdx_symmetric_eig.argtypes = [C.c_int, _dp, _dp, _dp]
     L.sbdx_set_symmetry.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_rdm_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.sbdx_sector_stats.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p, C.c_void_p]
     L.sbdx_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_set_coarse_start.argtypes = [C.c_void_p, C.c_size_t, C.c_double]
