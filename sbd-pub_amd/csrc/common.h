@@ -64,6 +64,15 @@ struct SpinTables {
   uint32_t *sl_len = nullptr;  // [nslice]
   int nslice = 0;
   bool has_ell = false;
+  // The same entries packed for the staged mixed kernel k_mixed_ell (same
+  // perm and slices): (K + ket) << 4 | pq << 19 | (sign<0) << 31, padding
+  // ((K + n) << 4); slice lengths sl_len2 rounded up to a multiple of 4 and
+  // each group of 4 positions stored lane-major (word u of lane l of group g
+  // at sl_off2 + 256 g + 4 l + u); needs K <= 4096 and K + n < 2^15.
+  uint32_t *ell2 = nullptr;
+  uint64_t *sl_off2 = nullptr;  // [nslice]
+  uint32_t *sl_len2 = nullptr;  // [nslice]
+  bool has_ell2 = false;
   // The same singles split by ket range into `nparts` parts of `part_h`
   // kets (ket stored relative to its part), same perm and slices; built on
   // demand when one ket row does not fit in LDS.

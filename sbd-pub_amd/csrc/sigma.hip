@@ -132,6 +132,9 @@ template <typename T> struct Pack<T, 1> { typedef T V; };
 template <typename T, int EB> __device__ __forceinline__ T &lane_of(typename Pack<T, EB>::V &v, int q) {
   return reinterpret_cast<T *>(&v)[q];
 }
+template <typename T, int EB> __device__ __forceinline__ T lane_c(const typename Pack<T, EB>::V &v, int q) {
+  return reinterpret_cast<const T *>(&v)[q];
+}
 // v with the sign bit sb (0 or 0x80000000) flipped in each of its EB values
 // (high word of a double): one XOR per value, so the product folds into an FMA.
 __device__ __forceinline__ float flip_sign(float v, uint32_t sb) {
@@ -153,30 +156,34 @@ template <typename T, int EB> __device__ __forceinline__ T hsum(const typename P
   return s;
 }
 
-// Mixed alpha-beta term, beta singles in sliced-ELL form (SpinTables::ell).
-// One block per alpha bra row I.  Each LDS round stages EB "pairs": for an
-// alpha single of I its ket row x[J,:] and sign-folded ERI row s_a*(pq|..);
-// after the singles, one spectator pair x[I,:] with the row
-// sum_{k in alpha(I)} (kk|..) (spectator_val).  The pairs are interleaved,
-// X[j][q] = x_q[j] and W[i][q] = v_q[i], so a sliced-ELL entry (ket j, index
-// i) costs two LDS reads of EB values each (4 to 16 bytes: gfx950 serves a
-// 16-byte read per half-wavefront at about the cycles of a 4-byte one) for
-// EB multiply-adds.  Index K of W is a zero for padding entries.  Thread
-// slot = tid + t*1024 walks beta row perm[slot]; slices are 64 rows of
-// similar length, so the trip count is uniform per wavefront and each entry
-// load is a coalesced 256-byte access shared by the EB staged pairs.  Per
-// pair the slot also adds the alpha single's dressing x[J,r] *
-// sum_{k in beta(r)} s_a (pq|kk) from the staged ERI row's diagonal (zero in
-// the spectator row).  A row's slots may be split over gridDim.y blocks of
-// 1024*NT slots each (fewer registers per thread, more resident waves, the
-// row staged once per block).
-// SYM (equal string sets, symmetric x): block p handles bra I = perm[p] and
-// only the slots s >= p of the length-sorted table, adding each value at
-// y[I,r] and, for s > p, at y[r,I] (r = perm[s]): every unordered pair
-// {I, r} is produced once, and element (a,b) is written by exactly one block
-// (a's when pos(b) >= pos(a), b's otherwise).  Blocks start at the slice of
-// position p and exit when their chunk lies past the table; p ascending
-// puts the longest rows first.
+// Mixed alpha-beta term, beta singles in the packed sliced-ELL form
+// (SpinTables::ell2).  One block per alpha bra row I.  Each LDS round stages
+// EB "pairs": for an alpha single of I its sign-folded ERI row s_a*(pq|..)
+// and ket row x[J,:]; after the singles, one spectator pair with the row
+// sum_{k in alpha(I)} (kk|..) (spectator_val) and x[I,:].  The pairs are
+// interleaved, W[i][q] = v_q[i] at LDS elements 0..K-1 and X[j][q] = x_q[j]
+// at elements K..K+nb-1, then one zero element (padding entries), so a
+// table word holds both byte offsets pre-shifted (entry = (K + ket) << 4 |
+// pq << 19 | sign << 31 for 16-byte elements) and one entry costs one mask,
+// one shift-and-mask, two LDS reads of EB values (4 to 32 bytes: gfx950
+// serves a 16-byte read per quarter-wavefront at about the cycles of a
+// 4-byte one), the EB-term dot product and one signed add.  Thread slot =
+// tid + t*1024 walks beta row perm[slot]; slices are 64 rows of similar
+// length, so the trip count is uniform per wavefront and each entry load is
+// a coalesced 256-byte access shared by the EB staged pairs.  Per pair the
+// slot also adds the alpha single's dressing x[J,r] * sum_{k in beta(r)} s_a
+// (pq|kk) from the staged ERI row's diagonal (zero in the spectator row).
+// A row's slots may be split over gridDim.y blocks of 1024*NT slots each
+// (fewer registers per thread, more resident waves, the row staged once per
+// block).
+// SYM (equal string sets, x = x^T or, odd != 0, x = -x^T): block p handles
+// bra I = perm[p] and only the slots s >= p of the length-sorted table,
+// adding each value at y[I,r] and, for s > p, +-value at y[r,I] (r =
+// perm[s]): every unordered pair {I, r} is produced once, and element (a,b)
+// is written by exactly one block (a's when pos(b) >= pos(a), b's
+// otherwise); the odd sector leaves the diagonal 0.  Blocks start at the
+// slice of position p and exit when their chunk lies past the table; p
+// ascending puts the longest rows first.
 template <typename T, int NT, int EB, bool SYM>
 __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int nslots,
                                                     const uint64_t *__restrict__ a_off,
@@ -191,20 +198,46 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
                                                     const T *__restrict__ x, T *__restrict__ y, int odd) {
   typedef typename Pack<T, EB>::V V;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  V *X = reinterpret_cast<V *>(lds_raw);  // nb ket values per pair
-  V *W = X + nb;                          // K + 1 ERI values per pair
+  V *W = reinterpret_cast<V *>(lds_raw);  // K ERI values per pair
+  V *X = W + K;                           // nb ket values per pair, then a zero
+  // byte offsets of an entry's two operands, from offsets pre-shifted for
+  // 16-byte elements
+  constexpr uint32_t vb = (uint32_t)sizeof(V);
+  auto xoff = [](uint32_t w) { return vb >= 16 ? (w & 0x7fff0u) * (vb / 16) : (w & 0x7fff0u) / (16 / vb); };
+  auto woff = [](uint32_t w) {
+    return vb >= 16 ? ((w >> 15) & 0xfff0u) * (vb / 16) : ((w >> 15) & 0xfff0u) / (16 / vb);
+  };
+  // the kernel has no static LDS, so the dynamic array starts at LDS
+  // address 0 (checked by the launcher) and an offset is an address
+  typedef __attribute__((address_space(3))) const V lds_v;
+  auto at = [](uint32_t off) { return *reinterpret_cast<lds_v *>((size_t)off); };
+  // sum_q w_q x_q
+  auto dot = [](const V &w, const V &xv) {
+    T t = lane_c<T, EB>(w, 0) * lane_c<T, EB>(xv, 0);
+#pragma unroll
+    for (int q = 1; q < EB; q++) t = fma(lane_c<T, EB>(w, q), lane_c<T, EB>(xv, q), t);
+    return t;
+  };
+  // one entry's contribution from its staged values: a += s * sum_q W_q X_q
+  // (one FMA with the signed ket value when EB == 1)
+  auto entry_vals = [&](uint32_t w, const V &wv, const V &xv, T &a) {
+    if (EB == 1)
+      a = fma(lane_c<T, EB>(wv, 0), flip_sign(lane_c<T, EB>(xv, 0), w & 0x80000000u), a);
+    else
+      a += flip_sign(dot(wv, xv), w & 0x80000000u);
+  };
   const int L1 = L + 1;
   const int pos = blockIdx.x;
   const int I = SYM ? (int)perm[pos] : blockIdx.x;
   const int base = (SYM ? (pos & ~63) : 0) + blockIdx.y * (1024 * NT);
   if (SYM && base >= nslots) return;  // uniform per block, before any barrier
   const int tid = base + threadIdx.x;
-  V acc[NT];
+  T acc[NT];
   uint64_t occ[NT];
   int rr[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) {
-    acc[t] = V(T(0));
+    acc[t] = T(0);
     const int slot = tid + t * 1024;
     const uint32_t r = slot < nslots ? perm[slot] : 0xffffffffu;
     rr[t] = r != 0xffffffffu ? (int)r : 0;
@@ -236,21 +269,19 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
       }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nb; i += 1024) {
+    for (int i = threadIdx.x; i <= nb; i += 1024) {
       V v;
 #pragma unroll
-      for (int q = 0; q < EB; q++) lane_of<T, EB>(v, q) = kind[q] ? xsrc[q][i] : T(0);
+      for (int q = 0; q < EB; q++) lane_of<T, EB>(v, q) = (kind[q] && i < nb) ? xsrc[q][i] : T(0);
       X[i] = v;
     }
-    for (int i = threadIdx.x; i <= K; i += 1024) {
+    for (int i = threadIdx.x; i < K; i += 1024) {
       V v;
 #pragma unroll
       for (int q = 0; q < EB; q++) {
         T w = T(0);
-        if (i < K) {
-          if (kind[q] == 1) w = sgn[q] * vsrc[q][i];
-          else if (kind[q] == 2) w = spectator_val(eri, occ_a, L1, K, i);
-        }
+        if (kind[q] == 1) w = sgn[q] * vsrc[q][i];
+        else if (kind[q] == 2) w = spectator_val(eri, occ_a, L1, K, i);
         lane_of<T, EB>(v, q) = w;
       }
       W[i] = v;
@@ -261,40 +292,43 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
       const int slot = tid + t * 1024;
       if (slot < nslots) {
         // a wavefront's 64 slots lie in one slice (slot bases are multiples
-        // of 64), so the slice's length and offset are wave-uniform
+        // of 64), so the slice's group count and offset are wave-uniform
         const int sl = __builtin_amdgcn_readfirstlane(slot >> 6);
-        const uint32_t len = __builtin_amdgcn_readfirstlane(sl_len[sl]);
-        const uint32_t *p = ell + sl_off[sl] + (slot & 63);  // entry k at p[64k]
-        V a = V(T(0));
-        if (len > 0) {
-          // software-pipelined over groups of 4 entries: the next group's
-          // table words load while this group's LDS reads and FMAs run; the
-          // loads run up to 4 positions past the slice's end (into the next
-          // slice or the table's 4 x 64 padding entries) and those words are
-          // never used
-          uint32_t nxt[4];
-          const uint32_t *q = p;  // first entry of the group being loaded
+        const uint32_t ng = __builtin_amdgcn_readfirstlane(sl_len[sl]) / 4;
+        // group g of this lane (4 entries, one 16-byte load) at p[64 g]
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 *p = reinterpret_cast<const u32x4 *>(ell + sl_off[sl]) + (slot & 63);
+        T a = T(0);
+        if (ng > 0) {
+          // table words are loaded two groups ahead, so their L2 latency
+          // overlaps two groups of LDS reads and FMAs; the loads run up to two
+          // groups past the slice's end (into the next slice or the table's
+          // two groups of padding) and those words are never used
+          u32x4 g1 = p[0], g2 = p[64];
+          typedef __attribute__((address_space(1))) const u32x4 gl_u4;
+          gl_u4 *q = (gl_u4 *)(p + 128);  // group g + 2
+          for (uint32_t g = 0; g < ng; g++) {
+            const u32x4 cur = g1;
+            g1 = g2;
+            // an opaque address (an empty asm statement, no instruction):
+            // otherwise the compiler re-loads each group where it is used
+            // and the loads no longer run ahead
+            asm volatile("" : "+v"(q));
+            g2 = *q;
+            q += 64;
+            const uint32_t w4[4] = {cur.x, cur.y, cur.z, cur.w};
+            // all eight LDS reads of the group issue before the first use
+            V wv[4], xv[4];
 #pragma unroll
-          for (uint32_t u = 0; u < 4; u++) nxt[u] = q[64 * u];
-          uint32_t k = 0;
-          for (; k + 4 <= len; k += 4) {
-            uint32_t cur[4];
-            q += 4 * 64;
-#pragma unroll
-            for (uint32_t u = 0; u < 4; u++) {
-              cur[u] = nxt[u];
-              nxt[u] = q[64 * u];
+            for (int u = 0; u < 4; u++) {
+              wv[u] = at(woff(w4[u]));
+              xv[u] = at(xoff(w4[u]));
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-              a += W[(cur[u] >> 16) & 0x7fffu] * flip_sign<T, EB>(X[cur[u] & 0xffffu], cur[u] & 0x80000000u);
+            for (int u = 0; u < 4; u++) entry_vals(w4[u], wv[u], xv[u], a);
           }
-#pragma unroll
-          for (uint32_t u = 0; u < 3; u++)  // the last len - k < 4 entries, already in nxt
-            if (k + u < len)
-              a += W[(nxt[u] >> 16) & 0x7fffu] * flip_sign<T, EB>(X[nxt[u] & 0xffffu], nxt[u] & 0x80000000u);
         }
-        a += X[rr[t]] * diag_sum(W, occ[t], L1);
+        a += dot(X[rr[t]], diag_sum(W, occ[t], L1));
         acc[t] += a;
       }
     }
@@ -303,7 +337,7 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
   for (int t = 0; t < NT; t++) {
     const int slot = tid + t * 1024;
     if (slot < nslots) {
-      const T v = hsum<T, EB>(acc[t]);
+      const T v = acc[t];
       const uint32_t r = perm[slot];
       if (SYM) {
         if (r != 0xffffffffu && slot > pos) {
@@ -832,12 +866,20 @@ void transpose_into(Context &c, const double *in, double *out, int R, int Cc) {
 template <typename T, int NT, int EB, bool SYM>
 static void launch_mixed_ell(Context &c, const T *eri, const T *x, T *y, size_t lds) {
   auto fn = k_mixed_ell<T, NT, EB, SYM>;
+  static bool lds_base_checked = false;
+  if (!lds_base_checked) {
+    // k_mixed_ell addresses LDS by byte offset from 0: no static LDS allowed
+    hipFuncAttributes attr;
+    SBDX_HIP(hipFuncGetAttributes(&attr, (const void *)fn));
+    if (attr.sharedSizeBytes != 0) throw Error(2, "k_mixed_ell: static LDS would offset the dynamic array");
+    lds_base_checked = true;
+  }
   SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int nslots = c.B.nslice * 64;
   const unsigned nsplit = cdiv(nslots, 1024 * NT);
   hipLaunchKernelGGL(fn, dim3((unsigned)c.na, nsplit), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
-                     nslots, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell, c.B.perm, c.B.sl_off,
-                     c.B.sl_len, eri, x, y, (SYM && c.sym_par < 0) ? 1 : 0);
+                     nslots, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell2, c.B.perm, c.B.sl_off2,
+                     c.B.sl_len2, eri, x, y, (SYM && c.sym_par < 0) ? 1 : 0);
 }
 
 // EB pairs staged per round, NT slots per thread (1, 2, 4 or 8); false when
@@ -845,7 +887,7 @@ static void launch_mixed_ell(Context &c, const T *eri, const T *x, T *y, size_t 
 template <typename T, int EB, bool SYM = false>
 static bool mixed_ell_dispatch(Context &c, int nt, const T *eri, const T *x, T *y) {
   const size_t lds = sizeof(T) * EB * ((size_t)c.nb + c.K + 1);
-  if (lds > c.lds_limit) return false;
+  if (lds > c.lds_limit || !c.B.has_ell2) return false;
   if (nt <= 1) launch_mixed_ell<T, 1, EB, SYM>(c, eri, x, y, lds);
   else if (nt <= 2) launch_mixed_ell<T, 2, EB, SYM>(c, eri, x, y, lds);
   else if (nt <= 4) launch_mixed_ell<T, 4, EB, SYM>(c, eri, x, y, lds);

@@ -8,6 +8,9 @@
 // masked popcount give each match its slot, so the order is ascending
 // without sorting.
 #include <algorithm>
+#include <array>
+#include <atomic>
+#include <thread>
 
 #include "common.h"
 
@@ -332,6 +335,200 @@ static void ell_layout(Context &c, const SpinTables &t, const std::vector<uint64
   }
 }
 
+// Packed entries of the staged mixed kernel (SpinTables::ell2): bits 4..18
+// hold K + ket, the LDS element of the ket's staged value (the staged ERI
+// row occupies elements 0..K-1, the ket row K..K+nb-1, a zero at K+nb),
+// so (w & 0x7fff0) is its byte offset for 16-byte elements; bits 19..30
+// hold pq, (w >> 15) & 0xfff0 its byte offset; bit 31 the sign.  Padding
+// entries read the zero ket element and ERI element 0, one address shared
+// by every padding lane (an LDS broadcast).
+//
+// Dealing a slice's entries to positions.  Measured on gfx950 with the
+// SQ_LDS_BANK_CONFLICT counter over per-lane index patterns (lanes in the
+// same coset of a 2-dimensional subspace of lane-index bits sharing a
+// residue, for every pair of bits): a 16-byte LDS read of a wavefront takes
+// 4 cycles when conflict-free, lane l being served in lane group
+// 2 * bit5(l) + (bit4 ^ bit3 ^ bit2)(l) (16 lanes per group), and two
+// distinct addresses of one group conflict when their 16-byte element
+// indices agree mod 16 (64 banks of 4 bytes); the cycles of one read are the
+// sum over the 4 groups of the largest number of distinct addresses sharing
+// a residue.  The groups are
+// independent: for each, every position takes one entry per lane (or, for a
+// lane with fewer entries than positions left, the shared padding address),
+// and a depth-first search (the lane with the fewest options first, entries
+// with the most heavily loaded residues first, a node budget) looks for a
+// position whose ket residues and ERI residues each repeat at most (cx, cw)
+// times in the group, trying (1,1), then (1,2) and (2,1), and so on; an
+// unbounded greedy choice ends it.  On the norb-36 workload this takes the
+// modelled cost of one read from 8.9 cycles (dealing that ignores the lane
+// grouping) to 5.2 (4 without conflicts).
+namespace qsched {
+struct Lane {
+  std::vector<uint32_t> ent;
+  uint16_t cnt[256];  // entries per (x, w) residue pair
+  uint16_t xw[16];    // bit w of xw[x]: some entry has residues (x, w)
+};
+static inline int rx(uint32_t w) { return (w >> 4) & 15; }
+static inline int rw(uint32_t w) { return (w >> 19) & 15; }
+struct Search {
+  Lane *L;
+  int remx[16], remw[16];
+  uint8_t ux[16], uw[16];
+  int cx, cw;
+  int padx, padw;
+  bool padused;
+  long nodes, limit;
+  int pick[16];  // pair index x*16+w, or -1 idle
+  bool need[16], done[16];
+  bool fits(int x, int w) const { return ux[x] < cx && uw[w] < cw; }
+  int options(int l, uint32_t wfree) const {
+    int n = 0;
+    for (int x = 0; x < 16; x++)
+      if (ux[x] < cx) n += __builtin_popcount(L[l].xw[x] & wfree);
+    return n;
+  }
+  uint32_t wfree_mask() const {
+    uint32_t m = 0;
+    for (int w = 0; w < 16; w++) if (uw[w] < cw) m |= 1u << w;
+    return m;
+  }
+  bool dfs(int depth) {
+    if (depth == 16) return true;
+    if (++nodes > limit) return false;
+    int bl = -1, bn = 1 << 30;
+    const uint32_t wf = wfree_mask();
+    for (int l = 0; l < 16; l++) {
+      if (done[l]) continue;
+      int n = options(l, wf) + (need[l] ? 0 : 1);
+      if (n < bn) { bn = n; bl = l; }
+    }
+    if (bn == 0) return false;
+    const int l = bl;
+    done[l] = true;
+    std::array<std::pair<int, int>, 256> cand;
+    int nc = 0;
+    for (int x = 0; x < 16; x++) {
+      if (ux[x] >= cx) continue;
+      uint32_t m = L[l].xw[x] & wf;
+      while (m) {
+        const int w = __builtin_ctz(m);
+        m &= m - 1;
+        cand[nc++] = {-(remx[x] + remw[w]), x * 16 + w};
+      }
+    }
+    std::sort(cand.begin(), cand.begin() + nc);
+    for (int i = 0; i < nc; i++) {
+      const int p = cand[i].second, x = p >> 4, w = p & 15;
+      ux[x]++; uw[w]++;
+      pick[l] = p;
+      if (dfs(depth + 1)) return true;
+      ux[x]--; uw[w]--;
+      if (nodes > limit) break;
+    }
+    if (!need[l]) {
+      const bool was = padused;
+      if (was || fits(padx, padw)) {
+        if (!was) { ux[padx]++; uw[padw]++; padused = true; }
+        pick[l] = -1;
+        if (dfs(depth + 1)) return true;
+        if (!was) { ux[padx]--; uw[padw]--; padused = false; }
+      }
+    }
+    done[l] = false;
+    return false;
+  }
+};
+// rows[lanes[i]]: the 16 lanes' entries (consumed); out[64*k + lanes[i]]
+static void deal_quarter(std::vector<uint32_t> *rows, const int *lanes, uint32_t len, uint32_t pad, uint32_t *out,
+                         long limit) {
+  Lane L[16];
+  Search s;
+  s.L = L;
+  std::fill(s.remx, s.remx + 16, 0);
+  std::fill(s.remw, s.remw + 16, 0);
+  for (int l = 0; l < 16; l++) {
+    L[l].ent = rows[lanes[l]];
+    std::fill(L[l].cnt, L[l].cnt + 256, 0);
+    std::fill(L[l].xw, L[l].xw + 16, 0);
+    for (uint32_t w : L[l].ent) {
+      L[l].cnt[rx(w) * 16 + rw(w)]++;
+      L[l].xw[rx(w)] |= (uint16_t)(1u << rw(w));
+      s.remx[rx(w)]++;
+      s.remw[rw(w)]++;
+    }
+  }
+  s.padx = rx(pad);
+  s.padw = rw(pad);
+  static const int caps[][2] = {{1, 1}, {1, 2}, {2, 1}, {2, 2}, {2, 3}, {3, 2}, {3, 3}};
+  const int ncaps = sizeof(caps) / sizeof(caps[0]);
+  for (uint32_t k = 0; k < len; k++) {
+    const uint32_t left = len - k;
+    for (int l = 0; l < 16; l++) s.need[l] = L[l].ent.size() >= left;
+    bool ok = false;
+    for (int c = 0; c < ncaps && !ok; c++) {
+      s.cx = caps[c][0];
+      s.cw = caps[c][1];
+      std::fill(s.ux, s.ux + 16, 0);
+      std::fill(s.uw, s.uw + 16, 0);
+      std::fill(s.done, s.done + 16, false);
+      s.padused = false;
+      s.nodes = 0;
+      s.limit = limit;
+      ok = s.dfs(0);
+    }
+    if (!ok) {
+      // unbounded: lane by lane, the pair minimizing the larger use count
+      std::fill(s.ux, s.ux + 16, 0);
+      std::fill(s.uw, s.uw + 16, 0);
+      bool pu = false;
+      for (int l = 0; l < 16; l++) {
+        if (L[l].ent.empty()) {
+          s.pick[l] = -1;
+          if (!pu) { s.ux[s.padx]++; s.uw[s.padw]++; pu = true; }
+          continue;
+        }
+        long best = 1L << 60;
+        int bp = 0;
+        for (int p = 0; p < 256; p++) {
+          if (!L[l].cnt[p]) continue;
+          const int x = p >> 4, w = p & 15;
+          const long sc = (long)std::max(s.ux[x], s.uw[w]) * 1000000 + (long)(s.ux[x] + s.uw[w]) * 1000 -
+                          (s.remx[x] + s.remw[w]);
+          if (sc < best) { best = sc; bp = p; }
+        }
+        s.pick[l] = bp;
+        s.ux[bp >> 4]++;
+        s.uw[bp & 15]++;
+      }
+    }
+    for (int l = 0; l < 16; l++) {
+      uint32_t w = pad;
+      if (s.pick[l] >= 0 && !L[l].ent.empty()) {
+        const int p = s.pick[l];
+        auto &E = L[l].ent;
+        for (size_t i = 0; i < E.size(); i++)
+          if (rx(E[i]) * 16 + rw(E[i]) == p) { w = E[i]; E[i] = E.back(); E.pop_back(); break; }
+        if (--L[l].cnt[p] == 0) L[l].xw[p >> 4] &= (uint16_t)~(1u << (p & 15));
+        s.remx[p >> 4]--;
+        s.remw[p & 15]--;
+      }
+      out[(size_t)64 * k + lanes[l]] = w;
+    }
+  }
+}
+}  // namespace qsched
+
+// The packed entries of one slice (rows: the 64 lanes' entries, consumed),
+// lane group by lane group.
+static void schedule_slice_packed(std::vector<uint32_t> *rows, uint32_t len, uint32_t pad, uint32_t *out) {
+  for (int c = 0; c < 4; c++) {
+    int lanes[16], n = 0;
+    for (int l = 0; l < 64; l++)
+      if (((l >> 5) & 1) * 2 + (((l >> 4) ^ (l >> 3) ^ (l >> 2)) & 1) == c) lanes[n++] = l;
+    qsched::deal_quarter(rows, lanes, len, pad, out, 200);
+  }
+}
+
 static void download_singles(Context &c, const SpinTables &t, std::vector<uint64_t> &off,
                              std::vector<uint64_t> &ent) {
   off.resize(t.n + 1);
@@ -367,6 +564,60 @@ void build_singles_ell(Context &c, SpinTables &t) {
   SBDX_HIP(hipMemcpyAsync(t.sl_len, sl_len.data(), sizeof(uint32_t) * nsl, hipMemcpyHostToDevice, c.stream));
   SBDX_HIP(hipStreamSynchronize(c.stream));
   t.has_ell = true;
+  // the packed form for k_mixed_ell (same rows, slices and lengths)
+  t.has_ell2 = false;
+  if (c.K > 4096 || (size_t)c.K + (size_t)n > 32767) return;
+  const uint32_t pad2 = ((uint32_t)(c.K + n)) << 4;
+  // groups of 4 positions: slice lengths rounded up to a multiple of 4, each
+  // group stored lane-major (lane l's 4 words contiguous, one 16-byte load),
+  // plus 2 groups of padding past the end for the kernel's prefetch
+  std::vector<uint32_t> sl_len2(nsl);
+  std::vector<uint64_t> sl_off2(nsl);
+  uint64_t total2 = 0;
+  for (int sl = 0; sl < nsl; sl++) {
+    sl_len2[sl] = (sl_len[sl] + 3) & ~3u;
+    sl_off2[sl] = total2;
+    total2 += (uint64_t)sl_len2[sl] * 64;
+  }
+  std::vector<uint32_t> ell2(total2 + 2 * 256, pad2);
+  // slices are independent: dealt on up to 16 host threads
+  const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::atomic<int> next{0};
+  auto worker = [&]() {
+    std::vector<uint32_t> rows[64];
+    std::vector<uint32_t> dealt;
+    for (int sl = next++; sl < nsl; sl = next++) {
+      for (int l = 0; l < 64; l++) {
+        rows[l].clear();
+        const uint32_t r = perm[(size_t)sl * 64 + l];
+        if (r == 0xffffffffu) continue;
+        for (uint64_t k = off[r]; k < off[r + 1]; k++) {
+          const uint64_t e = ent[k];
+          const uint32_t ket = (uint32_t)(e & 0xffffffffull);
+          const uint32_t pq = (uint32_t)((e >> 32) & 0x7fffffffull);
+          rows[l].push_back(((uint32_t)(c.K + ket) << 4) | (pq << 19) | ((e >> 63) ? 0x80000000u : 0u));
+        }
+      }
+      const uint32_t len2 = sl_len2[sl];
+      dealt.assign((size_t)len2 * 64, pad2);
+      schedule_slice_packed(rows, sl_len[sl], pad2, dealt.data());
+      uint32_t *o = ell2.data() + sl_off2[sl];
+      for (uint32_t k = 0; k < len2; k++)
+        for (int l = 0; l < 64; l++) o[256 * (k / 4) + 4 * l + (k % 4)] = dealt[(size_t)64 * k + l];
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int i = 1; i < nthr; i++) pool.emplace_back(worker);
+  worker();
+  for (auto &th : pool) th.join();
+  t.ell2 = (uint32_t *)dalloc(c, sizeof(uint32_t) * ell2.size());
+  t.sl_off2 = (uint64_t *)dalloc(c, sizeof(uint64_t) * nsl);
+  t.sl_len2 = (uint32_t *)dalloc(c, sizeof(uint32_t) * nsl);
+  SBDX_HIP(hipMemcpyAsync(t.ell2, ell2.data(), sizeof(uint32_t) * ell2.size(), hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(t.sl_off2, sl_off2.data(), sizeof(uint64_t) * nsl, hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipMemcpyAsync(t.sl_len2, sl_len2.data(), sizeof(uint32_t) * nsl, hipMemcpyHostToDevice, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+  t.has_ell2 = true;
 }
 
 // Ket-range parts of the sliced-ELL singles (SpinTables::pell): part p holds
