@@ -27,6 +27,7 @@ for _p in (ROOT, os.path.join(ROOT, "sbd-pub_amd")):
 METRIC = "time-to-E0 (|ΔE|<1e-8 Ha) & sigma GB/s: mixed+learned vs plain FP64, 1 GPU"
 HBM_PEAK_GBPS = 8000.0
 MFMA_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}  # MI355X dense matrix peaks
+LDS_PEAK_TBPS = 256 * 256 * 2.4e9 / 1e12  # 256 CUs x 256 B/clk x 2.4 GHz
 
 
 def log(msg):
@@ -62,19 +63,33 @@ def kernel_bytes(wc, na, nb, e=8.0):
     }
 
 
-def profiled_traffic(name, width, slots):
-    """HBM bytes per launch of kernel group `name` at the given mixed-kernel
-    configuration, from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
-    profiles/ (None when that configuration was not profiled)."""
+def profiled_traffic(name, width=None, slots=None):
+    """HBM bytes per launch of kernel group `name` (at the given mixed-kernel
+    configuration, when it has one), from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary in profiles/ (None when not profiled)."""
     path = os.path.join(ROOT, "profiles", "c36_hbm_traffic.md")
     if not os.path.exists(path):
         return None
-    want = f"eb={width} nt={slots}"
+    want = f"eb={width} nt={slots}" if width is not None else "-"
     for line in open(path):
         f = line.split()
-        if len(f) == 5 and f[0] == "traffic_bytes" and f[1] == name and f"{f[2]} {f[3]}" == want:
-            return float(f[4])
+        if f and f[0] == "traffic_bytes" and f[1] == name:
+            cfg = f"{f[2]} {f[3]}" if len(f) == 5 else "-"
+            if cfg == want:
+                return float(f[-1])
     return None
+
+
+def lds_bytes_mixed(wc, na, nb, n_elec, eb, e=4.0, symmetric=True):
+    """LDS bytes one launch of the staged mixed-term kernel reads: per round of
+    eb staged alpha singles (plus one spectator round per bra), every slot
+    reads two eb-wide values (staged ket row, staged ERI row) per beta single
+    and n_elec + 1 eb-wide values for its dressing; the symmetric form visits
+    half the slots."""
+    rounds = (wc["alpha_singles"] + na) / eb
+    slots = nb / 2 if symmetric else nb
+    singles = wc["beta_singles"] / 2 if symmetric else wc["beta_singles"]
+    return rounds * (2 * e * eb * singles + e * eb * slots * (n_elec + 1))
 
 
 def profiled_limiter(name, width, slots):
@@ -92,6 +107,18 @@ def profiled_limiter(name, width, slots):
             out = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in f[4:]}
             out["source"] = "profiles/c36_sq_counters.md (rocprofv3 SQ counters)"
             return out
+    return None
+
+
+def cpu_calibration():
+    """Reference-over-oracle sigma time ratio measured by oracle/calibrate.py
+    (profiles/cpu_calibration.md), or None."""
+    path = os.path.join(ROOT, "profiles", "cpu_calibration.md")
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 2 and f[0] == "ratio_reference_over_oracle":
+                return float(f[1])
     return None
 
 
@@ -196,6 +223,7 @@ def main():
     ap.add_argument("--no-compare", dest="compare", action="store_false",
                     help="skip the plain FP64 reference solve")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    # the host cores one GPU's job gets (16 on the benchmark machine)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--factored", action="store_true",
@@ -354,6 +382,24 @@ def main():
     SIGMA_PARTS = ("same_gemm", "same_alpha", "transpose", "same_beta", "transpose_add", "mixed",
                    "sym_same", "sym_mixed")
 
+    def sigma_hbm_rate(kernels, suffix, forms):
+        """(HBM GB, GB/s) of one product from the profiled per-launch traffic
+        of its kernel groups; None when a group was not profiled."""
+        names = [n + suffix for n in SIGMA_PARTS if n + suffix in kernels]
+        if not any("mixed" in n for n in names) or args.workload != "c36":
+            return None
+        by = 0.0
+        for n in names:
+            base = n[:-4] if n.endswith("_f32") else n
+            pre = "sym_" if base.startswith("sym_") else ""
+            t = (profiled_traffic(n, forms[pre + "mixed_width"], forms[pre + "mixed_slots_per_thread"])
+                 if "mixed" in base else profiled_traffic(n))
+            if t is None:
+                return None
+            by += t
+        ms = sum(kernels[n]["avg_ms"] for n in names)
+        return dict(hbm_bytes=by, ms=round(ms, 3), GBps=round(by / (ms * 1e-3) / 1e9, 1))
+
     def sigma_rate(kernels, suffix):
         names = [n + suffix for n in SIGMA_PARTS if n + suffix in kernels]
         if not any("mixed" in n for n in names):
@@ -371,24 +417,46 @@ def main():
         roofline = dict(kernel=dom, bound="mfma", achieved=dk["TFLOPs"], peak=peak, unit="TFLOP/s",
                         frac=round(dk["TFLOPs"] / peak, 4), traffic=None)
     else:
-        forms = op.same_spin_forms()["fp32" if dom.endswith("_f32") else "fp64"]
+        # The mixed-term kernel gathers from rows it stages in LDS, so the
+        # bytes it moves from HBM are far fewer than the gathers it performs:
+        # `achieved` is the HBM traffic the rocprofv3 counters measured for
+        # one launch over the launch's live average duration, and the gather
+        # count and the LDS bytes it reads are reported beside it.
+        f32 = dom.endswith("_f32")
+        forms = op.same_spin_forms()["fp32" if f32 else "fp64"]
         pre = "sym_" if dom.startswith("sym_") else ""
         width, slots = forms[pre + "mixed_width"], forms[pre + "mixed_slots_per_thread"]
         traffic = limiter = None
         if args.workload == "c36":
             traffic = profiled_traffic(dom, width, slots)
             limiter = profiled_limiter(dom, width, slots)
-        roofline = dict(kernel=dom, bound="hbm", achieved=dk["GBps"], peak=HBM_PEAK_GBPS,
-                        unit="GB/s", frac=round(dk["GBps"] / HBM_PEAK_GBPS, 4), traffic=traffic,
+        avg_s = dk["avg_ms"] * 1e-3
+        achieved = round(traffic / avg_s / 1e9, 1) if traffic else None
+        n_elec = spec["nelec_half"]
+        lds = lds_bytes_mixed(wc, na, nb, n_elec, max(width, 1), 4.0 if f32 else 8.0, bool(pre))
+        roofline = dict(kernel=dom, bound="hbm", achieved=achieved, peak=HBM_PEAK_GBPS,
+                        unit="GB/s", frac=round(achieved / HBM_PEAK_GBPS, 4) if achieved else None,
+                        traffic=traffic,
                         traffic_source="profiles/c36_hbm_traffic.md (rocprofv3 FETCH_SIZE x2.0 + "
                                        "WRITE_SIZE x1.0, calibrated on streaming kernels)"
                         if traffic else None,
+                        avg_ms=dk["avg_ms"],
+                        gather_equivalent=dict(bytes=dk["alg_bytes"], GBps=dk["GBps"],
+                                               note="every gathered element counted as one read"),
+                        lds=dict(bytes=lds, achieved_TBps=round(lds / avg_s / 1e12, 2),
+                                 peak_TBps=LDS_PEAK_TBPS, frac=round(lds / avg_s / 1e12 / LDS_PEAK_TBPS, 4),
+                                 note="LDS read bytes of the staged gathers; peak = 256 CUs x 256 B/clk "
+                                      "x 2.4 GHz"),
                         limiter=limiter)
-    s64_ms, s64_gbps = sigma_rate(kernels, "")
-    s32_ms, s32_gbps = sigma_rate(kernels, "_f32")
+    s64_ms, _ = sigma_rate(kernels, "")
+    s32_ms, _ = sigma_rate(kernels, "_f32")
     if plain is not None:
         pk = kernel_table(plain["stats"])
-        s64_ms, s64_gbps = sigma_rate(pk, "")
+        s64_ms, _ = sigma_rate(pk, "")
+    # sigma GB/s from HBM bytes the counters measured for the product's
+    # kernels (profiles/), over the product's live kernel time
+    s64_gbps = sigma_hbm_rate(pk if plain is not None else kernels, "", op.same_spin_forms()["fp64"])
+    s32_gbps = sigma_hbm_rate(kernels, "_f32", op.same_spin_forms()["fp32"])
 
     T = res["time"]
     dtypes = {"plain": "f64", "thick": "f64", "mixed": "f32+f64", "block": "f32+f64"}
@@ -433,9 +501,13 @@ def main():
         "block_roots": res.get("roots"),
         "sigma_fp64_ms": s64_ms,
         "sigma_fp64_host_buffers_ms": round(sigma_host_ms, 1),
-        "sigma_GBps": s64_gbps,
+        "sigma_GBps": s64_gbps["GBps"] if s64_gbps else None,
+        "sigma_fp64_hbm": s64_gbps,
         "sigma_fp32_ms": s32_ms,
-        "sigma_fp32_GBps": s32_gbps,
+        "sigma_fp32_GBps": s32_gbps["GBps"] if s32_gbps else None,
+        "sigma_fp32_hbm": s32_gbps,
+        "sigma_GBps_note": "HBM bytes measured by rocprofv3 counters for the product's kernels "
+                           "(profiles/c36_hbm_traffic.md) over their live time",
         "roofline": roofline,
         "kernels": kernels,
         "same_spin_form": op.same_spin_forms(),
@@ -451,6 +523,11 @@ def main():
         out["cpu_baseline"] = {k: (round(v, 3) if isinstance(v, float) else v)
                                for k, v in cb.items() if k != "sigma_s"}
         out["cpu_baseline"]["sigma_s"] = round(cb["sigma_s"], 3)
+        ratio = cpu_calibration()
+        if ratio:
+            out["cpu_baseline"]["calibration"] = dict(
+                reference_over_oracle=ratio, reference_equivalent_value=round(cb["value"] * ratio, 1),
+                source="profiles/cpu_calibration.md (reference binary vs oracle, same instances, 8 threads)")
     op.close()
     print(json.dumps(out), flush=True)
 

@@ -927,8 +927,8 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
 }
 
 // ------------------------------------------------------- spin-flip sectors
-// Smallest hii[a*n + b] over a < b (block partials: value, flat index).
-__global__ void __launch_bounds__(kRedThreads) k_argmin_offdiag(size_t n, const double *__restrict__ hii,
+// Smallest hii[a*n + b] over a < b < lim (block partials: value, flat index).
+__global__ void __launch_bounds__(kRedThreads) k_argmin_offdiag(size_t n, size_t lim, const double *__restrict__ hii,
                                                                 double *__restrict__ part_v,
                                                                 unsigned long long *__restrict__ part_i) {
   __shared__ double sv[kRedThreads];
@@ -937,7 +937,7 @@ __global__ void __launch_bounds__(kRedThreads) k_argmin_offdiag(size_t n, const 
   unsigned long long bi = ~0ull;
   const size_t N = n * n;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
-    if (i / n < i % n && hii[i] < best) {
+    if (i / n < i % n && i % n < lim && hii[i] < best) {
       best = hii[i];
       bi = i;
     }
@@ -967,7 +967,8 @@ __global__ void __launch_bounds__(kRedThreads) k_argmin_offdiag(size_t n, const 
 // runs the even sector from the even part of the start vector, then the odd
 // sector from its odd part (or, when that part is negligible -- the
 // Hartree-Fock vector has none -- from the antisymmetric pair on the lowest
-// off-diagonal diagonal element, (e_ab - e_ba)/sqrt 2), and keeps the lower
+// off-diagonal diagonal element, (e_ab - e_ba)/sqrt 2, taken among the first
+// coarse_m strings when the coarse start is on), and keeps the lower
 // of the two.  The odd stage stops once its Ritz value lies above the even
 // result by more than ten times its residual norm (near convergence the Ritz
 // value exceeds its eigenvalue by ~|r|^2 / gap), so when the even sector
@@ -1028,8 +1029,11 @@ int davidson_solve_sectors(Context &c, int max_steps, int nb, int keep, double e
       if (n_steps) *n_steps = se;
       return rce;
     }
+    // within the coarse-start block when there is one, so the odd stage's
+    // coarse solve starts from it too
+    const size_t lim = (c.coarse_m >= 2 && c.coarse_m < n) ? c.coarse_m : n;
     TmpBuf pv(sizeof(double) * kRedBlocks), pi(sizeof(unsigned long long) * kRedBlocks);
-    hipLaunchKernelGGL(k_argmin_offdiag, dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, n, c.hii,
+    hipLaunchKernelGGL(k_argmin_offdiag, dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, n, lim, c.hii,
                        (double *)pv.p, (unsigned long long *)pi.p);
     check_launch("k_argmin_offdiag");
     std::vector<double> hv(kRedBlocks);
