@@ -206,6 +206,8 @@ def main():
                          "with FP32 correction products")
     ap.add_argument("--block", type=int, default=4)
     ap.add_argument("--roots", type=int, default=1, help="roots of the block solver")
+    ap.add_argument("--block-basis", type=int, default=0,
+                    help="basis size of the block solver (0: as large as memory allows, up to 64)")
     ap.add_argument("--sectors", choices=["lowest", "even", "odd", "off"], default="lowest",
                     help="spin-flip sectors of the optimized solver when the string sets are equal: "
                          "lowest (default) solves the even sector from the start vector, then the "
@@ -291,25 +293,40 @@ def main():
             op.set_symmetry(args.sectors if symmetric else "off")
         if solver in ("thick", "mixed", "block"):
             op.set_coarse_start(args.coarse if args.coarse < min(na, nb) else 0, args.coarse_eps)
+        if solver == "block":
+            op.set_block_basis(args.block_basis)
         op.set_preconditioner(1 if precond == "mlp" else 0)
         op.set_verbose(0 if args.quiet else 1)
         op.profile(True)
         times, energies, nsteps, conv = [], [], [], []
-        roots = sectors = None
+        roots = sectors = block_info = None
         for k in range(nsolve):
             op.set_vector(W0)
             t0 = time.perf_counter()
             if solver == "block":
-                Eb, _, n, ok = op.davidson_block(args.roots, args.block,
-                                                 W0=W0[None, :] if args.roots == 1 else None,
-                                                 max_it=args.max_it, eps=args.eps, max_time=args.max_time)
-                E = float(Eb[0])
-                roots = [float(e) for e in Eb]
-                log(f"block roots: {roots}")
+                Eb, Wb, n, ok = op.davidson_block(args.roots, args.block,
+                                                  W0=W0[None, :] if args.roots == 1 else None,
+                                                  max_it=args.max_it, eps=args.eps, max_time=args.max_time)
             else:
                 E, n, ok = op.solve(max_it=args.max_it, nb=args.nb, eps=args.eps,
                                     max_time=args.max_time)
             dt = time.perf_counter() - t0
+            if solver == "block":
+                E = float(Eb[0])
+                roots = [float(e) for e in Eb]
+                # each root against FP64 refinement (outside the timed
+                # region): the FP64 Rayleigh quotient and residual norm of
+                # the returned vector, from exact FP64 products
+                rq, rr = [], []
+                for j in range(args.roots):
+                    sj = op.sigma(Wb[j])
+                    nw = float(Wb[j] @ Wb[j])
+                    rq.append(float(Wb[j] @ sj) / nw)
+                    rr.append(float(np.linalg.norm(sj - Eb[j] * Wb[j])))
+                block_info = dict(op.block_stats(args.roots), block=args.block, roots=roots,
+                                  fp64_rayleigh=rq, abs_dE_vs_fp64=[abs(a - b) for a, b in zip(roots, rq)],
+                                  fp64_resid=rr)
+                log(f"block roots: {block_info}")
             times.append(dt)
             energies.append(E)
             nsteps.append(n)
@@ -326,6 +343,7 @@ def main():
         op.profile(False)
         return dict(time=float(np.median(times)), E=energies[-1], steps=int(np.median(nsteps)),
                     converged=all(conv), stats=stats, precond_steps=op.precond_stats(), roots=roots,
+                    block_info=block_info,
                     sectors=sectors,
                     fp64_switch_step=op.fp64_switch_step() if solver in ("thick", "mixed") else None,
                     coarse=op.coarse_stats() if solver in ("thick", "mixed", "block") else None)
@@ -410,9 +428,13 @@ def main():
 
     kernels = kernel_table(stats)
     sig_kernels = [n for n in kernels if "alg_bytes" in kernels[n]]
-    dom = max(sig_kernels, key=lambda n: kernels[n]["total_ms"])
-    dk = kernels[dom]
-    if dom.startswith("same_gemm") or dom.startswith("sym_same"):
+    dom = max(sig_kernels, key=lambda n: kernels[n]["total_ms"]) if sig_kernels else None
+    dk = kernels[dom] if dom else None
+    if dom is None:
+        # no kernel group with a per-unit byte count was profiled (block
+        # products through the multi-vector kernel)
+        roofline = None
+    elif dom.startswith("same_gemm") or dom.startswith("sym_same"):
         peak = MFMA_PEAK_TFLOPS["f32" if dom.endswith("_f32") else "f64"]
         roofline = dict(kernel=dom, bound="mfma", achieved=dk["TFLOPs"], peak=peak, unit="TFLOP/s",
                         frac=round(dk["TFLOPs"] / peak, 4), traffic=None)
@@ -499,6 +521,7 @@ def main():
         "hipgraph_solve": graph_res,
         "dE_vs_plain": None if plain is None else abs(res["E"] - plain["E"]),
         "block_roots": res.get("roots"),
+        "block_solver": res.get("block_info"),
         "sigma_fp64_ms": s64_ms,
         "sigma_fp64_host_buffers_ms": round(sigma_host_ms, 1),
         "sigma_GBps": s64_gbps["GBps"] if s64_gbps else None,

@@ -105,21 +105,37 @@ int sbdx_davidson(sbdx_handle_t h, double *W, int max_it, int nb, double eps,
 /*
  * Block Davidson for the lowest nroots eigenpairs (no counterpart in the
  * reference, whose Davidson is single-root): `block` (nroots <= block <= 16)
- * correction vectors per iteration go through one block sigma; basis up to
- * min(64, 4*block) vectors, thick restart on the Ritz block; convergence
- * when |E_j W_j - H W_j| < eps for every j < nroots, judged on exact FP64
- * products (with SBDX_PREC_MIXED the corrections' products are FP32 and
- * every restart recomputes the block's products in FP64).  The
- * preconditioner is the handle's (sbdx_set_preconditioner).
+ * correction vectors per iteration go through one block sigma; converged
+ * roots are locked (no corrections); thick restart on the lowest Ritz
+ * vectors when the basis (sbdx_set_block_basis) is full.  Convergence when
+ * |E_j W_j - H W_j| < eps for every j < nroots on a basis whose columns and
+ * products are all FP64.  With SBDX_PREC_MIXED the correction vectors and
+ * their products are FP32 in arithmetic and in storage (dot products and
+ * combinations in FP64, Rayleigh-Ritz with the overlap matrix of the stored
+ * columns), the roots are FP64 with exact FP64 products from every restart
+ * on, and convergence is confirmed on a restart that keeps the roots alone.
+ * The preconditioner is the handle's (sbdx_set_preconditioner).
  * W: host, nroots consecutive n_det vectors, linearly independent start
  * vectors on entry (block - nroots more are unit vectors on the lowest
  * diagonal elements), the Ritz vectors on exit; energies[nroots] ascending;
  * *n_products counts single-vector products.  max_it bounds iterations
- * (block expansions).  Returns SBDX_ERR_NOT_CONVERGED with valid outputs
- * when eps was not reached.
+ * (block expansions and restarts).  Returns SBDX_ERR_NOT_CONVERGED with
+ * valid outputs when eps was not reached.
  */
 int sbdx_davidson_block(sbdx_handle_t h, int nroots, int block, double *W, int max_it, double eps,
                         double max_time, double *energies, int *n_products);
+
+/* Basis size of sbdx_davidson_block (no counterpart in the reference): m
+ * columns (nroots + 2*block <= m <= 64), or 0 (default) for the largest m
+ * up to 64, in steps of `block`, whose basis fits in 80% of free device
+ * memory.  A thick restart keeps max(nroots + block, m/3) Ritz vectors
+ * (at most m - block and 32). */
+int sbdx_set_block_basis(sbdx_handle_t h, int m);
+
+/* What the last sbdx_davidson_block used: basis size, Ritz vectors kept at
+ * a restart, restarts, and the final residual norms of the nroots roots
+ * (resid may be null).  No counterpart in the reference. */
+int sbdx_block_stats(sbdx_handle_t h, int *basis, int *keep, int *restarts, double *resid, int nroots);
 
 /* Split form of sbdx_davidson for callers that keep the vector resident:
  * upload the start vector, solve on the device, download the solution. */
@@ -211,6 +227,15 @@ int sbdx_set_mixed_form(sbdx_handle_t h, int form);
  * Replaces the reference's LAPACK dsyev call (hp_numeric::MatHeev,
  * framework/hp_numeric.h:503-522); needs no handle or GPU. */
 int sbdx_symmetric_eig(int n, const double *A, double *V, double *E);
+
+/* The block solver's Rayleigh-Ritz step, on the host: H y = E S y for
+ * symmetric H and positive definite S (n x n column-major, not modified),
+ * by Cholesky S = L L^T and sbdx_symmetric_eig of L^-1 H L^-T; E[n]
+ * ascending, S-orthonormal Y (columns, n x n column-major).  No
+ * counterpart in the reference (its Davidson keeps an orthonormal basis);
+ * needs no handle or GPU.  SBDX_ERR_NOT_CONVERGED (5) when S is not
+ * numerically positive definite. */
+int sbdx_symmetric_geig(int n, const double *H, const double *S, double *Y, double *E);
 
 /* Spin-flip symmetry (no counterpart in the reference): with equal alpha and
  * beta string sets, H commutes with the exchange of the two strings, so its

@@ -418,6 +418,27 @@ int sbdx_davidson_block(sbdx_handle_t h, int nroots, int block, double *W, int m
   SBDX_GUARD_END
 }
 
+int sbdx_set_block_basis(sbdx_handle_t h, int m) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (m != 0 && (m < 3 || m > kNbMax)) throw Error(SBDX_ERR_ARG, "block basis size must be 0 or in [3, 64]");
+  c.blk_m_req = m;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_block_stats(sbdx_handle_t h, int *basis, int *keep, int *restarts, double *resid, int nroots) {
+  SBDX_GUARD_BEGIN
+  Context &c = ctx_of(h);
+  if (basis) *basis = c.blk_m_used;
+  if (keep) *keep = c.blk_keep_used;
+  if (restarts) *restarts = c.blk_restarts;
+  if (resid)
+    for (int j = 0; j < nroots; j++) resid[j] = j < (int)c.blk_resid.size() ? c.blk_resid[j] : 0.0;
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
 int sbdx_set_vector(sbdx_handle_t h, const double *W) {
   SBDX_GUARD_BEGIN
   Context &c = ctx_of(h);
@@ -544,6 +565,15 @@ int sbdx_symmetric_eig(int n, const double *A, double *V, double *E) {
   if (!A || !V || !E) throw Error(SBDX_ERR_ARG, "null argument");
   std::vector<double> a(A, A + (size_t)n * n);
   host_syev(n, a.data(), V, E);
+  return SBDX_OK;
+  SBDX_GUARD_END
+}
+
+int sbdx_symmetric_geig(int n, const double *H, const double *S, double *Y, double *E) {
+  SBDX_GUARD_BEGIN
+  if (n < 1 || n > 4096) throw Error(SBDX_ERR_ARG, "n must be in [1, 4096]");
+  if (!H || !S || !Y || !E) throw Error(SBDX_ERR_ARG, "null argument");
+  host_sygv(n, H, S, n, Y, E);
   return SBDX_OK;
   SBDX_GUARD_END
 }

@@ -203,12 +203,16 @@ struct Context {
   std::vector<double> h_h1, h_eri;
   int restart_keep = 0;    // 0: restart from W alone (reference); k: thick restart keeping k Ritz vectors
   std::vector<double *> spareY, spareHY;  // thick-restart staging vectors
-  // Block Davidson: contiguous basis and products (blk_m vectors each), the
-  // Ritz block W, HW and residuals R (blk_b vectors each), small matrices.
-  double *blk_C = nullptr, *blk_HC = nullptr, *blk_W = nullptr, *blk_HW = nullptr,
-         *blk_R = nullptr, *blk_small = nullptr;
-  int blk_m = 0, blk_b = 0;
-  std::vector<double> blk_E;  // Ritz values of the last block solve
+  // Block Davidson: basis columns and products in an FP64 bank (blk_r64
+  // columns) and an FP32 bank (blk_m - blk_r64 columns), residual block R
+  // (blk_block FP64 vectors), reduction partials, small matrices; the basis
+  // size asked for (0: as large as memory allows, up to 64) and what the
+  // last solve used.
+  double *blk_C64 = nullptr, *blk_HC64 = nullptr, *blk_R = nullptr, *blk_part = nullptr, *blk_small = nullptr;
+  float *blk_C32 = nullptr, *blk_HC32 = nullptr;
+  int blk_m = 0, blk_r64 = 0, blk_r64cap = 0, blk_mixed = -1, blk_block = 0, blk_m_alloc_req = -1;
+  int blk_m_req = 0, blk_m_used = 0, blk_keep_used = 0, blk_restarts = 0;
+  std::vector<double> blk_E, blk_resid;  // Ritz values and residual norms of the last block solve
   MlpState mlp;
   int precond = 0;         // 0: diagonal R/(E0-d); 1: MLP with diagonal fallback on stall
   // work vectors
@@ -292,11 +296,15 @@ void sigma_sym_f32(Context &c, const double *x, double *y);
 double sym_defect(Context &c, const double *v, int p);  // max|v - p v^T| / max|v|, p = +-1
 void symmetrize(Context &c, double *v, int p);         // v = (v + p v^T)/2
 void sigma_block_f32(Context &c, int k, const double *x, double *y);
+void sigma_block_f32_raw(Context &c, int k, const float *x, float *y);  // FP32 vectors in and out
 void same_dense_release(Context &c);  // frees the rocBLAS handle
 void graphs_release(Context &c);      // frees captured graphs
 // eigenpairs of a symmetric n x n column-major A (overwritten), ascending:
 // E[n], eigenvectors in the columns of Q (n x n)
 void host_syev(int n, double *A, double *Q, double *E);
+// H y = theta S y (S positive definite), n x n column-major with leading
+// dimension ld: S-orthonormal eigenvectors Y (n x n, ld n), E ascending
+void host_sygv(int n, const double *H, const double *S, int ld, double *Y, double *E);
 void transpose_into(Context &c, const double *in, double *out, int R, int Cc);  // out = in^T
 
 // davidson.hip

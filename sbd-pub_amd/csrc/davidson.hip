@@ -27,7 +27,6 @@ constexpr int kRedBlocks = 1024;
 // kFp32Floor |E0| a stalled mixed-precision solve switches to FP64 products
 constexpr double kFp32Floor = 1000.0 * 5.9604644775390625e-8;
 constexpr int kRedThreads = 256;
-constexpr int kBlockCoarseIt = 25;  // iteration cap of the block solver's coarse stage
 
 struct OutList {
   double *p[8];
@@ -1083,428 +1082,6 @@ int davidson_solve_sectors(Context &c, int max_steps, int nb, int keep, double e
   if (energy) *energy = odd ? Eo : Ee;
   if (n_steps) *n_steps = se + so;
   return odd ? rco : rce;
-}
-
-// ------------------------------------------------------------ block Davidson
-// R_j = HW_j - E_j W_j for j < b (vectors contiguous, stride n); partial
-// |R_j|^2 of this block into part[blk*b + j].
-__global__ void __launch_bounds__(kRedThreads) k_block_resid(int b, size_t n, const double *__restrict__ W,
-                                                             const double *__restrict__ HW,
-                                                             const double *__restrict__ E,
-                                                             double *__restrict__ R,
-                                                             double *__restrict__ part) {
-  __shared__ double sh[kRedThreads];
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (int j = 0; j < b; j++) {
-    const double e = E[j];
-    const double *w = W + (size_t)j * n;
-    const double *hw = HW + (size_t)j * n;
-    double *r = R + (size_t)j * n;
-    double s = 0.0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-      const double v = hw[i] - e * w[i];
-      r[i] = v;
-      s += v * v;
-    }
-    const double t = block_sum(s, sh);
-    if (threadIdx.x == 0) part[(size_t)blockIdx.x * b + j] = t;
-  }
-}
-
-// Columns j in [j0, j1) of the subspace matrix hold HC_i . C_j for every row
-// i; keep the entries with i <= j (the older vector's product) and mirror
-// them: H[j][i] = H[i][j] for i < j.
-__global__ void k_symm_cols(double *H, int ld, int j0, int j1) {
-  const int j = j0 + blockIdx.x;
-  const int i = threadIdx.x;
-  if (j < j1 && i < j) H[j + (size_t)ld * i] = H[i + (size_t)ld * j];
-}
-
-// Block Davidson (no counterpart in the reference, whose solver is
-// single-root): the lowest nroots eigenpairs with `block` >= nroots
-// correction vectors per iteration, all pushed through one block sigma
-// (sigma_block: batched GEMMs and the mixed kernels).
-//   basis C (contiguous, up to m vectors) and products HC, subspace matrix
-//   H[i][j] = HC_i . C_j (i <= j) by GEMM, Rayleigh-Ritz on the device;
-//   Ritz block W = C U, HW = HC U, R = HW - E W (GEMMs + one pass);
-//   corrections: preconditioned residuals of the lowest `block` Ritz pairs
-//   (diagonal, or the MLP with the stall fallback of the thick solver),
-//   classical Gram-Schmidt twice against the basis by GEMM, then modified
-//   Gram-Schmidt inside the block; a direction whose norm falls below 1e-10
-//   of its norm before projection is dropped;
-//   thick restart when the next block would not fit: the basis becomes the
-//   lowest `keep` Ritz vectors with their products by combination; in mixed
-//   mode the first nroots of them get exact FP64 products, so they are the
-//   older index in every coupling (H[i][j] = HC_i . C_j) and the roots'
-//   Rayleigh quotients use exact products;
-//   convergence: |R_j| < eps for j < nroots on a basis whose root products
-//   are exact, confirmed in mixed mode by exact FP64 products of the final
-//   vectors.
-// m = min(64, max(4*block, 16)), reduced by `block` steps while the basis,
-// products and Ritz buffers would take more than 80% of free memory.
-// W0/Wout: device, nroots vectors in the alpha-first basis.
-int davidson_block(Context &c, int nroots, int block, int max_it, double eps, double max_time,
-                   bool mixed, const double *W0, double *Wout, double *Eout, int *n_products) {
-  if (nroots < 1 || block < nroots || block > 16) throw Error(1, "davidson_block: need 1 <= nroots <= block <= 16");
-  const size_t N = c.N;
-  const int keep_want = std::max(2 * block, nroots + block);
-  int m = std::min(kNbMax, std::max(4 * block, 16));
-  m = std::max(m, keep_want + block);
-  if (m > kNbMax) throw Error(1, "davidson_block: block too large for the basis limit");
-  if (c.blk_m < m || c.blk_b < keep_want) {
-    dfree(c, c.blk_C);
-    dfree(c, c.blk_HC);
-    dfree(c, c.blk_W);
-    dfree(c, c.blk_HW);
-    dfree(c, c.blk_R);
-    c.blk_C = c.blk_HC = c.blk_W = c.blk_HW = c.blk_R = nullptr;
-    c.blk_m = c.blk_b = 0;
-    size_t free_b = 0, total_b = 0;
-    SBDX_HIP(hipMemGetInfo(&free_b, &total_b));
-    const double budget = 0.8 * (double)free_b;
-    auto need = [&](int mm, int kk) { return (double)sizeof(double) * N * (2.0 * mm + 2.0 * kk + block); };
-    while (need(m, keep_want) > budget && m - block >= keep_want + block) m -= block;
-    if (need(m, keep_want) > budget) throw Error(2, "davidson_block: basis does not fit in device memory");
-    c.blk_C = (double *)dalloc(c, sizeof(double) * N * m);
-    c.blk_HC = (double *)dalloc(c, sizeof(double) * N * m);
-    c.blk_W = (double *)dalloc(c, sizeof(double) * N * keep_want);
-    c.blk_HW = (double *)dalloc(c, sizeof(double) * N * keep_want);
-    c.blk_R = (double *)dalloc(c, sizeof(double) * N * block);
-    c.blk_m = m;
-    c.blk_b = keep_want;
-  }
-  m = std::min(m, c.blk_m);
-  const int keep = std::min(keep_want, m - block);
-  if (!c.blk_small) c.blk_small = (double *)dalloc(c, sizeof(double) * kNbMax * kNbMax * 2);
-  rocblas_handle hb = (rocblas_handle)blas_handle(c);
-  const int ld = kNbMax;
-  double *H = c.dsmall, *U = c.dsmall + ld * ld, *E = c.dsmall + 2 * ld * ld;
-  double *S = c.blk_small;             // CGS coefficients (d x block, ld d)
-  double *nrm = c.blk_small + ld * ld;  // block residual norms^2
-  const double eps_reg = 1.0e-12;
-  const dim3 g(kRedBlocks), bthr(kRedThreads);
-  double *C = c.blk_C, *HC = c.blk_HC;
-  auto col = [&](double *base, int j) { return base + (size_t)j * N; };
-  auto chk = [](rocblas_status st, const char *what) {
-    if (st != rocblas_status_success) throw Error(3, std::string("davidson_block: ") + what + " failed");
-  };
-  const double one = 1.0, zero = 0.0, mone = -1.0;
-  auto t_start = std::chrono::steady_clock::now();
-  auto elapsed = [&]() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-  };
-  // classical Gram-Schmidt (twice) of the nv vectors at C[d..] against C[0..d)
-  auto cgs = [&](int d, int nv) {
-    if (d == 0) return;
-    for (int pass = 0; pass < 2; pass++) {
-      chk(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, d, nv, (int)N, &one, C,
-                        (int)N, col(C, d), (int)N, &zero, S, d),
-          "CGS coefficients");
-      chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nv, d, &mone, C,
-                        (int)N, S, d, &one, col(C, d), (int)N),
-          "CGS update");
-    }
-  };
-  // norms^2 of the nv vectors at base (device nrm[0..nv)), to the host
-  std::vector<double> hn(kNbMax);
-  auto norms2 = [&](const double *base, int nv) {
-    for (int j = 0; j < nv; j++) hn[j] = device_dot(c, base + (size_t)j * N, base + (size_t)j * N, N);
-  };
-  // MGS of C[d..d+nv) among themselves; keeps those whose norm stays above
-  // 1e-10 of ref[j] (norm before the projections), packed at C[d..);
-  // returns how many were kept
-  auto mgs_pack = [&](int d, int nv, const std::vector<double> &ref) {
-    int kept = 0;
-    for (int j = 0; j < nv; j++) {
-      double *v = col(C, d + j);
-      for (int k = 0; k < kept; k++) {
-        const double *X[1] = {col(C, d + k)};
-        dots(c, X, 1, v, c.part);
-        hipLaunchKernelGGL(k_axpy_part, g, bthr, 0, c.stream, v, col(C, d + k), N, c.part, kRedBlocks);
-        check_launch("k_axpy_part");
-      }
-      const double nv2 = device_dot(c, v, v, N);
-      if (!(nv2 > 1e-20 * ref[j]) || !(nv2 > 0.0)) continue;
-      double *dst = col(C, d + kept);
-      if (dst != v)
-        SBDX_HIP(hipMemcpyAsync(dst, v, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
-      const double *X[1] = {dst};
-      dots(c, X, 1, dst, c.part);
-      hipLaunchKernelGGL(k_scale_part, g, bthr, 0, c.stream, dst, N, c.part, kRedBlocks);
-      check_launch("k_scale_part");
-      kept++;
-    }
-    return kept;
-  };
-  auto product = [&](double *x, double *y, int nv, bool approx) {
-    ProfScope ps(c, approx ? "blk_sigma_f32" : "blk_sigma");
-    if (approx)
-      sigma_block_f32(c, nv, x, y);
-    else
-      sigma_block(c, nv, x, y);
-  };
-  // subspace columns [d0, d1): H[i][j] = HC_i . C_j for i <= j
-  auto columns = [&](int d0, int d1) {
-    ProfScope ps(c, "blk_subspace");
-    chk(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, d1, d1 - d0, (int)N, &one,
-                      HC, (int)N, col(C, d0), (int)N, &zero, H + (size_t)ld * d0, ld),
-        "subspace columns");
-    hipLaunchKernelGGL(k_symm_cols, dim3(d1 - d0), dim3(kNbMax), 0, c.stream, H, ld, d0, d1);
-    check_launch("k_symm_cols");
-  };
-  auto ritz = [&](int d, int nv, double *Wd, double *HWd) {
-    subspace_eig(c, d, ld, H, U, E);
-    chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nv, d, &one, C, (int)N, U,
-                      ld, &zero, Wd, (int)N),
-        "Ritz vectors");
-    if (HWd)
-      chk(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (int)N, nv, d, &one, HC, (int)N,
-                        U, ld, &zero, HWd, (int)N),
-          "Ritz products");
-  };
-  // residual norms of nv Ritz pairs (R = HW - E W), to the host
-  std::vector<double> hE(kNbMax), hR(kNbMax);
-  auto residuals = [&](int nv, const double *Wd, const double *HWd, const double *Ed) {
-    hipLaunchKernelGGL(k_block_resid, g, bthr, 0, c.stream, nv, N, Wd, HWd, Ed, c.blk_R, c.part);
-    check_launch("k_block_resid");
-    hipLaunchKernelGGL(k_finish_vec, dim3(1), dim3(kNbMax), 0, c.stream, nv, kRedBlocks, c.part, nrm,
-                       (const int *)nullptr);
-    check_launch("k_finish_vec");
-    SBDX_HIP(hipMemcpyAsync(hE.data(), Ed, sizeof(double) * nv, hipMemcpyDeviceToHost, c.stream));
-    SBDX_HIP(hipMemcpyAsync(hR.data(), nrm, sizeof(double) * nv, hipMemcpyDeviceToHost, c.stream));
-    SBDX_HIP(hipStreamSynchronize(c.stream));
-    for (int j = 0; j < nv; j++) hR[j] = std::sqrt(hR[j]);
-  };
-
-  // coarse start, as in the thick solver: the lowest roots of the problem
-  // on the first coarse_m strings of each set are found by this solver
-  // (same settings, nested levels) to coarse_eps and lifted back as the
-  // start vectors (zero elsewhere; the basis sign depends only on the two
-  // strings).  The sub-problem starts from the given vectors restricted to
-  // it when each keeps at least 1e-3 of its squared norm there, else from
-  // unit vectors on its lowest diagonal elements.  When the sub-problem
-  // cannot start, the given vectors are used as they are.
-  const double *start = W0;
-  std::unique_ptr<TmpBuf> lifted;
-  c.coarse_products = 0;
-  c.coarse_energy = 0.0;
-  c.coarse_seconds = 0.0;
-  const size_t mc = c.coarse_m;
-  if (mc > 0 && mc < c.na && mc < c.nb) {
-    ProfScope ps(c, "coarse_start");
-    const auto tc = std::chrono::steady_clock::now();
-    Context *sub = make_sub_context(c, mc);
-    struct Owner {
-      Context *p;
-      ~Owner() { destroy_context(p); }
-    } owner{sub};
-    sub->precision = c.precision;
-    sub->coarse_m = mc / 4 >= 256 ? mc / 4 : 0;
-    sub->coarse_eps = c.coarse_eps;
-    const size_t Ns = mc * mc;
-    TmpBuf sw0(sizeof(double) * Ns * nroots), sw1(sizeof(double) * Ns * nroots);
-    double *s0 = (double *)sw0.p, *s1 = (double *)sw1.p;
-    SBDX_HIP(hipStreamSynchronize(c.stream));
-    for (int j = 0; j < nroots; j++)
-      SBDX_HIP(hipMemcpy2DAsync(s0 + j * Ns, sizeof(double) * mc, W0 + j * N, sizeof(double) * c.nb,
-                                sizeof(double) * mc, mc, hipMemcpyDeviceToDevice, sub->stream));
-    bool restricted_ok = true;
-    for (int j = 0; j < nroots && restricted_ok; j++) {
-      const double full = device_dot(c, W0 + j * N, W0 + j * N, N);
-      const double part = device_dot(*sub, s0 + j * Ns, s0 + j * Ns, Ns);
-      restricted_ok = part > 1e-3 * full && part > 0.0;
-    }
-    if (!restricted_ok) {
-      std::vector<double> hh(Ns);
-      SBDX_HIP(hipMemcpyAsync(hh.data(), sub->hii, sizeof(double) * Ns, hipMemcpyDeviceToHost, sub->stream));
-      SBDX_HIP(hipStreamSynchronize(sub->stream));
-      std::vector<size_t> idx(Ns);
-      for (size_t i = 0; i < Ns; i++) idx[i] = i;
-      const size_t kk = std::min<size_t>(Ns, (size_t)nroots);
-      std::partial_sort(idx.begin(), idx.begin() + kk, idx.end(),
-                        [&](size_t a, size_t b) { return hh[a] < hh[b] || (hh[a] == hh[b] && a < b); });
-      std::vector<double> h0(Ns * nroots, 0.0);
-      for (size_t k = 0; k < kk; k++) h0[k * Ns + idx[k]] = 1.0;
-      SBDX_HIP(hipMemcpyAsync(s0, h0.data(), sizeof(double) * Ns * nroots, hipMemcpyHostToDevice, sub->stream));
-      SBDX_HIP(hipStreamSynchronize(sub->stream));
-    }
-    std::vector<double> es(nroots);
-    int np = 0;
-    bool ok = true;
-    try {
-      // at most kBlockCoarseIt block iterations: excited roots of the
-      // sub-problem can take hundreds, and partly converged Ritz vectors
-      // already make far better starts than unit vectors
-      (void)davidson_block(*sub, nroots, block, std::min(max_it, kBlockCoarseIt), c.coarse_eps, max_time, mixed,
-                           s0, s1, es.data(), &np);
-    } catch (const Error &) {
-      ok = false;
-    }
-    SBDX_HIP(hipStreamSynchronize(sub->stream));
-    if (ok) {
-      lifted.reset(new TmpBuf(sizeof(double) * N * nroots));
-      double *l = (double *)lifted->p;
-      SBDX_HIP(hipMemsetAsync(l, 0, sizeof(double) * N * nroots, c.stream));
-      for (int j = 0; j < nroots; j++)
-        SBDX_HIP(hipMemcpy2DAsync(l + j * N, sizeof(double) * c.nb, s1 + j * Ns, sizeof(double) * mc,
-                                  sizeof(double) * mc, mc, hipMemcpyDeviceToDevice, c.stream));
-      SBDX_HIP(hipStreamSynchronize(c.stream));
-      start = l;
-      c.coarse_products = np;
-      c.coarse_energy = es[0];
-    }
-    c.coarse_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
-    if (c.verbose) {
-      fprintf(stderr, "[sbdx] block coarse start on %zu x %zu strings: %s, E0 = %.12f, %d products, %.3f s\n", mc,
-              mc, ok ? "used" : "skipped", c.coarse_energy, c.coarse_products, c.coarse_seconds);
-      fflush(stderr);
-    }
-  }
-  // initial block: the given (or coarse-refined) vectors, then unit vectors
-  // on the lowest diagonal elements, orthonormalized
-  SBDX_HIP(hipMemcpyAsync(C, start, sizeof(double) * N * nroots, hipMemcpyDeviceToDevice, c.stream));
-  if (block > nroots) {
-    std::vector<double> hh(N);
-    SBDX_HIP(hipMemcpyAsync(hh.data(), c.hii, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
-    SBDX_HIP(hipStreamSynchronize(c.stream));
-    std::vector<size_t> idx(N);
-    for (size_t i = 0; i < N; i++) idx[i] = i;
-    const size_t kk = std::min<size_t>(N, (size_t)(block - nroots));
-    std::partial_sort(idx.begin(), idx.begin() + kk, idx.end(),
-                      [&](size_t a, size_t b) { return hh[a] < hh[b] || (hh[a] == hh[b] && a < b); });
-    SBDX_HIP(hipMemsetAsync(col(C, nroots), 0, sizeof(double) * N * (block - nroots), c.stream));
-    const double v1 = 1.0;
-    for (size_t k = 0; k < kk; k++)
-      SBDX_HIP(hipMemcpyAsync(col(C, nroots + (int)k) + idx[k], &v1, sizeof(double), hipMemcpyHostToDevice,
-                              c.stream));
-    SBDX_HIP(hipStreamSynchronize(c.stream));
-  }
-  norms2(C, block);
-  int d = mgs_pack(0, block, hn);
-  if (d < nroots) throw Error(1, "davidson_block: start vectors are linearly dependent");
-  product(C, HC, d, false);
-  columns(0, d);
-  int products = d;
-  bool done = false, exact_now = true;
-  c.trace.clear();
-  c.blk_E.assign(nroots, 0.0);
-  std::vector<double> rhist;
-  int diag_left = 0;
-  c.mlp.used_mlp = c.mlp.used_diag = 0;
-  for (int it = 0;; it++) {
-    const int nr = std::min(block, d);
-    {
-      ProfScope ps(c, "blk_ritz");
-      ritz(d, nr, c.blk_W, c.blk_HW);
-      residuals(nr, c.blk_W, c.blk_HW, E);
-    }
-    double rmax = 0.0;
-    for (int j = 0; j < nroots; j++) {
-      rmax = std::max(rmax, hR[j]);
-      c.blk_E[j] = hE[j];
-    }
-    c.trace.push_back(hE[0]);
-    c.trace.push_back(rmax);
-    if (c.verbose) {
-      fprintf(stderr, "[sbdx] block it %d dim %d products %d%s E0 = %.15f max|r| = %.3e\n", it, d, products,
-              exact_now ? " (exact)" : "", hE[0], rmax);
-      fflush(stderr);
-    }
-    if (rmax < eps && exact_now) {
-      if (!mixed) {
-        done = true;
-        break;
-      }
-      // confirm on exact products of the final root vectors
-      ProfScope ps(c, "blk_confirm");
-      product(c.blk_W, c.blk_HW, nroots, false);
-      products += nroots;
-      double *Ec = nrm + kNbMax;  // Rayleigh quotients of the final vectors
-      for (int j = 0; j < nroots; j++) {
-        const double *X[1] = {col(c.blk_W, j)};
-        dots(c, X, 1, col(c.blk_HW, j), c.part);
-        hipLaunchKernelGGL(k_finish_scalar, dim3(1), dim3(1), 0, c.stream, kRedBlocks, c.part, Ec + j);
-        check_launch("k_finish_scalar");
-      }
-      residuals(nroots, c.blk_W, c.blk_HW, Ec);
-      double rx = 0.0;
-      for (int j = 0; j < nroots; j++) rx = std::max(rx, hR[j]);
-      if (c.verbose) {
-        fprintf(stderr, "[sbdx] block confirm E0 = %.15f exact max|r| = %.3e\n", hE[0], rx);
-        fflush(stderr);
-      }
-      if (rx < eps) {
-        for (int j = 0; j < nroots; j++) c.blk_E[j] = hE[j];
-        done = true;
-        break;
-      }
-      exact_now = false;  // not yet: force a restart below
-      rmax = rx;
-    }
-    const bool out_of_time = elapsed() > max_time || it >= max_it;
-    if (d + block > m || (rmax < eps && !exact_now) || out_of_time) {
-      // thick restart on the lowest nk Ritz vectors
-      ProfScope ps(c, "blk_restart");
-      const int nk = out_of_time ? nr : std::min(keep, d);
-      ritz(d, nk, c.blk_W, c.blk_HW);
-      SBDX_HIP(hipMemcpyAsync(C, c.blk_W, sizeof(double) * N * nk, hipMemcpyDeviceToDevice, c.stream));
-      SBDX_HIP(hipMemcpyAsync(HC, c.blk_HW, sizeof(double) * N * nk, hipMemcpyDeviceToDevice, c.stream));
-      if (mixed) {
-        product(C, HC, std::min(nroots, nk), false);
-        products += std::min(nroots, nk);
-      }
-      d = nk;
-      columns(0, d);
-      exact_now = true;
-      if (out_of_time) {
-        // one more Rayleigh-Ritz on the restarted basis for the outputs
-        ritz(d, nroots, c.blk_W, nullptr);
-        SBDX_HIP(hipMemcpyAsync(hE.data(), E, sizeof(double) * nroots, hipMemcpyDeviceToHost, c.stream));
-        SBDX_HIP(hipStreamSynchronize(c.stream));
-        for (int j = 0; j < nroots; j++) c.blk_E[j] = hE[j];
-        break;
-      }
-      continue;  // Rayleigh-Ritz on the restarted basis gives the residuals
-    }
-    // corrections from the lowest nr Ritz pairs
-    rhist.push_back(rmax);
-    const size_t nh = rhist.size();
-    if (nh >= 4 && rhist[nh - 1] > 0.9 * rhist[nh - 4] && diag_left == 0) diag_left = 3;
-    const bool use_mlp = c.precond == 1 && c.mlp.valid && diag_left == 0;
-    if (diag_left > 0) diag_left--;
-    {
-      ProfScope ps(c, use_mlp ? "precond_mlp" : "blk_orth");
-      for (int j = 0; j < nr; j++) {
-        if (use_mlp) {
-          mlp_precond(c, col(c.blk_R, j), E + j, col(C, d + j));
-          c.mlp.used_mlp++;
-        } else {
-          hipLaunchKernelGGL(k_precond, g, bthr, 0, c.stream, col(c.blk_R, j), c.hii, E + j, N, eps_reg,
-                             col(C, d + j));
-          check_launch("k_precond");
-          c.mlp.used_diag++;
-        }
-      }
-    }
-    int nv;
-    {
-      ProfScope ps(c, "blk_orth");
-      norms2(col(C, d), nr);
-      cgs(d, nr);
-      nv = mgs_pack(d, nr, hn);
-    }
-    if (nv == 0) throw Error(5, "davidson_block: no new direction (basis exhausted)");
-    product(col(C, d), col(HC, d), nv, mixed);
-    columns(d, d + nv);
-    products += nv;
-    d += nv;
-    exact_now = !mixed;
-  }
-  SBDX_HIP(hipMemcpyAsync(Wout, c.blk_W, sizeof(double) * N * nroots, hipMemcpyDeviceToDevice, c.stream));
-  SBDX_HIP(hipStreamSynchronize(c.stream));
-  for (int j = 0; j < nroots; j++) Eout[j] = c.blk_E[j];
-  if (n_products) *n_products = products;
-  return done ? 0 : 5;
 }
 
 }  // namespace sbdx

@@ -1546,13 +1546,16 @@ __global__ void __launch_bounds__(256) k_fac_apply(int na, int nb, int L, int r0
   y[o] += g - x[o] * (T)corr;
 }
 
-static void fac_workspace(Context &c) {
-  if (c.fac_D) return;
-  const size_t row = sizeof(double) * (size_t)c.K * c.na;  // one beta row of D
-  c.fac_cap = std::max(row, std::min(kFacBytes, row * c.nb));
-  c.fac_D = dalloc(c, c.fac_cap);
-  c.fac_G = dalloc(c, c.fac_cap);
-  c.fac_xT = dalloc(c, sizeof(double) * c.N);
+// G and x^T on first use; D (in the product's own precision) only for the
+// DGEMM/SGEMM forms, since the split form writes its FP16 operand directly
+static void fac_workspace(Context &c, bool need_D) {
+  if (!c.fac_G) {
+    const size_t row = sizeof(double) * (size_t)c.K * c.na;  // one beta row of D
+    c.fac_cap = std::max(row, std::min(kFacBytes, row * c.nb));
+    c.fac_G = dalloc(c, c.fac_cap);
+    c.fac_xT = dalloc(c, sizeof(double) * c.N);
+  }
+  if (need_D && !c.fac_D) c.fac_D = dalloc(c, c.fac_cap);
 }
 
 // G = E D for the batch: the product's own precision (DGEMM / SGEMM), or for
@@ -1610,7 +1613,7 @@ static const float *fac_gemm(Context &c, int M, const float *eri, const float *D
 
 template <typename T>
 static void mixed_factored(Context &c, const T *eri, const T *x, T *y) {
-  fac_workspace(c);
+  fac_workspace(c, !(sizeof(T) == 4 && c.mixed_form == 2));
   const int na = (int)c.na, nb = (int)c.nb, K = c.K;
   const size_t row = (size_t)K * na;
   // beta rows per batch: as many as fit in the workspace, at most 32768
@@ -1839,6 +1842,12 @@ void sigma_block_f32(Context &c, int k, const double *x, double *y) {
   sigma_block_impl<float>(c, k, c.f32.xb, c.f32.yb);
   hipLaunchKernelGGL((k_cast<float, double>), dim3(g), dim3(256), 0, c.stream, n, c.f32.yb, y);
   check_launch("k_cast");
+}
+
+void sigma_block_f32_raw(Context &c, int k, const float *x, float *y) {
+  if (k < 1) return;
+  ensure_f32(c);
+  sigma_block_impl<float>(c, k, x, y);
 }
 
 void sigma_alpha_first_f32(Context &c, const double *x, double *y) {

@@ -46,7 +46,8 @@ ABI_SYMBOLS = [
     "sbdx_same_spin_form", "sbdx_sigma_block", "sbdx_davidson_block",
     "sbdx_mixed_staging", "sbdx_split_gemm", "sbdx_set_mixed_form", "sbdx_fp64_switch_step", "sbdx_sigma_block_fp32", "sbdx_symmetric_eig", "sbdx_set_symmetry", "sbdx_sector_stats",
     "sbdx_sigma_sym", "sbdx_mlp_apply", "sbdx_set_coarse_start", "sbdx_coarse_stats",
-    "sbdx_set_graphs", "sbdx_rdm_device_noalloc",
+    "sbdx_set_graphs", "sbdx_rdm_device_noalloc", "sbdx_set_block_basis", "sbdx_block_stats",
+    "sbdx_symmetric_geig",
 ]
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -95,6 +96,7 @@ def lib():
     L.sbdx_sigma_fp32.argtypes = [C.c_void_p, _dp, _dp]
     L.sbdx_sigma_block.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
     L.sbdx_symmetric_eig.argtypes = [C.c_int, _dp, _dp, _dp]
+    L.sbdx_symmetric_geig.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
     L.sbdx_set_symmetry.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_rdm_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.sbdx_sector_stats.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -107,6 +109,9 @@ def lib():
     L.sbdx_sigma_block_fp32.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.sbdx_davidson_block.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_double,
                                       C.c_double, _dp, C.POINTER(C.c_int)]
+    L.sbdx_set_block_basis.argtypes = [C.c_void_p, C.c_int]
+    L.sbdx_block_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   _dp, C.c_int]
     L.sbdx_set_restart.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_set_verbose.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_same_spin_form.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
@@ -389,6 +394,19 @@ class TPBOperator:
                                               max_time, E, C.byref(n)), allow=(5,))
         return E, W, n.value, rc == 0
 
+    def set_block_basis(self, m):
+        """Basis size of the block solver (0: as large as memory allows, up
+        to 64)."""
+        _check(lib().sbdx_set_block_basis(self.h, int(m)))
+
+    def block_stats(self, nroots):
+        """Basis size, Ritz vectors kept per restart, restarts and final
+        residual norms of the last block solve."""
+        m, k, r = C.c_int(), C.c_int(), C.c_int()
+        res = np.zeros(nroots)
+        _check(lib().sbdx_block_stats(self.h, C.byref(m), C.byref(k), C.byref(r), res, nroots))
+        return dict(basis=m.value, keep=k.value, restarts=r.value, resid=[float(x) for x in res])
+
     def trace(self):
         n = C.c_int()
         _check(lib().sbdx_trace(self.h, None, 0, C.byref(n)))
@@ -468,6 +486,20 @@ def symmetric_eig(A):
     E = np.zeros(n)
     _check(lib().sbdx_symmetric_eig(n, a, V, E))
     return E, V.reshape(n, n).T
+
+
+def symmetric_geig(H, S):
+    """Eigenvalues (ascending) and S-orthonormal eigenvectors (columns) of
+    H y = E S y by the block solver's Rayleigh-Ritz step (host code, no
+    GPU)."""
+    H = np.asarray(H, dtype=np.float64)
+    S = np.asarray(S, dtype=np.float64)
+    n = H.shape[0]
+    Y = np.zeros(n * n)
+    E = np.zeros(n)
+    _check(lib().sbdx_symmetric_geig(n, np.ascontiguousarray(H.T).reshape(-1),
+                                     np.ascontiguousarray(S.T).reshape(-1), Y, E))
+    return E, Y.reshape(n, n).T
 
 
 def basis_init_vector(ndet):

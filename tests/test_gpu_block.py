@@ -93,37 +93,80 @@ def test_block_ground_state_matches_oracle(oracle_mod):
 
 # Coarse start of the block solver: the lowest roots on the first 100 of
 # 275 strings of each set found first, then lifted as the start vectors.
-# Same roots as without it, in FP64 and in mixed precision, whether the
-# sub-problem starts from the given vectors or (when they vanish there) from
-# its own lowest diagonal elements.
+# The lowest three eigenvalues come from ARPACK (scipy eigsh, random start)
+# on the FP64 product, which is pinned to the oracle elsewhere; the block
+# solver must find them without the stage, with it, in FP64 and in mixed
+# precision, and from start vectors that vanish on the first 100 strings
+# (the sub-problem then starts from its own lowest diagonal elements).  h2o
+# has spatial symmetry, so start vectors of one symmetry alone cannot reach
+# roots of another: the extra start vectors' random component is what lets
+# every start find the same three.
 @pytest.mark.parametrize("solver", ["plain", "mixed"])
 def test_block_coarse_start(solver, oracle_mod):
+    import scipy.sparse.linalg as sla
     import sbdx
     inst = molecule("h2o", "1em3")
     op = sbdx.TPBOperator.from_instance(inst)
+    lin = sla.LinearOperator((op.ndet, op.ndet), matvec=lambda v: op.sigma(np.ascontiguousarray(v).ravel()),
+                             dtype=np.float64)
+    ev = np.sort(sla.eigsh(lin, k=3, which="SA", tol=1e-13,
+                           v0=np.random.default_rng(0).standard_normal(op.ndet))[0])
     op.configure(solver)
-    print(f"[block coarse] h2o 3 roots, block 4, {solver}", flush=True)
+    print(f"[block coarse] h2o 3 roots, block 4, {solver}; eigsh {ev}", flush=True)
     E0, _, n0, c0 = op.davidson_block(nroots=3, block=4, eps=1e-9, max_it=300)
     op.set_coarse_start(100, 1e-3)
     E1, W1, n1, c1 = op.davidson_block(nroots=3, block=4, eps=1e-9, max_it=300)
     st = op.coarse_stats()
-    print(f"[block coarse] products {n0} -> {n1} (+{st['products']} coarse), E {E1}", flush=True)
+    print(f"[block coarse] products {n0} -> {n1} (+{st['products']} coarse), E {E0} {E1}", flush=True)
     assert c0 and c1
     assert st["products"] > 0
-    assert np.max(np.abs(E1 - E0)) < 1e-9
+    assert np.max(np.abs(E0 - ev)) < 1e-9
+    assert np.max(np.abs(E1 - ev)) < 1e-9
     orc = oracle_mod.Oracle.from_instance(inst)
     for j in range(3):
         assert np.linalg.norm(orc.sigma(W1[j]) - E1[j] * W1[j]) < 1e-8
-    # start vectors vanishing on the first 100 strings (from which alone
-    # block Davidson finds eigenpairs, not necessarily the lowest three): the
-    # sub-problem starts from its own lowest diagonal elements instead
     Wout = np.zeros((3, op.ndet))
     for j in range(3):
         Wout[j, (200 + j) * op.nb + 230 + 3 * j] = 1.0
     E2, W2, n2, c2 = op.davidson_block(nroots=3, block=4, W0=Wout, eps=1e-9, max_it=300)
+    print(f"[block coarse] vanishing start: E {E2}", flush=True)
     assert op.coarse_stats()["products"] > 0
-    assert c2 and np.max(np.abs(E2 - E0)) < 1e-9
+    assert c2 and np.max(np.abs(E2 - ev)) < 1e-9
     for j in range(3):
         assert np.linalg.norm(orc.sigma(W2[j]) - E2[j] * W2[j]) < 1e-8
+    op.close()
+    orc.close()
+
+
+# Basis size: the default (as large as memory allows, 64 here) and the
+# smallest allowed one (nroots + 2 block) reach the same roots; the stats
+# report what was used.  In mixed mode the correction columns are FP32 in
+# storage (Rayleigh-Ritz with the overlap matrix) and the roots still match
+# the FP64 ones to 1e-9.  A basis below nroots + 2 block is refused.
+@pytest.mark.parametrize("solver", ["plain", "mixed"])
+def test_block_basis_size(solver, oracle_mod):
+    import sbdx
+    inst = molecule("h2o", "1em3")
+    op = sbdx.TPBOperator.from_instance(inst)
+    op.configure(solver)
+    print(f"[block basis] h2o 3 roots, block 3, {solver}", flush=True)
+    E0, W0, n0, c0 = op.davidson_block(nroots=3, block=3, eps=1e-9, max_it=400)
+    s0 = op.block_stats(3)
+    op.set_block_basis(9)
+    E1, W1, n1, c1 = op.davidson_block(nroots=3, block=3, eps=1e-9, max_it=400)
+    s1 = op.block_stats(3)
+    print(f"[block basis] m={s0['basis']} keep={s0['keep']}: {n0} products, {s0['restarts']} restarts; "
+          f"m={s1['basis']} keep={s1['keep']}: {n1} products, {s1['restarts']} restarts", flush=True)
+    assert c0 and c1
+    assert s0["basis"] == 64 and s1["basis"] == 9 and s1["keep"] == 6
+    assert np.max(np.abs(E1 - E0)) < 1e-9
+    assert max(s0["resid"]) < 1e-9 and max(s1["resid"]) < 1e-9
+    orc = oracle_mod.Oracle.from_instance(inst)
+    for j in range(3):
+        assert np.linalg.norm(orc.sigma(W1[j]) - E1[j] * W1[j]) < 1e-8
+        assert np.linalg.norm(orc.sigma(W0[j]) - E0[j] * W0[j]) < 1e-8
+    op.set_block_basis(8)
+    with pytest.raises(sbdx.SbdxError):
+        op.davidson_block(nroots=3, block=3, eps=1e-9, max_it=10)
     op.close()
     orc.close()
