@@ -267,12 +267,12 @@ def main():
     # which is spin-flip symmetric)
     for _ in range(args.warmup):
         op.sigma(W0)
-        if args.solver == "mixed":
+        if args.solver in ("mixed", "block"):
             op.sigma_fp32(W0)
-        if symmetric and args.solver in ("thick", "mixed"):
+        if symmetric and args.solver in ("thick", "mixed", "block"):
             e0 = sbdx.basis_init_vector(op.ndet)
             op.sigma_sym(e0)
-            if args.solver == "mixed":
+            if args.solver in ("mixed", "block"):
                 op.sigma_sym(e0, fp32=True)
     log(f"warmup done; same-spin form {op.same_spin_forms()}")
     # one FP64 sigma through the C ABI with host buffers in and out (PCIe
@@ -290,6 +290,7 @@ def main():
         op.configure(solver)
         if solver in ("thick", "mixed"):
             op.set_restart(args.keep)
+        if solver in ("thick", "mixed", "block"):
             op.set_symmetry(args.sectors if symmetric else "off")
         if solver in ("thick", "mixed", "block"):
             op.set_coarse_start(args.coarse if args.coarse < min(na, nb) else 0, args.coarse_eps)

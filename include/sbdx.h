@@ -112,8 +112,10 @@ int sbdx_davidson(sbdx_handle_t h, double *W, int max_it, int nb, double eps,
  * products are all FP64.  With SBDX_PREC_MIXED the correction vectors and
  * their products are FP32 in arithmetic and in storage (dot products and
  * combinations in FP64, Rayleigh-Ritz with the overlap matrix of the stored
- * columns), the roots are FP64 with exact FP64 products from every restart
- * on, and convergence is confirmed on a restart that keeps the roots alone.
+ * columns), the roots are FP64 (with exact FP64 products at every restart
+ * once max|r| < max(100 eps, 6e-5 |E|)), and convergence is confirmed on a
+ * restart that
+ * keeps the roots alone, with exact products.
  * The preconditioner is the handle's (sbdx_set_preconditioner).
  * W: host, nroots consecutive n_det vectors, linearly independent start
  * vectors on entry (block - nroots more are unit vectors on the lowest
@@ -133,9 +135,11 @@ int sbdx_davidson_block(sbdx_handle_t h, int nroots, int block, double *W, int m
 int sbdx_set_block_basis(sbdx_handle_t h, int m);
 
 /* What the last sbdx_davidson_block used: basis size, Ritz vectors kept at
- * a restart, restarts, and the final residual norms of the nroots roots
- * (resid may be null).  No counterpart in the reference. */
-int sbdx_block_stats(sbdx_handle_t h, int *basis, int *keep, int *restarts, double *resid, int nroots);
+ * a restart, restarts, and per root (nroots entries, pointers may be null)
+ * the final residual norm and the spin-flip sector (+1 even, -1 odd, 0
+ * when sectors are off).  No counterpart in the reference. */
+int sbdx_block_stats(sbdx_handle_t h, int *basis, int *keep, int *restarts, double *resid, int *parity,
+                     int nroots);
 
 /* Split form of sbdx_davidson for callers that keep the vector resident:
  * upload the start vector, solve on the device, download the solution. */
@@ -233,7 +237,7 @@ int sbdx_symmetric_eig(int n, const double *A, double *V, double *E);
  * by Cholesky S = L L^T and sbdx_symmetric_eig of L^-1 H L^-T; E[n]
  * ascending, S-orthonormal Y (columns, n x n column-major).  No
  * counterpart in the reference (its Davidson keeps an orthonormal basis);
- * needs no handle or GPU.  SBDX_ERR_NOT_CONVERGED (5) when S is not
+ * needs no handle or GPU.  SBDX_ERR_UNSUPPORTED (3) when S is not
  * numerically positive definite. */
 int sbdx_symmetric_geig(int n, const double *H, const double *S, double *Y, double *E);
 

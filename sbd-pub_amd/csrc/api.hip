@@ -427,7 +427,8 @@ int sbdx_set_block_basis(sbdx_handle_t h, int m) {
   SBDX_GUARD_END
 }
 
-int sbdx_block_stats(sbdx_handle_t h, int *basis, int *keep, int *restarts, double *resid, int nroots) {
+int sbdx_block_stats(sbdx_handle_t h, int *basis, int *keep, int *restarts, double *resid, int *parity,
+                     int nroots) {
   SBDX_GUARD_BEGIN
   Context &c = ctx_of(h);
   if (basis) *basis = c.blk_m_used;
@@ -435,6 +436,8 @@ int sbdx_block_stats(sbdx_handle_t h, int *basis, int *keep, int *restarts, doub
   if (restarts) *restarts = c.blk_restarts;
   if (resid)
     for (int j = 0; j < nroots; j++) resid[j] = j < (int)c.blk_resid.size() ? c.blk_resid[j] : 0.0;
+  if (parity)
+    for (int j = 0; j < nroots; j++) parity[j] = j < (int)c.blk_par.size() ? c.blk_par[j] : 0;
   return SBDX_OK;
   SBDX_GUARD_END
 }

@@ -28,11 +28,11 @@
 // product; the new columns of H = HC^T C (entry i <= j from the older
 // column's product HC_i . C_j) and of S.  Thick restart when the next block
 // would not fit: the basis becomes the lowest `keep` Ritz vectors and their
-// products, combined in place; in mixed mode the roots get exact FP64
-// products then, so the residuals checked right after a restart are exact
-// up to second-order terms.  Convergence: |R_j| < eps for every root on a
-// restarted basis (mixed mode: the restart keeps the roots alone, so every
-// column and product is FP64).
+// products, combined in place; in mixed mode, once max|r| is below 100 eps
+// or near the FP32 products' error level, the roots get exact FP64
+// products at every restart.  Convergence: |R_j| < eps
+// for every root on a basis whose columns and products are all FP64 (mixed
+// mode: a confirming restart keeps the roots alone, with exact products).
 //
 // Every reduction is a fixed tree (fixed grid, fixed shuffle pattern, fixed
 // order over blocks), so a solve is reproducible run to run.
@@ -53,6 +53,8 @@ constexpr int kBG = 1024;     // blocks: a fixed grid, so reductions are fixed t
 constexpr int kOutMax = 32;   // columns written by one combination pass
 constexpr int kResMax = 16;   // residuals of one pass (the largest block)
 constexpr int kCoarseIt = 25;  // iteration cap of the coarse stage
+// error level of an FP32 product relative to |E| (1000 x 2^-24)
+constexpr double kFp32Level = 1000.0 * 5.9604644775390625e-8;
 
 // Basis columns: the first nd in FP64 storage, then nf in FP32 storage.
 struct Cols {
@@ -377,7 +379,12 @@ bool host_cholesky(int n, const double *S, int ld, std::vector<double> &L) {
 // Q^T (host_syev), Y = L^-T Q (S-orthonormal columns), ascending.
 void host_sygv(int n, const double *H, const double *S, int ld, double *Y, double *E) {
   std::vector<double> L;
-  if (!host_cholesky(n, S, ld, L)) throw Error(5, "block Davidson: basis overlap matrix is not positive definite");
+  if (!host_cholesky(n, S, ld, L)) {
+    std::string diag;
+    for (int i = 0; i < n && i < 24; i++) diag += " " + std::to_string(S[i + (size_t)ld * i]);
+    throw Error(3, "block Davidson: basis overlap matrix (order " + std::to_string(n) +
+                       ") is not positive definite; diagonal:" + diag);
+  }
   // A = L^-1 H L^-T: first B = L^-1 H (forward substitution per column),
   // then A = L^-1 B^T (H symmetric, so (L^-1 H L^-T)^T = L^-1 (L^-1 H)^T)
   std::vector<double> B((size_t)n * n), A((size_t)n * n), Q((size_t)n * n);
@@ -415,22 +422,36 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
     throw Error(1, "davidson_block: need 1 <= nroots <= block <= 16");
   const size_t N = c.N;
   const dim3 g(kBG), bt(kBT);
+  // spin-flip sectors (equal string sets): every column is even (X = X^T)
+  // or odd (X = -X^T) and its product is the symmetric one of its sector;
+  // H and S are block-diagonal by sector, so the Rayleigh-Ritz problem is
+  // solved per sector and every Ritz vector belongs to one.  Mode 1: even
+  // sector, 2: odd sector, 3: both in one basis: the lowest roots of H, and
+  // the lowest pair of a sector that holds none of them is converged too
+  // (and corrected every iteration), so that neither sector is left
+  // unexplored.
+  const int symmode = c.symmetric;
+  if (symmode && !sigma_sym_available(c))
+    throw Error(1, "spin-flip sectors need equal alpha and beta string sets and room for the dense blocks");
+  const int nrep = symmode == 3 ? 1 : 0;  // at most one sector holds no root
+  const int nslot = block + nrep;  // residual slots
+  if (nslot > kResMax) throw Error(1, "davidson_block: with both sectors, block must be below 16");
   // ---- basis size and storage
-  const int r64cap = mixed ? nroots : kNbMax;  // FP64 columns (all of them in FP64 mode)
-  auto keep_of = [&](int m) { return std::min({kOutMax, m - block, std::max(nroots + block, m / 3)}); };
-  const int m_min = nroots + 2 * block;
+  const int r64cap = mixed ? nroots + nrep : kNbMax;  // FP64 columns (all of them in FP64 mode)
+  auto keep_of = [&](int m) { return std::min({kOutMax, m - nslot, std::max(nroots + nslot, m / 3)}); };
+  const int m_min = nroots + nrep + 2 * nslot;
   auto need = [&](int m) {
     const int r64 = std::min(m, r64cap), nf = m - r64;
-    return (double)N * (8.0 * (2.0 * r64 + block) + 4.0 * 2.0 * nf);
+    return (double)N * (8.0 * (2.0 * r64 + nslot) + 4.0 * 2.0 * nf);
   };
   int m = c.blk_m_req > 0 ? c.blk_m_req : kNbMax;
   if (m < m_min || m > kNbMax)
-    throw Error(1, "davidson_block: basis size must lie in [nroots + 2*block, 64] (" + std::to_string(m_min) +
-                       " here)");
+    throw Error(1, "davidson_block: basis size must lie in [nroots + 2*block (+3 with both sectors), 64] (" +
+                       std::to_string(m_min) + " here)");
   // buffers of an earlier solve are reused when they were sized for the
   // same request (mode, FP64 column count, block, requested basis size)
   const bool same_layout = c.blk_C64 && c.blk_mixed == (int)mixed && c.blk_r64cap == r64cap &&
-                           c.blk_block >= block && c.blk_m_alloc_req == c.blk_m_req;
+                           c.blk_block >= nslot && c.blk_m_alloc_req == c.blk_m_req;
   if (!same_layout) {
     for (void *p : {(void *)c.blk_C64, (void *)c.blk_HC64, (void *)c.blk_C32, (void *)c.blk_HC32,
                     (void *)c.blk_R})
@@ -442,7 +463,7 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
     SBDX_HIP(hipMemGetInfo(&free_b, &total_b));
     const double budget = 0.8 * (double)free_b;
     if (c.blk_m_req <= 0)
-      while (need(m) > budget && m - block >= m_min) m -= block;
+      while (need(m) > budget && m - nslot >= m_min) m -= nslot;
     if (need(m) > budget)
       throw Error(4, "davidson_block: a basis of " + std::to_string(m) + " vectors does not fit in device memory");
     const int r64 = std::min(m, r64cap), nf = m - r64;
@@ -452,12 +473,12 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
       c.blk_C32 = (float *)dalloc(c, sizeof(float) * N * nf);
       c.blk_HC32 = (float *)dalloc(c, sizeof(float) * N * nf);
     }
-    c.blk_R = (double *)dalloc(c, sizeof(double) * N * block);
+    c.blk_R = (double *)dalloc(c, sizeof(double) * N * nslot);
     c.blk_m = m;
     c.blk_r64 = r64;
     c.blk_r64cap = r64cap;
     c.blk_mixed = mixed;
-    c.blk_block = block;
+    c.blk_block = nslot;
     c.blk_m_alloc_req = c.blk_m_req;
   }
   m = c.blk_m;
@@ -465,14 +486,27 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
   if (m < m_min) throw Error(1, "davidson_block: basis too small for this block");
   const int keep = keep_of(m);
   if (!c.blk_part) c.blk_part = (double *)dalloc(c, sizeof(double) * kBG * 2 * kNbMax * kResMax);
-  if (!c.blk_small) c.blk_small = (double *)dalloc(c, sizeof(double) * (kNbMax * kNbMax + 2 * kNbMax * kResMax + 64));
-  double *U = c.blk_small;                           // kNbMax x kNbMax, column-major
-  double *Ed = U + kNbMax * kNbMax;                  // Ritz values
-  double *coef = Ed + kNbMax;                        // CGS coefficients (d x nv)
+  if (!c.blk_small)
+    c.blk_small = (double *)dalloc(c, sizeof(double) * (2 * kNbMax * kNbMax + kNbMax * kResMax + 4 * kNbMax));
+  double *U = c.blk_small;                           // kNbMax x kNbMax, column-major: all Ritz pairs, ascending
+  double *Ed = U + kNbMax * kNbMax;                  // their values
+  double *Us = Ed + kNbMax;                          // selected pairs (residuals, restarts)
+  double *Es = Us + kNbMax * kNbMax;                 // their values
+  double *coef = Es + kNbMax;                        // CGS coefficients (d x nv)
   double *nrm = coef + kNbMax * kResMax;             // residual norms^2
   double *part = c.blk_part;
   c.blk_m_used = m;
   c.blk_keep_used = keep;
+  std::vector<int> par(kNbMax, 0), rpar(kNbMax, 0);  // column parities, Ritz-pair parities
+  struct SectorGuard {  // no graph replays here (columns alternate sectors); restore the caller's state
+    Context &c;
+    int graphs, spar;
+    ~SectorGuard() {
+      c.use_graphs = graphs;
+      c.sym_par = spar;
+    }
+  } sguard{c, c.use_graphs, c.sym_par};
+  c.use_graphs = 0;
 
   auto col_d = [&](double *b64, float *b32, int k) -> std::pair<double *, float *> {
     if (k < r64) return {b64 + (size_t)k * N, nullptr};
@@ -508,6 +542,7 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
   // host copies of H and S (column-major, ld kNbMax)
   std::vector<double> H((size_t)kNbMax * kNbMax, 0.0), S((size_t)kNbMax * kNbMax, 0.0);
   std::vector<double> hU((size_t)kNbMax * kNbMax, 0.0), hE(kNbMax, 0.0), hR(kResMax, 0.0);
+  std::vector<double> hUs((size_t)kNbMax * kNbMax, 0.0), hEs(kNbMax, 0.0);
   std::vector<double> gbuf(2 * kNbMax * kNbMax);
   // G = A^T B (na x nb, row-major a*nb + b) of two column lists, to dst
   // (device) and, when host is given, to the host
@@ -534,7 +569,7 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
       for (int i = 0; i < d1; i++)
         for (int jj = 0; jj < nb; jj++) {
           const int j = j0 + jj;
-          const double v = gbuf[(size_t)i * nb + jj];  // HC_i . C_j
+          const double v = par[i] * par[j] < 0 ? 0.0 : gbuf[(size_t)i * nb + jj];  // HC_i . C_j
           if (i <= j) {
             H[i + (size_t)kNbMax * j] = v;
             H[j + (size_t)kNbMax * i] = v;
@@ -544,7 +579,7 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
       for (int i = 0; i < d1; i++)
         for (int jj = 0; jj < nb; jj++) {
           const int j = j0 + jj;
-          const double v = gbuf[(size_t)i * nb + jj];
+          const double v = par[i] * par[j] < 0 ? 0.0 : gbuf[(size_t)i * nb + jj];
           if (i <= j) {
             S[i + (size_t)kNbMax * j] = v;
             S[j + (size_t)kNbMax * i] = v;
@@ -554,22 +589,83 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
   };
   // Rayleigh-Ritz on the d-column basis: U, E on the device and host
   auto rayleigh_ritz = [&](int d) {
-    std::vector<double> Y((size_t)d * d), Hs((size_t)d * d), Ss((size_t)d * d);
-    for (int j = 0; j < d; j++)
-      for (int i = 0; i < d; i++) {
-        Hs[i + (size_t)d * j] = H[i + (size_t)kNbMax * j];
-        Ss[i + (size_t)d * j] = S[i + (size_t)kNbMax * j];
+    // per sector (one group when sectors are off): the group's columns,
+    // its pairs, merged in ascending order (ties: even first)
+    std::vector<std::pair<double, std::pair<int, std::vector<double>>>> pairs;
+    for (int sgn : {1, -1, 0}) {
+      std::vector<int> idx;
+      for (int k = 0; k < d; k++)
+        if (par[k] == sgn) idx.push_back(k);
+      const int q = (int)idx.size();
+      if (!q) continue;
+      std::vector<double> Y((size_t)q * q), Hs((size_t)q * q), Ss((size_t)q * q), Eq(q);
+      for (int b = 0; b < q; b++)
+        for (int a = 0; a < q; a++) {
+          Hs[a + (size_t)q * b] = H[idx[a] + (size_t)kNbMax * idx[b]];
+          Ss[a + (size_t)q * b] = S[idx[a] + (size_t)kNbMax * idx[b]];
+        }
+      host_sygv(q, Hs.data(), Ss.data(), q, Y.data(), Eq.data());
+      for (int b = 0; b < q; b++) {
+        std::vector<double> u(d, 0.0);
+        for (int a = 0; a < q; a++) u[idx[a]] = Y[a + (size_t)q * b];
+        pairs.push_back({Eq[b], {sgn, std::move(u)}});
       }
-    host_sygv(d, Hs.data(), Ss.data(), d, Y.data(), hE.data());
+    }
+    std::stable_sort(pairs.begin(), pairs.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
     std::fill(hU.begin(), hU.end(), 0.0);
-    for (int j = 0; j < d; j++)
-      for (int i = 0; i < d; i++) hU[i + (size_t)kNbMax * j] = Y[i + (size_t)d * j];
+    for (int j = 0; j < d; j++) {
+      hE[j] = pairs[j].first;
+      rpar[j] = pairs[j].second.first;
+      for (int i = 0; i < d; i++) hU[i + (size_t)kNbMax * j] = pairs[j].second.second[i];
+    }
     SBDX_HIP(hipMemcpyAsync(U, hU.data(), sizeof(double) * kNbMax * d, hipMemcpyHostToDevice, c.stream));
     SBDX_HIP(hipMemcpyAsync(Ed, hE.data(), sizeof(double) * d, hipMemcpyHostToDevice, c.stream));
     SBDX_HIP(hipStreamSynchronize(c.stream));  // hU and hE are rewritten by the next call
   };
+  // Us, Es: the Ritz pairs L[0..) of the last Rayleigh-Ritz step, in order
+  auto upload_sel = [&](const std::vector<int> &L, int d) {
+    std::fill(hUs.begin(), hUs.end(), 0.0);
+    for (size_t q = 0; q < L.size(); q++) {
+      hEs[q] = hE[L[q]];
+      for (int i = 0; i < d; i++) hUs[i + (size_t)kNbMax * q] = hU[i + (size_t)kNbMax * L[q]];
+    }
+    SBDX_HIP(hipMemcpyAsync(Us, hUs.data(), sizeof(double) * kNbMax * L.size(), hipMemcpyHostToDevice, c.stream));
+    SBDX_HIP(hipMemcpyAsync(Es, hEs.data(), sizeof(double) * L.size(), hipMemcpyHostToDevice, c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+  };
+  // the pairs that must converge: the nroots lowest, and with both sectors
+  // the lowest pair of a sector that holds none of them
+  auto required = [&](int d) {
+    std::vector<int> req;
+    for (int j = 0; j < std::min(nroots, d); j++) req.push_back(j);
+    if (symmode == 3)
+      for (int sgn : {1, -1}) {
+        bool has = false;
+        for (int j : req) has = has || rpar[j] == sgn;
+        if (has) continue;
+        for (int j = 0; j < d; j++)
+          if (rpar[j] == sgn) {
+            req.push_back(j);
+            break;
+          }
+      }
+    return req;
+  };
   auto product = [&](int k0, int k1) {  // HC[k0,k1) from C[k0,k1)
     if (k1 <= k0) return;
+    if (symmode) {  // one symmetric product per column, in its sector
+      for (int k = k0; k < k1; k++) {
+        c.sym_par = par[k];
+        if (k < r64) {
+          ProfScope ps(c, "blk_sigma_sym");
+          sigma_sym(c, c.blk_C64 + (size_t)k * N, c.blk_HC64 + (size_t)k * N);
+        } else {
+          ProfScope ps(c, "blk_sigma_sym_f32");
+          sigma_sym_f32_raw(c, c.blk_C32 + (size_t)(k - r64) * N, c.blk_HC32 + (size_t)(k - r64) * N);
+        }
+      }
+      return;
+    }
     const int a = std::min(k0, r64), b = std::min(k1, r64);
     if (b > a) {
       ProfScope ps(c, "blk_sigma");
@@ -596,9 +692,10 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
   // vector whose squared norm falls to 1e-20 of ref[j] or to zero is
   // dropped; returns the kept ones with their norms (not yet scaled)
   auto mgs = [&](const std::vector<double *> &vs, const std::vector<double> &ref, std::vector<double *> &kept,
-                 std::vector<double> &knorm) {
+                 std::vector<double> &knorm, std::vector<int> &kidx) {
     kept.clear();
     knorm.clear();
+    kidx.clear();
     for (size_t j = 0; j < vs.size(); j++) {
       double *v = vs[j];
       for (size_t k = 0; k < kept.size(); k++) {
@@ -610,6 +707,7 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
       if (!(n2 > 1e-20 * ref[j]) || !(n2 > 0.0)) continue;
       kept.push_back(v);
       knorm.push_back(std::sqrt(n2));
+      kidx.push_back((int)j);
     }
   };
 
@@ -636,6 +734,7 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
       ~Owner() { destroy_context(p); }
     } owner{sub};
     sub->precision = c.precision;
+    if (symmode && sigma_sym_available(*sub)) sub->symmetric = symmode;
     sub->coarse_m = mc / 4 >= 256 ? mc / 4 : 0;
     sub->coarse_eps = c.coarse_eps;
     const size_t Ns = mc * mc;
@@ -706,40 +805,95 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
   // representation, spin) never leaves it in exact arithmetic; the random
   // component gives the basis a share of every sector, so the solver finds
   // the lowest roots whatever symmetry they have.
-  std::vector<double *> Rs(block);
-  for (int j = 0; j < block; j++) Rs[j] = c.blk_R + (size_t)j * N;
-  SBDX_HIP(hipMemcpyAsync(c.blk_R, start, sizeof(double) * N * nroots, hipMemcpyDeviceToDevice, c.stream));
-  if (block > nroots) {
-    std::vector<double> hh(N);
-    SBDX_HIP(hipMemcpyAsync(hh.data(), c.hii, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
-    SBDX_HIP(hipStreamSynchronize(c.stream));
-    std::vector<size_t> idx(N);
-    for (size_t i = 0; i < N; i++) idx[i] = i;
-    const size_t kk = std::min<size_t>(N, (size_t)(block - nroots));
-    std::partial_sort(idx.begin(), idx.begin() + kk, idx.end(),
-                      [&](size_t a, size_t b) { return hh[a] < hh[b] || (hh[a] == hh[b] && a < b); });
-    hipLaunchKernelGGL(k_zero, g, bt, 0, c.stream, Rs[nroots], N * (block - nroots));
-    check_launch("k_zero");
-    const double v1 = 1.0;
-    for (size_t k = 0; k < kk; k++)
-      SBDX_HIP(hipMemcpyAsync(Rs[nroots + k] + idx[k], &v1, sizeof(double), hipMemcpyHostToDevice, c.stream));
-    SBDX_HIP(hipStreamSynchronize(c.stream));
-    // |u|^2 ~ N/3 for uniform u in [-1, 1): scale to ~0.01 squared norm
-    const double s = 0.1 / std::sqrt((double)N / 3.0);
-    for (int j = nroots; j < block; j++) {
-      hipLaunchKernelGGL(k_add_noise, g, bt, 0, c.stream, Rs[j], N, 0x5bd1e995ull * (unsigned long long)(j + 1), s);
-      check_launch("k_add_noise");
+  // With spin-flip sectors every start column is projected on one sector
+  // (mode 3: the even and the odd part of each start vector, then extra
+  // vectors alternately even and odd) and all of them get the random
+  // component.  A projection that keeps less than 1e-6 of the start
+  // vector's squared norm (the odd part of a closed-shell start) is
+  // replaced by the unit pair on the lowest diagonal element of its sector
+  // (odd: the lowest off-diagonal hii[a,b], as X[a,b] = -X[b,a] = 1).
+  std::vector<double *> Rs(nslot);
+  for (int j = 0; j < nslot; j++) Rs[j] = c.blk_R + (size_t)j * N;
+  std::vector<int> src(block), spar(block, 0);  // source (start j, or -1-k: k-th extra), sector
+  {
+    int nextra = 0;
+    for (int t = 0; t < block; t++) {
+      if (symmode == 3) {
+        spar[t] = t % 2 == 0 ? 1 : -1;
+        src[t] = t < 2 * nroots ? t / 2 : -1 - nextra++;
+      } else {
+        spar[t] = symmode == 1 ? 1 : symmode == 2 ? -1 : 0;
+        src[t] = t < nroots ? t : -1 - nextra++;
+      }
     }
+    // the block + 1 lowest diagonal elements, over all determinants and
+    // over those off the string-exchange diagonal (a != b), by one host pass
+    std::vector<size_t> low_all, low_off;
+    if (nextra > 0 || symmode) {
+      std::vector<double> hh(N);
+      SBDX_HIP(hipMemcpyAsync(hh.data(), c.hii, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
+      SBDX_HIP(hipStreamSynchronize(c.stream));
+      const size_t kk = (size_t)block + 1, nbs = c.nb;
+      auto less = [&](size_t a, size_t b) { return hh[a] < hh[b] || (hh[a] == hh[b] && a < b); };
+      auto push = [&](std::vector<size_t> &v, size_t i) {
+        if (v.size() == kk && !less(i, v.back())) return;
+        v.insert(std::upper_bound(v.begin(), v.end(), i, less), i);
+        if (v.size() > kk) v.pop_back();
+      };
+      for (size_t i = 0; i < N; i++) {
+        push(low_all, i);
+        if (i / nbs != i % nbs) push(low_off, i);
+      }
+    }
+    // |u|^2 ~ N/3 for uniform u in [-1, 1): scale to ~0.01 squared norm
+    const double sn = 0.1 / std::sqrt((double)N / 3.0);
+    const size_t nbs = c.nb;
+    auto unit = [&](double *v, int p, size_t k) {  // unit vector (pair) of sector p on the k-th lowest element
+      hipLaunchKernelGGL(k_zero, g, bt, 0, c.stream, v, N);
+      check_launch("k_zero");
+      const std::vector<size_t> &L = p < 0 ? low_off : low_all;
+      if (k >= L.size()) return;
+      const size_t i = L[k], a = i / nbs, b = i % nbs;
+      const double one = 1.0, sgn = p < 0 ? -1.0 : 1.0;
+      SBDX_HIP(hipMemcpyAsync(v + i, &one, sizeof(double), hipMemcpyHostToDevice, c.stream));
+      if (p && a != b) SBDX_HIP(hipMemcpyAsync(v + b * nbs + a, &sgn, sizeof(double), hipMemcpyHostToDevice, c.stream));
+      SBDX_HIP(hipStreamSynchronize(c.stream));
+    };
+    int nrepl = 0;
+    for (int t = 0; t < block; t++) {
+      if (src[t] >= 0) {
+        SBDX_HIP(hipMemcpyAsync(Rs[t], start + (size_t)src[t] * N, sizeof(double) * N, hipMemcpyDeviceToDevice,
+                                c.stream));
+        if (spar[t]) {
+          const double n0 = device_dot(c, Rs[t], Rs[t], N);
+          symmetrize(c, Rs[t], spar[t]);
+          if (!(device_dot(c, Rs[t], Rs[t], N) > 1e-6 * n0)) unit(Rs[t], spar[t], (size_t)nrepl++);
+        }
+      } else {
+        unit(Rs[t], spar[t], (size_t)(-1 - src[t]) + (symmode ? (size_t)nrepl : 0));
+      }
+      if (src[t] < 0 || symmode) {
+        hipLaunchKernelGGL(k_add_noise, g, bt, 0, c.stream, Rs[t], N, 0x5bd1e995ull * (unsigned long long)(t + 1),
+                           sn);
+        check_launch("k_add_noise");
+      }
+      if (spar[t]) symmetrize(c, Rs[t], spar[t]);
+    }
+    SBDX_HIP(hipStreamSynchronize(c.stream));
   }
   int d = 0;
   {
     std::vector<double> ref(block);
     for (int j = 0; j < block; j++) ref[j] = device_dot(c, Rs[j], Rs[j], N);
-    std::vector<double *> kept;
+    std::vector<double *> kept, first(Rs.begin(), Rs.begin() + block);
     std::vector<double> kn;
-    mgs(Rs, ref, kept, kn);
+    std::vector<int> ki;
+    mgs(first, ref, kept, kn, ki);
     if ((int)kept.size() < nroots) throw Error(1, "davidson_block: start vectors are linearly dependent");
-    for (size_t k = 0; k < kept.size(); k++) store((int)k, kept[k], 1.0 / kn[k]);
+    for (size_t k = 0; k < kept.size(); k++) {
+      par[k] = spar[ki[k]];
+      store((int)k, kept[k], 1.0 / kn[k]);
+    }
     d = (int)kept.size();
   }
   product(0, d);
@@ -756,16 +910,25 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
   std::unique_ptr<TmpBuf> mlp_tmp;
   const double eps_reg = 1.0e-12;
   for (int it = 0;; it++) {
-    const int nres = std::min(d, block);
     // the learned preconditioner unless the residual stalled (below)
     const bool use_mlp_now = c.precond == 1 && c.mlp.valid && diag_left == 0;
+    std::vector<int> req, P;  // required pairs; pairs whose residuals are formed (ascending)
+    int nres = 0;
     {
       ProfScope ps(c, "blk_ritz");
       rayleigh_ritz(d);
+      req = required(d);
+      P = req;
+      const int cap = std::max((int)req.size(), std::min(d, block));
+      for (int j = 0; j < d && (int)P.size() < cap; j++)
+        if (std::find(P.begin(), P.end(), j) == P.end()) P.push_back(j);
+      std::sort(P.begin(), P.end());
+      nres = (int)P.size();
+      upload_sel(P, d);
       OutD o{};
-      for (int j = 0; j < nres; j++) o.p[j] = Rs[j];
-      hipLaunchKernelGGL(k_bresid, g, bt, 0, c.stream, cols(false, 0, d), cols(true, 0, d), (const double *)U,
-                         kNbMax, (const double *)Ed, nres, (const double *)c.hii, eps_reg, use_mlp_now ? 0 : 1, o,
+      for (int q = 0; q < nres; q++) o.p[q] = Rs[q];
+      hipLaunchKernelGGL(k_bresid, g, bt, 0, c.stream, cols(false, 0, d), cols(true, 0, d), (const double *)Us,
+                         kNbMax, (const double *)Es, nres, (const double *)c.hii, eps_reg, use_mlp_now ? 0 : 1, o,
                          N, part);
       check_launch("k_bresid");
       hipLaunchKernelGGL(k_bsum, dim3(1), dim3(256), 0, c.stream, nres, kBG, part, nrm);
@@ -774,9 +937,11 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
       SBDX_HIP(hipStreamSynchronize(c.stream));
       for (int j = 0; j < nres; j++) hR[j] = std::sqrt(hR[j]);
     }
+    // P is ascending and holds 0..nroots-1 first, so hR[j] is root j's
     double rmax = 0.0;
+    auto resid_of = [&](int j) { return hR[std::find(P.begin(), P.end(), j) - P.begin()]; };
+    for (int j : req) rmax = std::max(rmax, resid_of(j));
     for (int j = 0; j < nroots; j++) {
-      rmax = std::max(rmax, hR[j]);
       c.blk_E[j] = hE[j];
       c.blk_resid[j] = hR[j];
     }
@@ -785,7 +950,7 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
     if (c.verbose) {
       fprintf(stderr, "[sbdx] block it %d dim %d products %d%s E0 = %.15f max|r| = %.3e |r_j|", it, d, products,
               exact_now ? " (exact)" : "", hE[0], rmax);
-      for (int j = 0; j < nroots; j++) fprintf(stderr, " %.2e", hR[j]);
+      for (int j : req) fprintf(stderr, " %.2e%s", resid_of(j), symmode == 3 ? (rpar[j] > 0 ? "e" : "o") : "");
       fprintf(stderr, "\n");
       fflush(stderr);
     }
@@ -794,20 +959,40 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
       break;
     }
     const bool out_of_time = elapsed() > max_time || it >= max_it;
-    if (d + block > m || (rmax < eps && !exact_now) || out_of_time) {
-      // ---- thick restart on the lowest nk Ritz vectors, in place; to
-      // confirm convergence (or at the end) the roots alone, so that every
-      // column and product is FP64 in mixed mode
+    if (d + nslot > m || (rmax < eps && !exact_now) || out_of_time) {
+      // ---- thick restart on the lowest Ritz vectors (and the required
+      // ones), in place; to confirm convergence (or at the end) the
+      // required pairs alone, so that every column and product is FP64 in
+      // mixed mode
       ProfScope ps(c, "blk_restart");
-      const int nk = (out_of_time || rmax < eps) ? nroots : std::min(keep, d);
+      std::vector<int> K;
+      if (out_of_time || rmax < eps) {
+        K = req;
+      } else {
+        for (int j = 0; j < std::min(keep, d); j++) K.push_back(j);
+        for (int j : req)
+          if (std::find(K.begin(), K.end(), j) == K.end()) K.back() = j;  // past the roots, so never one of them
+        std::sort(K.begin(), K.end());
+      }
+      const int nk = (int)K.size();
+      upload_sel(K, d);
       {
         OutCols oc = outcols(false, 0, nk), oh = outcols(true, 0, nk);
-        hipLaunchKernelGGL(k_bcomb, g, bt, 0, c.stream, cols(false, 0, d), (const double *)U, kNbMax, oc, N);
+        hipLaunchKernelGGL(k_bcomb, g, bt, 0, c.stream, cols(false, 0, d), (const double *)Us, kNbMax, oc, N);
         check_launch("k_bcomb");
-        hipLaunchKernelGGL(k_bcomb, g, bt, 0, c.stream, cols(true, 0, d), (const double *)U, kNbMax, oh, N);
+        hipLaunchKernelGGL(k_bcomb, g, bt, 0, c.stream, cols(true, 0, d), (const double *)Us, kNbMax, oh, N);
         check_launch("k_bcomb");
       }
-      if (mixed) {
+      // the restarted columns' sectors, before their exact products
+      for (int j = 0; j < nk; j++) par[j] = rpar[K[j]];
+      // exact FP64 products of the roots: on the confirming restart, and on
+      // every restart once max|r| falls below 100 eps or below the level
+      // where FP32 products' errors (1000 x 2^-24 |E|) can hold the
+      // residuals up; before that the roots' products stay combinations of
+      // the basis products
+      double emax = 0.0;
+      for (int j = 0; j < nroots; j++) emax = std::max(emax, std::fabs(hE[j]));
+      if (mixed && (nk <= r64 || rmax < std::max(100.0 * eps, kFp32Level * emax))) {
         const int ne = std::min(nk, r64);
         product(0, ne);
         products += ne;
@@ -832,17 +1017,21 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
     if (diag_left > 0) diag_left--;
     else if (nh >= 4 && rhist[nh - 1] > 0.9 * rhist[nh - 4]) diag_left = 3;
     std::vector<double *> sel;
-    std::vector<int> selj;
-    for (int j = 0; j < nres; j++)
-      if (!(j < nroots && hR[j] < eps)) {
-        sel.push_back(Rs[j]);
-        selj.push_back(j);
+    std::vector<int> selj;  // Ritz index of each correction
+    std::vector<int> selq;  // its slot in P
+    for (int q = 0; q < nres; q++) {
+      const bool is_req = std::find(req.begin(), req.end(), P[q]) != req.end();
+      if (!(is_req && hR[q] < eps)) {
+        sel.push_back(Rs[q]);
+        selj.push_back(P[q]);
+        selq.push_back(q);
       }
+    }
     if (use_mlp_now) {
       ProfScope ps(c, "precond_mlp");
       if (!mlp_tmp) mlp_tmp.reset(new TmpBuf(sizeof(double) * N));
       for (size_t q = 0; q < sel.size(); q++) {
-        mlp_precond(c, sel[q], Ed + selj[q], (double *)mlp_tmp->p);
+        mlp_precond(c, sel[q], Es + selq[q], (double *)mlp_tmp->p);
         SBDX_HIP(hipMemcpyAsync(sel[q], mlp_tmp->p, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
         c.mlp.used_mlp++;
       }
@@ -865,11 +1054,19 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
       }
       std::vector<double *> kept;
       std::vector<double> kn;
-      mgs(sel, ref, kept, kn);
+      std::vector<int> ki;
+      mgs(sel, ref, kept, kn, ki);
       nv = (int)kept.size();
-      for (int q = 0; q < nv; q++) store(d + q, kept[q], 1.0 / kn[q]);
+      for (int q = 0; q < nv; q++) {
+        // each correction stays in its Ritz pair's sector; projecting it
+        // there removes the rounding-level share of the other sector that
+        // the Gram-Schmidt coefficients leave
+        par[d + q] = rpar[selj[ki[q]]];
+        if (par[d + q]) symmetrize(c, kept[q], par[d + q]);
+        store(d + q, kept[q], 1.0 / kn[q]);
+      }
     }
-    if (nv == 0) throw Error(5, "davidson_block: no new direction (basis exhausted)");
+    if (nv == 0) throw Error(3, "davidson_block: no new direction (basis exhausted)");
     product(d, d + nv);
     products += nv;
     columns(d, d + nv);
@@ -885,6 +1082,7 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
   }
   SBDX_HIP(hipStreamSynchronize(c.stream));
   for (int j = 0; j < nroots; j++) Eout[j] = c.blk_E[j];
+  c.blk_par.assign(rpar.begin(), rpar.begin() + nroots);
   if (n_products) *n_products = products;
   return done ? 0 : 5;
 }

@@ -213,6 +213,7 @@ struct Context {
   int blk_m = 0, blk_r64 = 0, blk_r64cap = 0, blk_mixed = -1, blk_block = 0, blk_m_alloc_req = -1;
   int blk_m_req = 0, blk_m_used = 0, blk_keep_used = 0, blk_restarts = 0;
   std::vector<double> blk_E, blk_resid;  // Ritz values and residual norms of the last block solve
+  std::vector<int> blk_par;              // and the roots' spin-flip sectors (+1, -1; 0: sectors off)
   MlpState mlp;
   int precond = 0;         // 0: diagonal R/(E0-d); 1: MLP with diagonal fallback on stall
   // work vectors
@@ -293,6 +294,7 @@ void sigma_block(Context &c, int k, const double *x, double *y);
 bool sigma_sym_available(Context &c);
 void sigma_sym(Context &c, const double *x, double *y);
 void sigma_sym_f32(Context &c, const double *x, double *y);
+void sigma_sym_f32_raw(Context &c, const float *x, float *y);  // FP32 vectors in and out, no graph
 double sym_defect(Context &c, const double *v, int p);  // max|v - p v^T| / max|v|, p = +-1
 void symmetrize(Context &c, double *v, int p);         // v = (v + p v^T)/2
 void sigma_block_f32(Context &c, int k, const double *x, double *y);

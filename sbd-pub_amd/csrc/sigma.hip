@@ -1768,6 +1768,11 @@ void graphs_release(Context &c) {
   c.g_sym32 = nullptr;
 }
 
+void sigma_sym_f32_raw(Context &c, const float *x, float *y) {
+  ensure_f32(c);
+  sigma_sym_impl<float>(c, x, y);
+}
+
 void sigma_sym_f32(Context &c, const double *x, double *y) {
   ensure_f32(c);
   const unsigned g = (unsigned)std::min<size_t>(cdiv(c.N, 256), 65536);

@@ -111,7 +111,7 @@ def lib():
                                       C.c_double, _dp, C.POINTER(C.c_int)]
     L.sbdx_set_block_basis.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_block_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                   _dp, C.c_int]
+                                   _dp, C.c_void_p, C.c_int]
     L.sbdx_set_restart.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_set_verbose.argtypes = [C.c_void_p, C.c_int]
     L.sbdx_same_spin_form.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
@@ -273,10 +273,12 @@ class TPBOperator:
         _check(lib().sbdx_set_restart(self.h, int(keep)))
 
     def set_symmetry(self, mode):
-        """Spin-flip sectors of the thick-restart solves (equal string sets):
-        0/False general form, 1/True/"even" the even sector (symmetric start
-        vector), 2/"odd" the odd sector (antisymmetric start vector),
-        3/"lowest" both sectors, the lower result returned."""
+        """Spin-flip sectors of the thick-restart and block solves (equal
+        string sets): 0/False general form, 1/True/"even" the even sector
+        (symmetric start vector), 2/"odd" the odd sector (antisymmetric start
+        vector), 3/"lowest" both sectors (thick: the lower result returned;
+        block: one basis holding both sectors' columns, the lowest roots of
+        H)."""
         names = {"off": 0, "even": 1, "odd": 2, "lowest": 3}
         m = names[mode] if isinstance(mode, str) else (1 if mode is True else int(mode))
         _check(lib().sbdx_set_symmetry(self.h, m))
@@ -404,8 +406,11 @@ class TPBOperator:
         residual norms of the last block solve."""
         m, k, r = C.c_int(), C.c_int(), C.c_int()
         res = np.zeros(nroots)
-        _check(lib().sbdx_block_stats(self.h, C.byref(m), C.byref(k), C.byref(r), res, nroots))
-        return dict(basis=m.value, keep=k.value, restarts=r.value, resid=[float(x) for x in res])
+        par = np.zeros(nroots, dtype=np.int32)
+        _check(lib().sbdx_block_stats(self.h, C.byref(m), C.byref(k), C.byref(r), res,
+                                      par.ctypes.data_as(C.c_void_p), nroots))
+        return dict(basis=m.value, keep=k.value, restarts=r.value, resid=[float(x) for x in res],
+                    sectors=[int(x) for x in par])
 
     def trace(self):
         n = C.c_int()

@@ -170,3 +170,55 @@ def test_block_basis_size(solver, oracle_mod):
         op.davidson_block(nroots=3, block=3, eps=1e-9, max_it=10)
     op.close()
     orc.close()
+
+
+# Spin-flip sectors in the block solver (equal string sets): with both
+# sectors in one basis the roots are the lowest of H (ARPACK on the general
+# FP64 product), each root vector lies in one sector (P W = W or P W = -W
+# for the exchange P of the two strings, in the reference's convention) and
+# the reported sector matches it; restricted to the even sector, every root
+# is even and none lies below the lowest roots.
+@pytest.mark.parametrize("solver", ["plain", "mixed"])
+def test_block_spin_flip_sectors(solver, oracle_mod):
+    import scipy.sparse.linalg as sla
+    import sbdx
+    inst = molecule("h2o", "1em3")
+    assert np.array_equal(inst["astr"], inst["bstr"])
+    op = sbdx.TPBOperator.from_instance(inst)
+    lin = sla.LinearOperator((op.ndet, op.ndet), matvec=lambda v: op.sigma(np.ascontiguousarray(v).ravel()),
+                             dtype=np.float64)
+    ev = np.sort(sla.eigsh(lin, k=4, which="SA", tol=1e-13,
+                           v0=np.random.default_rng(1).standard_normal(op.ndet))[0])
+    op.configure(solver)
+    op.set_verbose(1)
+    op.set_symmetry("lowest")
+    print(f"[block sectors] h2o 4 roots, block 4, {solver}; eigsh {ev}", flush=True)
+    E, W, n, conv = op.davidson_block(nroots=4, block=4, eps=1e-9, max_it=400)
+    st = op.block_stats(4)
+    print(f"[block sectors] E {E} sectors {st['sectors']} products {n}", flush=True)
+    assert conv
+    assert np.max(np.abs(E - ev)) < 1e-9
+    from test_gpu_symmetry import exchange
+    orc = oracle_mod.Oracle.from_instance(inst)
+    for j in range(4):
+        p = st["sectors"][j]
+        assert p in (1, -1)
+        assert np.max(np.abs(exchange(inst, W[j]) - p * W[j])) < 1e-12 * np.max(np.abs(W[j]))
+        assert np.linalg.norm(orc.sigma(W[j]) - E[j] * W[j]) < 1e-8
+    # two roots, block 2: the lowest state of the odd sector lies below the
+    # second even one, and the odd sector must not be left unexplored
+    E2, W2, n2, c2 = op.davidson_block(nroots=2, block=2, eps=1e-9, max_it=400)
+    s2 = op.block_stats(2)
+    print(f"[block sectors] 2 roots: E {E2} sectors {s2['sectors']}", flush=True)
+    assert c2 and np.max(np.abs(E2 - ev[:2])) < 1e-9 and s2["sectors"] == [1, -1]
+    op.set_symmetry("even")
+    Ee, We, ne, ce = op.davidson_block(nroots=2, block=2, eps=1e-9, max_it=400)
+    se = op.block_stats(2)
+    print(f"[block sectors] even: E {Ee} sectors {se['sectors']}", flush=True)
+    assert ce and se["sectors"] == [1, 1]
+    assert Ee[0] >= ev[0] - 1e-9 and Ee[1] >= ev[1] - 1e-9
+    for j in range(2):
+        assert np.max(np.abs(exchange(inst, We[j]) - We[j])) < 1e-12 * np.max(np.abs(We[j]))
+        assert np.linalg.norm(orc.sigma(We[j]) - Ee[j] * We[j]) < 1e-8
+    op.close()
+    orc.close()
