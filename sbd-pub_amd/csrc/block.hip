@@ -939,7 +939,10 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
     }
     // P is ascending and holds 0..nroots-1 first, so hR[j] is root j's
     double rmax = 0.0;
-    auto resid_of = [&](int j) { return hR[std::find(P.begin(), P.end(), j) - P.begin()]; };
+    auto resid_of = [&](int j) {  // +inf for a pair whose residual was not formed
+      const auto it = std::find(P.begin(), P.end(), j);
+      return it == P.end() ? HUGE_VAL : hR[it - P.begin()];
+    };
     for (int j : req) rmax = std::max(rmax, resid_of(j));
     for (int j = 0; j < nroots; j++) {
       c.blk_E[j] = hE[j];
@@ -992,10 +995,16 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
       // the basis products
       double emax = 0.0;
       for (int j = 0; j < nroots; j++) emax = std::max(emax, std::fabs(hE[j]));
-      if (mixed && (nk <= r64 || rmax < std::max(100.0 * eps, kFp32Level * emax))) {
-        const int ne = std::min(nk, r64);
-        product(0, ne);
-        products += ne;
+      // (on a normal restart only the roots not yet below eps: a converged
+      // root's column barely changes and the confirming restart recomputes
+      // it anyway)
+      const bool confirm = nk <= r64;
+      if (mixed && (confirm || rmax < std::max(100.0 * eps, kFp32Level * emax))) {
+        for (int j = 0; j < std::min(nk, r64); j++)
+          if (confirm || resid_of(K[j]) >= eps) {
+            product(j, j + 1);
+            products++;
+          }
       }
       d = nk;
       c.blk_restarts++;
