@@ -250,7 +250,9 @@ int sbdx_symmetric_geig(int n, const double *H, const double *S, double *Y, doub
  * sigma needs one same-spin GEMM G = H X, y = hii*x + (G +- G^T), and the
  * mixed term for the upper triangle only, mirrored with the sector's sign --
  * about half the work of the general form.  The mode applies to the
- * thick-restart solves (sbdx_set_restart > 0):
+ * thick-restart solves (sbdx_set_restart > 0) and to sbdx_davidson_block
+ * (whose columns each lie in one sector; LOWEST keeps both sectors in one
+ * basis and returns the lowest roots of H):
  *   SBDX_SYM_OFF     the general form;
  *   SBDX_SYM_EVEN    the even sector: the start vector must be symmetric
  *                    (the Hartree-Fock vector of BasisInitVector is), the
