@@ -75,6 +75,7 @@ class MultTPBHip {
                       (double)I0, reinterpret_cast<const double *>(I1.store.data()),
                       reinterpret_cast<const double *>(I2.store.data()), I2.store.size()));
     norbs_ = norbs;
+    own_hii_.clear();
     check(sbdx_dims(h_, &na_, &nb_, &ndet_));
   }
 
@@ -84,13 +85,22 @@ class MultTPBHip {
     check(sbdx_diag(h_, reinterpret_cast<double *>(hii.data())));
   }
 
-  // Wb += H Wk (MultBase::run); the diagonal term uses the operator's own
-  // hii, which equals what makeQChamDiagTerms returns.
-  void run(const std::vector<ElemT> & /*hii*/, const std::vector<ElemT> &Wk,
-           std::vector<ElemT> &Wb) const {
+  // Wb += H Wk (MultBase::run) with the caller's diagonal hii: the device
+  // applies its own (ZeroExcite per determinant, what makeQChamDiagTerms
+  // returns), and where the caller's hii differs from it the difference
+  // times Wk is added on the host.  An empty hii means the operator's own.
+  void run(const std::vector<ElemT> &hii, const std::vector<ElemT> &Wk, std::vector<ElemT> &Wb) const {
     if (Wk.size() != ndet_ || Wb.size() != ndet_) throw std::invalid_argument("sbdx: vector size");
+    if (!hii.empty() && hii.size() != ndet_) throw std::invalid_argument("sbdx: hii size");
     check(sbdx_sigma(h_, reinterpret_cast<const double *>(Wk.data()),
                      reinterpret_cast<double *>(Wb.data()), 1));
+    if (hii.empty()) return;
+    if (own_hii_.size() != ndet_) {
+      own_hii_.resize(ndet_);
+      check(sbdx_diag(h_, own_hii_.data()));
+    }
+    for (size_t i = 0; i < ndet_; i++)
+      if ((double)hii[i] != own_hii_[i]) Wb[i] += (ElemT)(((double)hii[i] - own_hii_[i]) * (double)Wk[i]);
   }
 
   // one[s][i + L j], two[s + 2t][i + L j + L^2 k + L^3 l] (Correlation)
@@ -113,10 +123,13 @@ class MultTPBHip {
  private:
   sbdx_handle_t h_ = nullptr;
   size_t norbs_ = 0, na_ = 0, nb_ = 0, ndet_ = 0;
+  mutable std::vector<double> own_hii_;  // the device diagonal, fetched on first use by run
 };
 
 // Davidson(hii, W, mult, max_it, nb, eps, max_time): W in/out; returns the
-// lowest Ritz value of the last step.
+// lowest Ritz value of the last step.  The solve runs on the device with the
+// operator's own diagonal, which is what the reference passes as hii
+// (makeQChamDiagTerms).
 template <typename ElemT, typename RealT>
 ElemT Davidson(const std::vector<ElemT> & /*hii*/, std::vector<ElemT> &W, MultTPBHip<ElemT> &mult,
                int max_iteration, int num_block, RealT eps, RealT max_time) {

@@ -29,7 +29,24 @@ int main(int argc, char **argv) {
     for (double d : density) tr += d;
     for (auto &b : two)
       for (double x : b) n2 += x * x;
+    // run(hii, Wk, Wb) with the caller's hii: a diagonal shifted on every
+    // 7th determinant must change H x by exactly that shift times x there
+    double hdiff = 0.0;
+    {
+      sbdx::MultTPBHip<double> op;
+      op.Init(adet, adet, bl, (size_t)g.L, g.I0, g.I1, g.I2);
+      std::vector<double> hii;
+      op.makeQChamDiagTerms(hii);
+      const size_t n = hii.size();
+      std::vector<double> x(n), y1(n, 0.0), y2(n, 0.0), h2 = hii;
+      for (size_t i = 0; i < n; i++) x[i] = std::sin(0.37 * (double)i + 0.1);
+      for (size_t i = 0; i < n; i += 7) h2[i] += 0.5 + 1e-3 * (double)(i % 11);
+      op.run(hii, x, y1);
+      op.run(h2, x, y2);
+      for (size_t i = 0; i < n; i++) hdiff = std::fmax(hdiff, std::fabs((y2[i] - y1[i]) - (h2[i] - hii[i]) * x[i]));
+    }
     std::printf("norb %d nstr %zu\n", g.L, adet.size());
+    std::printf("hii_check %.3e\n", hdiff);
     std::printf("energy %.12f\n", energy);
     std::printf("electrons %.12f\n", tr);
     std::printf("rdm2_norm %.12e\n", std::sqrt(n2));

@@ -89,3 +89,59 @@ def test_cpp_caller_h2o(tmp_path):
     vals = dict(line.split()[:2] for line in r.stdout.strip().splitlines())
     assert abs(float(vals["energy"]) - (-76.23594663)) < 6e-9
     assert abs(float(vals["electrons"]) - 10.0) < 1e-9
+    assert float(vals["hii_check"]) < 1e-12  # run() applies the caller's hii
+
+
+REF = os.environ.get("SBD_REFERENCE", "/root/reference")
+MPICXX = "/opt/conda/bin/mpicxx"
+
+
+@pytest.mark.skipif(not (os.path.isdir(os.path.join(REF, "include", "sbd")) and os.path.exists(MPICXX)),
+                    reason="needs the reference tree and the MPI compiler of the build image")
+def test_adapter_compiles_against_reference_headers(tmp_path):
+    """The drop-in branch (the SBDX_HIP lines a maintainer adds to
+    chemistry/tpb/sbdiag.h) compiled together with the reference's own
+    headers and integral types: sbd::oneInt<double> / sbd::twoInt<double>
+    (chemistry/basic/integrals.h:11-90) and the variable types of
+    sbd::tpb::diag (sbdiag.h:162-187, 822), with the reference build's
+    compiler and flags (oracle/build_ref.sh) and -DSBDX_HIP."""
+    src = tmp_path / "branch.cc"
+    src.write_text("""
+// the include sequence of the reference app
+// (apps/chemistry_tpb_selected_basis_diagonalization/main.cc:1-13)
+#include <iostream>
+#include <fstream>
+#include <chrono>
+#include <random>
+#include <deque>
+#include <unistd.h>
+#define _USE_MATH_DEFINES
+#include <cmath>
+#include "sbd/sbd.h"
+#include "mpi.h"
+#ifdef SBDX_HIP
+#include "sbdx_tpb.hpp"
+#endif
+// the variables of sbd::tpb::diag the branch uses, with their types there
+void branch(const std::vector<std::vector<size_t>> &adet, const std::vector<std::vector<size_t>> &bdet,
+            size_t bit_length, int L, double I0, const sbd::oneInt<double> &I1,
+            const sbd::twoInt<double> &I2, int max_it, int max_nb, double eps, double max_time,
+            std::vector<double> &W, std::vector<double> &C, std::vector<std::vector<double>> &one_p_rdm,
+            std::vector<std::vector<double>> &two_p_rdm) {
+#ifdef SBDX_HIP
+  sbdx::MultTPBHip<double> device_mult;
+  device_mult.Init(adet, bdet, bit_length, static_cast<size_t>(L), I0, I1, I2);
+  std::vector<double> hii;
+  device_mult.makeQChamDiagTerms(hii);
+  sbdx::Davidson(hii, W, device_mult, max_it, max_nb, eps, max_time);
+  device_mult.run(hii, W, C);
+  device_mult.correlation(W, one_p_rdm, two_p_rdm);
+#endif
+}
+int main() { return 0; }
+""")
+    env = dict(os.environ, MPICH_CXX="g++")
+    r = subprocess.run([MPICXX, "-std=c++17", "-fopenmp", "-w", "-DSBDX_HIP", "-I", os.path.join(REF, "include"),
+                        "-I", "/opt/conda/include", "-I", INC, "-c", str(src), "-o", str(tmp_path / "branch.o")],
+                       capture_output=True, text=True, env=env)
+    assert r.returncode == 0, r.stderr[-4000:]

@@ -131,3 +131,75 @@ def test_diag_writes_rdm_text(tmp_path):
     g1, _ = sbdx.io.read_rdm_text(str(tmp_path), out["norb"])
     L = out["norb"]
     assert abs(sum(g1[i + L * i] for i in range(L)) - 10) < 1e-8  # h2o: 10 electrons
+
+
+# ---- against the reference driver's own files (tests/golden/rdm_text_h2o.npz,
+# made by tests/golden/make_rdm_text.py from the reference app built from its
+# sources, apps/chemistry_tpb_selected_basis_diagonalization/main.cc:264-318,
+# on the h2o-1em3 data set)
+def _ref_rdm_text():
+    from conftest import GOLDEN
+    d = np.load(os.path.join(GOLDEN, "rdm_text_h2o.npz"))
+    lines1 = d["lines1"].tobytes()
+    lines2 = d["lines2"].tobytes().split(b"\n")[:-1]
+    return d, lines1, dict(zip(d["idx2"].tolist(), lines2))
+
+
+def test_rdm_text_bytes_match_reference(tmp_path):
+    """The writer, fed the reference's printed values, reproduces the
+    reference's files byte for byte: 1pRDM.txt whole (its SHA-256), and
+    every sampled line of 2pRDM.txt (index order, separators, 16
+    significant digits, exponent form) with the same line count."""
+    import hashlib
+    from sbdx import io
+    d, lines1, sample2 = _ref_rdm_text()
+    L = 24
+    one = np.zeros((2, L * L))
+    two = np.zeros((4, L ** 4))
+    for ln in lines1.decode().splitlines():
+        i, j, v = ln.split()
+        one[0][int(i) + L * int(j)] = float(v)
+    for ln in sample2.values():
+        i, j, k, l, v = ln.decode().split()
+        two[0][int(i) + L * int(j) + L * L * (int(k) + L * int(l))] = float(v)
+    io.write_rdm_text(str(tmp_path), L, one, two)
+    b1 = open(tmp_path / "1pRDM.txt", "rb").read()
+    assert b1 == lines1
+    assert hashlib.sha256(b1).hexdigest().encode() == d["sha1p"].tobytes()
+    out2 = open(tmp_path / "2pRDM.txt", "rb").read().split(b"\n")[:-1]
+    assert len(out2) == int(d["nlines2"][0])
+    bad = [n for n, ln in sample2.items() if out2[n] != ln]
+    assert not bad, (bad[:5], [out2[n] for n in bad[:3]], [sample2[n] for n in bad[:3]])
+
+
+def test_rdm_of_oracle_ground_state_matches_reference(oracle_mod):
+    """The oracle's ground state of h2o-1em3 (Davidson to 1e-10, as the
+    reference app was run) gives, through the RDM and the energy
+    decomposition of this build, the reference driver's printed energies
+    (1e-9) and its 1pRDM.txt / sampled 2pRDM.txt values (1e-8)."""
+    import sbdx
+    from conftest import molecule
+    from sbdx import io
+    d, lines1, sample2 = _ref_rdm_text()
+    inst = molecule("h2o", "1em3")
+    L = inst["norb"]
+    orc = oracle_mod.Oracle.from_instance(inst)
+    res = orc.davidson(W=sbdx.basis_init_vector(len(inst["astr"]) ** 2), nb=12, eps=1e-10, max_it=300)
+    assert res["converged"]
+    one, two = orc.rdm(res["W"])
+    orc.close()
+    one = np.asarray(one).reshape(2, -1)
+    two = np.asarray(two).reshape(4, -1)
+    E = io.energy_decomposition(inst["I0"], io.spatial_h(L, inst["I1"]), io.dense_eri(L, inst["I2"]), one, two)
+    Etot, E0, E1, E2 = d["energies"]
+    assert abs(res["E"] - Etot) < 1e-9
+    assert E["zero"] == E0
+    assert abs(E["one"] - E1) < 1e-8 and abs(E["two"] - E2) < 1e-8
+    g1 = one[0] + one[1]
+    for ln in lines1.decode().splitlines():
+        i, j, v = ln.split()
+        assert abs(g1[int(i) + L * int(j)] - float(v)) < 1e-8
+    g2 = ((two[0] + two[1]) + two[2]) + two[3]
+    err = max(abs(g2[int(i) + L * int(j) + L * L * (int(k) + L * int(l))] - float(v))
+              for i, j, k, l, v in (ln.decode().split() for ln in sample2.values()))
+    assert err < 1e-8
