@@ -18,7 +18,9 @@
 #define SBDX_IO_HPP
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdlib>
+#include <numeric>
 #include <fstream>
 #include <map>
 #include <sstream>
@@ -228,6 +230,126 @@ inline void load_wavefunction(const std::string &file, const std::vector<std::ve
       if (pb != ib.end()) W[pa->second * bdet.size() + pb->second] += lw[a * head[1] + b];
     }
   }
+}
+
+// ---------------------------------------------------------------- carry-over
+// The carry-over string selection of the reference's SQD loop on one rank
+// (chemistry/tpb/rdmat.h:10-230, chemistry/tpb/extend.h:14-324, applied by
+// chemistry/tpb/sbdiag.h:692-735 after the final energy).  Strings are
+// masks (orbital i -> bit i), W is alpha-major (W[ia*nb + ib]).  String
+// weights: D_a[ia] = sum_ib W[ia,ib]^2, D_b[ib] = sum_ia W[ia,ib]^2.
+//   type 1, threshold > 0 (ratio 0): the fewest strings, heaviest first, whose
+//     weights sum to more than 1 - threshold;
+//   type 1, ratio > 0: the int(ratio * n) heaviest strings, and the weight
+//     left out (truncated_alpha / truncated_beta);
+//   type 2: either of these, then every selected string with all its single
+//     excitations, sorted, without duplicates;
+//   type 3: the strings of the determinants with W^2 > threshold, with all
+//     their singles, sorted, without duplicates, and the summed weight of
+//     those determinants (total_weight).
+// Equal weights keep the lower index first (the reference's std::sort leaves
+// their order unspecified); where the threshold rule finds no prefix above
+// 1 - threshold (W not normalized) every string is kept (the reference
+// reads past the end of its list there).
+struct Carryover {
+  std::vector<uint64_t> adet, bdet;
+  double truncated_alpha = 0.0, truncated_beta = 0.0, total_weight = 0.0;
+};
+
+namespace carryover_detail {
+inline std::vector<size_t> by_weight(const std::vector<double> &D) {
+  std::vector<size_t> o(D.size());
+  std::iota(o.begin(), o.end(), size_t(0));
+  std::stable_sort(o.begin(), o.end(), [&](size_t i, size_t j) { return D[i] > D[j]; });
+  return o;
+}
+inline std::vector<uint64_t> threshold_select(const std::vector<uint64_t> &s, const std::vector<double> &D,
+                                              double threshold) {
+  const auto o = by_weight(D);
+  double sum = 0.0;
+  size_t k = 0;
+  while (k < o.size()) {
+    sum += D[o[k++]];
+    if (sum > 1.0 - threshold) break;
+  }
+  std::vector<uint64_t> r(k);
+  for (size_t i = 0; i < k; i++) r[i] = s[o[i]];
+  return r;
+}
+inline std::vector<uint64_t> ratio_select(const std::vector<uint64_t> &s, const std::vector<double> &D,
+                                          double ratio, double &truncated) {
+  const auto o = by_weight(D);
+  const size_t k = std::min(o.size(), (size_t)(ratio * (double)D.size()));
+  std::vector<uint64_t> r(k);
+  double sum = 0.0;
+  for (size_t i = 0; i < k; i++) {
+    r[i] = s[o[i]];
+    sum += D[o[i]];
+  }
+  truncated = 1.0 - sum;
+  return r;
+}
+// the strings and all their single excitations, ascending, unique
+inline std::vector<uint64_t> singles_extend(const std::vector<uint64_t> &s, int norb) {
+  std::vector<uint64_t> r(s.begin(), s.end());
+  for (uint64_t m : s)
+    for (int c = 0; c < norb; c++) {
+      if (!((m >> c) & 1)) continue;
+      for (int o = 0; o < norb; o++)
+        if (!((m >> o) & 1)) r.push_back((m & ~(1ull << c)) | (1ull << o));
+    }
+  std::sort(r.begin(), r.end());
+  r.erase(std::unique(r.begin(), r.end()), r.end());
+  return r;
+}
+}  // namespace carryover_detail
+
+inline Carryover carryover(const std::vector<double> &W, const std::vector<uint64_t> &adet,
+                           const std::vector<uint64_t> &bdet, int norb, int type, double threshold,
+                           double ratio) {
+  namespace cd = carryover_detail;
+  const size_t na = adet.size(), nb = bdet.size();
+  if (W.size() != na * nb) throw std::invalid_argument("carryover: W size");
+  Carryover r;
+  if (type == 1 || type == 2) {
+    std::vector<double> Da(na, 0.0), Db(nb, 0.0);
+    for (size_t ia = 0; ia < na; ia++)
+      for (size_t ib = 0; ib < nb; ib++) {
+        const double w2 = W[ia * nb + ib] * W[ia * nb + ib];
+        Da[ia] += w2;
+      }
+    for (size_t ib = 0; ib < nb; ib++)
+      for (size_t ia = 0; ia < na; ia++) Db[ib] += W[ia * nb + ib] * W[ia * nb + ib];
+    if (ratio == 0.0) {
+      r.adet = cd::threshold_select(adet, Da, threshold);
+      r.bdet = cd::threshold_select(bdet, Db, threshold);
+    } else {
+      r.adet = cd::ratio_select(adet, Da, ratio, r.truncated_alpha);
+      r.bdet = cd::ratio_select(bdet, Db, ratio, r.truncated_beta);
+    }
+    if (type == 2) {
+      r.adet = cd::singles_extend(r.adet, norb);
+      r.bdet = cd::singles_extend(r.bdet, norb);
+    }
+  } else if (type == 3) {
+    std::vector<char> ua(na, 0), ub(nb, 0);
+    for (size_t ia = 0; ia < na; ia++)
+      for (size_t ib = 0; ib < nb; ib++) {
+        const double w2 = W[ia * nb + ib] * W[ia * nb + ib];
+        if (w2 > threshold) {
+          r.total_weight += w2;
+          ua[ia] = ub[ib] = 1;
+        }
+      }
+    std::vector<uint64_t> sa, sb;
+    for (size_t ia = 0; ia < na; ia++)
+      if (ua[ia]) sa.push_back(adet[ia]);
+    for (size_t ib = 0; ib < nb; ib++)
+      if (ub[ib]) sb.push_back(bdet[ib]);
+    r.adet = cd::singles_extend(sa, norb);
+    r.bdet = cd::singles_extend(sb, norb);
+  }
+  return r;
 }
 
 }  // namespace sbdx

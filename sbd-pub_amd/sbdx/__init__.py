@@ -528,7 +528,8 @@ def davidson(op: TPBOperator, W=None, max_it=100, nb=12, eps=1e-8, max_time=1e30
 
 
 def diag(fcidump_path, adet_path, bdet_path=None, max_it=100, nb=12, eps=1e-8, do_rdm=True,
-         bit_length=20, rdm_dir=None, loadname="", savename=""):
+         bit_length=20, rdm_dir=None, loadname="", savename="", carryover_type=0,
+         carryover_threshold=0.0, carryover_ratio=0.0):
     """FCIDUMP + string files -> energy <W|HW>, density and RDMs, like
     sbd::tpb::diag (chemistry/tpb/sbdiag.h:139-763) on one GPU.  With
     rdm_dir, also the driver's 1pRDM.txt / 2pRDM.txt there and its zero-,
@@ -536,7 +537,10 @@ def diag(fcidump_path, adet_path, bdet_path=None, max_it=100, nb=12, eps=1e-8, d
     (apps/chemistry_tpb_selected_basis_diagonalization/main.cc:264-318).
     loadname / savename: start from a saved wavefunction (its coefficients
     on the determinants both sets share) instead of the HF vector, and save
-    the result, as sbdiag.h:285-291 and :742-746 do."""
+    the result, as sbdiag.h:285-291 and :742-746 do.  carryover_type 1-3
+    with carryover_threshold / carryover_ratio: the carry-over strings of
+    sbdiag.h:692-735 (sbdx.carryover) in out["co_adet"], out["co_bdet"] and
+    their weights in out["carryover"]."""
     fd = io.read_fcidump(fcidump_path)
     I0, I1, I2 = io.setup_integrals(fd)
     L = fd["norb"]
@@ -555,6 +559,10 @@ def diag(fcidump_path, adet_path, bdet_path=None, max_it=100, nb=12, eps=1e-8, d
         io.save_wavefunction(savename, L, a, b, W, bit_length)
     E = op.energy(W)
     out = dict(energy=E, W=W, steps=steps, converged=conv, norb=L)
+    if carryover_type:
+        from . import carryover as co
+        out["co_adet"], out["co_bdet"], out["carryover"] = co.carryover(
+            W, a, b, L, carryover_type, threshold=carryover_threshold, ratio=carryover_ratio)
     if do_rdm:
         one, two = op.correlation(W)
         out["one"], out["two"] = one, two
