@@ -995,16 +995,13 @@ int davidson_block(Context &c, int nroots, int block, int max_it, double eps, do
       // the basis products
       double emax = 0.0;
       for (int j = 0; j < nroots; j++) emax = std::max(emax, std::fabs(hE[j]));
-      // (on a normal restart only the roots not yet below eps: a converged
-      // root's column barely changes and the confirming restart recomputes
-      // it anyway)
-      const bool confirm = nk <= r64;
-      if (mixed && (confirm || rmax < std::max(100.0 * eps, kFp32Level * emax))) {
-        for (int j = 0; j < std::min(nk, r64); j++)
-          if (confirm || resid_of(K[j]) >= eps) {
-            product(j, j + 1);
-            products++;
-          }
+      // (every root, converged or not: a locked root's combined product
+      // would otherwise collect the FP32 columns' errors restart after
+      // restart)
+      if (mixed && (nk <= r64 || rmax < std::max(100.0 * eps, kFp32Level * emax))) {
+        const int ne = std::min(nk, r64);
+        product(0, ne);
+        products += ne;
       }
       d = nk;
       c.blk_restarts++;
