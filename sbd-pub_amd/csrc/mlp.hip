@@ -5,7 +5,8 @@
 //   (GELU in tanh form, z * sigmoid(2u), u = sqrt(2/pi)(z + 0.044715 z^3))
 //                      -> Linear(128,1) = c_I
 // Weights are kept in FP32 (master copy, Adam state) and used in BF16; all
-// products run on MFMA (v_mfma_f32_32x32x8_bf16) with FP32 accumulation.
+// products run on MFMA (training: v_mfma_f32_32x32x8_bf16; the solver's
+// inference kernel: gfx950's v_mfma_f32_32x32x16_bf16) with FP32 accumulation.
 // Activations entering a product are rounded to BF16 at fixed points (X, H1,
 // H2, dZ1, dZ2, and Z1/Z2 where GELU' is evaluated), which the CPU gradient
 // check in the tests reproduces.
@@ -25,6 +26,8 @@
 //   D (32x32): lane l, register r holds D[8*(r/4) + 4*(l/32) + r%4][l%32]
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 
 #include "common.h"
 
@@ -51,6 +54,15 @@ __device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
 __device__ __forceinline__ float gelu_d(float z) {
   const float s = gelu_sig(z);
   return s + z * s * (1.0f - s) * 2.0f * kGeluC * (1.0f + 3.0f * 0.044715f * z * z);
+}
+// The same GELU for the inference kernel, its constants folded:
+// sigmoid(2u) = 1 / (1 + 2^t), t = z (k1 + k2 z^2), k1 = -2 sqrt(2/pi) log2(e),
+// k2 = 0.044715 k1; native exp2 and reciprocal (v_exp_f32, v_rcp_f32).
+constexpr float kGeluT1 = -2.0f * 0.79788456080286536f * 1.4426950408889634f;
+constexpr float kGeluT2 = 0.044715f * kGeluT1;
+__device__ __forceinline__ float gelu_fast(float z) {
+  const float t = z * __builtin_fmaf(kGeluT2, z * z, kGeluT1);
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
 }
 __device__ __forceinline__ int d_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
@@ -171,44 +183,62 @@ __global__ void __launch_bounds__(64) k_mlp_forward(size_t rows, const unsigned 
   if (lane < 32 && r0 + lane < rows) out[r0 + lane] = y;
 }
 
-// Preconditioner: c = MLP(features(I)) for every determinant of the CI
-// vector (rows = n_alpha*n_beta), features built on the fly.  Four waves per
-// block share the LDS weights; each wave keeps only its X and H1 tiles and
-// reduces the last layer from the MFMA accumulators.
 // Learned preconditioner over all N determinants: C[I] = MLP(features of I).
-// Computed transposed, H^T = W^T X^T, so that the weights are the A operand
-// (contiguous 8-byte LDS reads from [hidden][k] copies) and the features the
-// B operand, built in registers from the determinant's strings: lane l
-// handles determinant l%32 of a 32-determinant tile and features
-// ks + 4*(l/32) + j; features 136..159 are zero and their k-steps are
-// skipped.  Four independent accumulators (32 hidden units each) keep the
-// matrix cores busy.  H1 goes through a per-wave LDS tile [32][kH1s] so
-// that layer 2 reads its B operand (4 consecutive hidden units of one
-// determinant) as one 8-byte load; layer 3 reduces each determinant's
-// hidden units over the lanes that hold them and the two half-waves.
-// BF16 rounding points as in forward_tile (X, Z1, H1, Z2, H2, w3).
-constexpr int kK1 = 136;           // used layer-1 inputs (134 features, padded to a k-step)
-constexpr int kH1s = kHid + 8;     // H1 tile row stride (BF16), spreads LDS banks
-constexpr int kPrecondWaves = 8;   // waves per block
-__global__ void __launch_bounds__(64 * kPrecondWaves) k_mlp_precond(size_t N, int nb, const uint64_t *__restrict__ astr,
+// Computed transposed, H^T = W^T X^T, with the gfx950 MFMA
+// v_mfma_f32_32x32x16_bf16 (K = 16 per instruction: lane l holds A[l%32]
+// [8*(l/32) + j] and B[8*(l/32) + j][l%32], j < 8; D as for 32x32x8).  The
+// weights are the A operand, read from LDS as one 16-byte load per lane; the
+// features the B operand, built in registers from the determinant's strings
+// (lane l handles determinant l%32 of a 32-determinant tile).  Layer 1 runs
+// 9 k-steps (features 134..143 are zero), 4 accumulators of 32 hidden units.
+// The activations never leave the registers: after bias, BF16 rounding,
+// GELU and rounding, lane half h of accumulator m holds, in registers
+// 8s..8s+7, the hidden units 32m + 16s + 8(j/4) + 4h + j%4 -- which is the B
+// operand of one layer-2 k-step when layer 2's k index is permuted the same
+// way, so the LDS copy of W2 stores its columns in that order (W2p).  Row
+// strides of 19 and 17 16-byte elements (odd) make the A reads free of LDS
+// bank conflicts: the 16 lanes of one 16-byte-read lane group
+// (2*bit5 + (bit4^bit3^bit2)) have distinct (l%32) mod 16, hence distinct
+// element residues.  The weights take 75 KB, so two 8-wave blocks share a
+// CU.  Layer 3 reduces each determinant's 128 hidden units over the two
+// half-waves.  BF16 rounding points as in forward_tile (X, Z1, H1, Z2, H2,
+// w3); accumulation order differs (K = 16 per instruction, the layer-3 sum
+// in two halves).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+constexpr int kS1 = 152;           // W1t row stride (BF16): 19 16-byte elements
+constexpr int kS2 = 136;           // W2p row stride: 17 16-byte elements
+constexpr size_t kPrecondLds = sizeof(unsigned short) * (kHid * kS1 + kHid * kS2) + sizeof(float) * 3 * kHid;
+
+// the hidden unit that layer-2 k position p (16 kk + 8 h + j) carries:
+// kk = 2m + s -> 32m + 16s + 8(j/4) + 4h + j%4
+__device__ __forceinline__ int w2_unit(int p) {
+  const int kk = p >> 4, h = (p >> 3) & 1, j = p & 7;
+  return 32 * (kk >> 1) + 16 * (kk & 1) + 8 * (j >> 2) + 4 * h + (j & 3);
+}
+
+// WAVES waves per block: 8 (two blocks per CU, registers capped for 4 waves
+// per SIMD) or 12 (one block per CU, 3 waves per SIMD, no cap)
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES == 8 ? 4 : 3)))
+k_mlp_precond(size_t N, int nb, const uint64_t *__restrict__ astr,
                                                      const uint64_t *__restrict__ bstr,
                                                      const double *__restrict__ R,
                                                      const double *__restrict__ hii,
                                                      const double *__restrict__ E,
                                                      const float *__restrict__ prm, double *__restrict__ C) {
   extern __shared__ __align__(16) unsigned char sm_raw[];
-  unsigned short *W1t = (unsigned short *)sm_raw;  // [kHid][kK1]
-  unsigned short *W2t = W1t + kHid * kK1;          // [kHid][kHid]
-  float *bias1 = (float *)(W2t + kHid * kHid);     // [kHid]
+  unsigned short *W1t = (unsigned short *)sm_raw;  // [kHid][kS1], W1t[h][k] = W1[k][h], zero for k >= 134
+  unsigned short *W2p = W1t + kHid * kS1;          // [kHid][kS2], W2p[h][p] = W2[w2_unit(p)][h]
+  float *bias1 = (float *)(W2p + kHid * kS2);      // [kHid]
   float *bias2 = bias1 + kHid, *w3 = bias2 + kHid;
-  unsigned short *H1all = (unsigned short *)(w3 + kHid);
-  for (int i = threadIdx.x; i < kK1 * kHid; i += blockDim.x) {
-    const int k = i / kHid, h = i % kHid;  // prm W1 is [kDin][kHid]
-    W1t[h * kK1 + k] = f2bf(prm[kW1 + i]);
+  for (int i = threadIdx.x; i < kHid * kS1; i += blockDim.x) {
+    const int h = i / kS1, k = i % kS1;  // prm W1 is [kDin][kHid]
+    W1t[i] = k < 134 ? f2bf(prm[kW1 + k * kHid + h]) : (unsigned short)0;
   }
-  for (int i = threadIdx.x; i < kHid * kHid; i += blockDim.x) {
-    const int k = i / kHid, h = i % kHid;
-    W2t[h * kHid + k] = f2bf(prm[kW2 + i]);
+  for (int i = threadIdx.x; i < kHid * kS2; i += blockDim.x) {
+    const int h = i / kS2, p = i % kS2;  // prm W2 is [kHid (in)][kHid (out)]
+    W2p[i] = p < kHid ? f2bf(prm[kW2 + w2_unit(p) * kHid + h]) : (unsigned short)0;
   }
   for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
     bias1[i] = prm[kB1 + i];
@@ -218,115 +248,96 @@ __global__ void __launch_bounds__(64 * kPrecondWaves) k_mlp_precond(size_t N, in
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 31, half = lane >> 5;
-  unsigned short *H1 = H1all + wave * 32 * kH1s;
   const double e = E[0];
   const double sq = sqrt((double)N);
   const uint64_t a0 = astr[0], b0 = bstr[0];
   const float b3 = prm[kB3];
   const size_t ntiles = (N + 31) / 32;
-  for (size_t tile = (size_t)blockIdx.x * kPrecondWaves + wave; tile < ntiles;
-       tile += (size_t)gridDim.x * kPrecondWaves) {
+  for (size_t tile = (size_t)blockIdx.x * WAVES + wave; tile < ntiles; tile += (size_t)gridDim.x * WAVES) {
     const size_t I = tile * 32 + col;
     const bool live = I < N;
     const size_t Ic = live ? I : N - 1;
     const size_t ia = Ic / nb, ib = Ic % nb;
     const uint64_t sa = astr[ia], sb = bstr[ib];
-    // scalar features 128..133 exactly as features() writes them
+    // features 128..135 (half 0) and 136..143 (half 1, zero), as features()
     const double r_scaled = R[Ic] * sq, hmE = hii[Ic] - e;
     double den = -hmE;  // E - h_II
     if (fabs(den) < 1e-3) den = den < 0 ? -1e-3 : 1e-3;
     const double dstep = fmax(-50.0, fmin(50.0, r_scaled / den));
-    bf16x4 bvs;
+    s16x8 sv = {0, 0, 0, 0, 0, 0, 0, 0};
     if (half == 0) {
-      bvs[0] = (short)f2bf((float)r_scaled);
-      bvs[1] = (short)f2bf((float)(hmE / 8.0));
-      bvs[2] = (short)f2bf((float)(dstep / 10.0));
-      bvs[3] = (short)f2bf((float)(__popcll(sa ^ a0) / 8.0));
-    } else {
-      bvs[0] = (short)f2bf((float)(__popcll(sb ^ b0) / 8.0));
-      bvs[1] = (short)f2bf(1.0f);
-      bvs[2] = 0;
-      bvs[3] = 0;
+      sv[0] = (short)f2bf((float)r_scaled);
+      sv[1] = (short)f2bf((float)(hmE / 8.0));
+      sv[2] = (short)f2bf((float)(dstep / 10.0));
+      sv[3] = (short)f2bf((float)(__popcll(sa ^ a0) / 8.0));
+      sv[4] = (short)f2bf((float)(__popcll(sb ^ b0) / 8.0));
+      sv[5] = (short)f2bf(1.0f);
     }
-    // layer 1: acc[m] = rows 32m..32m+31 of W1^T X^T
+    // ---- layer 1: acc[m] = rows 32m..32m+31 of W1^T X^T
     f32x16 acc[4];
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
-#pragma unroll 2
-    for (int ks = 0; ks < 128; ks += 8) {
-      const uint64_t bits = ks < 64 ? sa : sb;
-      const int sh = (ks & 63) + 4 * half;
-      bf16x4 bv;
+#pragma unroll 1
+    for (int ks = 0; ks < 9; ks++) {
+      s16x8 bs;
+      if (ks < 8) {
+        const unsigned byte = (unsigned)(((ks < 4 ? sa : sb) >> (16 * (ks & 3) + 8 * half)) & 0xffull);
 #pragma unroll
-      for (int j = 0; j < 4; j++) bv[j] = ((bits >> (sh + j)) & 1ull) ? (short)0x3f80 : (short)0;
+        for (int j = 0; j < 8; j++) bs[j] = ((byte >> j) & 1u) ? (short)0x3f80 : (short)0;
+      } else {
+        bs = sv;
+      }
+      const bf16x8 bv = __builtin_bit_cast(bf16x8, bs);
 #pragma unroll
       for (int m = 0; m < 4; m++)
-        acc[m] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
-            *reinterpret_cast<const bf16x4 *>(W1t + (32 * m + col) * kK1 + ks + 4 * half), bv, acc[m], 0, 0, 0);
+        acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+            *reinterpret_cast<const bf16x8 *>(W1t + (32 * m + col) * kS1 + 16 * ks + 8 * half), bv, acc[m], 0, 0, 0);
     }
-#pragma unroll
-    for (int m = 0; m < 4; m++)
-      acc[m] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
-          *reinterpret_cast<const bf16x4 *>(W1t + (32 * m + col) * kK1 + 128 + 4 * half), bvs, acc[m], 0, 0, 0);
-    // Z1 + b1 in BF16 into the wave's tile, [determinant][hidden]; lane l
-    // holds hidden units 8t + 4*(l/32) + j (t < 16) of determinant l%32
+    // ---- H1 = BF16(GELU(BF16(Z1 + b1))) as layer 2's B operands, in registers
+    bf16x8 h1[4][2];
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int h0 = 32 * m + 8 * q + 4 * half;
-        bf16x4 v;
+      for (int s2 = 0; s2 < 2; s2++) {
+        s16x8 v;
 #pragma unroll
-        for (int j = 0; j < 4; j++) v[j] = (short)f2bf(acc[m][4 * q + j] + bias1[h0 + j]);
-        *reinterpret_cast<bf16x4 *>(H1 + col * kH1s + h0) = v;
+        for (int j = 0; j < 8; j++) {
+          const int unit = 32 * m + 16 * s2 + 8 * (j >> 2) + 4 * half + (j & 3);
+          const float z = bf2f(f2bf(acc[m][8 * s2 + j] + bias1[unit]));
+          v[j] = (short)f2bf(gelu_fast(z));
+        }
+        h1[m][s2] = __builtin_bit_cast(bf16x8, v);
       }
-    // H1 = GELU(Z1) in place, one 4-unit group at a time (rolled: few registers)
-    for (int t = 0; t < 16; t++) {
-      bf16x4 *p4 = reinterpret_cast<bf16x4 *>(H1 + col * kH1s + 8 * t + 4 * half);
-      bf16x4 v = *p4;
-#pragma unroll
-      for (int j = 0; j < 4; j++) v[j] = (short)f2bf(gelu(bf2f((unsigned short)v[j])));
-      *p4 = v;
-    }
-    __builtin_amdgcn_wave_barrier();
-    // layer 2: acc[m] = rows 32m..32m+31 of W2^T H1^T
+    // ---- layer 2: acc[m'] = rows 32m'..32m'+31 of W2^T H1^T
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
-#pragma unroll 2
-    for (int ks = 0; ks < kHid; ks += 8) {
-      const bf16x4 bv = *reinterpret_cast<const bf16x4 *>(H1 + col * kH1s + ks + 4 * half);
+#pragma unroll
+    for (int kk = 0; kk < 8; kk++) {
 #pragma unroll
       for (int m = 0; m < 4; m++)
-        acc[m] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
-            *reinterpret_cast<const bf16x4 *>(W2t + (32 * m + col) * kHid + ks + 4 * half), bv, acc[m], 0, 0, 0);
+        acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+            *reinterpret_cast<const bf16x8 *>(W2p + (32 * m + col) * kS2 + 16 * kk + 8 * half), h1[kk >> 1][kk & 1],
+            acc[m], 0, 0, 0);
+      // keep each k-step's four A loads next to their MFMAs (hoisting them
+      // all ahead costs more registers than the chip has for this occupancy)
+      __builtin_amdgcn_sched_barrier(0);
     }
-    __builtin_amdgcn_wave_barrier();
-    // Z2 + b2 in BF16 into the same tile (layer 2 has read H1)
+    // ---- layer 3: y = sum_h w3[h] BF16(GELU(BF16(Z2[h] + b2[h]))) + b3 over both half-waves
+    float ysum = 0.f;
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int h0 = 32 * m + 8 * q + 4 * half;
-        bf16x4 v;
-#pragma unroll
-        for (int j = 0; j < 4; j++) v[j] = (short)f2bf(acc[m][4 * q + j] + bias2[h0 + j]);
-        *reinterpret_cast<bf16x4 *>(H1 + col * kH1s + h0) = v;
+      for (int r = 0; r < 16; r++) {
+        const int unit = 32 * m + 8 * (r >> 2) + 4 * half + (r & 3);
+        const float z = bf2f(f2bf(acc[m][r] + bias2[unit]));
+        ysum += bf2f(f2bf(gelu_fast(z))) * w3[unit];
       }
-    // layer 3: y = sum_h w3[h] GELU(Z2[h]) + b3 over both half-waves
-    float s = 0.f;
-    for (int t = 0; t < 16; t++) {
-      const int h0 = 8 * t + 4 * half;
-      const bf16x4 v = *reinterpret_cast<const bf16x4 *>(H1 + col * kH1s + h0);
-#pragma unroll
-      for (int j = 0; j < 4; j++) s += bf2f(f2bf(gelu(bf2f((unsigned short)v[j])))) * w3[h0 + j];
-    }
-    __builtin_amdgcn_wave_barrier();
-    s += __shfl_xor(s, 32, 64);
-    if (half == 0 && live) C[I] = (double)(s + b3);
+    ysum += __shfl_xor(ysum, 32, 64);
+    if (half == 0 && live) C[I] = (double)(ysum + b3);
   }
 }
 
@@ -553,15 +564,27 @@ void mlp_adam_step(Context &c, float lr) {
   check_launch("k_adam");
 }
 
-void mlp_precond(Context &c, const double *R, const double *E, double *Cout) {
-  const size_t lds = sizeof(unsigned short) * ((size_t)kK1 * kHid + kHid * kHid) + sizeof(float) * 3 * kHid +
-                     sizeof(unsigned short) * kPrecondWaves * 32 * kH1s;
-  SBDX_HIP(hipFuncSetAttribute((const void *)k_mlp_precond, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const unsigned g = (unsigned)std::min<size_t>((c.N + 32 * kPrecondWaves - 1) / (32 * kPrecondWaves),
-                                                (size_t)c.n_cu * 2);
-  hipLaunchKernelGGL(k_mlp_precond, dim3(g), dim3(64 * kPrecondWaves), lds, c.stream, c.N, (int)c.nb, c.A.str, c.B.str,
-                     R, c.hii, E, c.mlp.prm, Cout);
+template <int WAVES>
+static void launch_precond(Context &c, const double *R, const double *E, double *Cout) {
+  auto fn = k_mlp_precond<WAVES>;
+  SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPrecondLds));
+  // every block loops over its tiles; 4 rounds of resident blocks
+  const unsigned g = (unsigned)std::min<size_t>((c.N + 32 * WAVES - 1) / (32 * WAVES), (size_t)c.n_cu * 4);
+  hipLaunchKernelGGL(fn, dim3(g), dim3(64 * WAVES), kPrecondLds, c.stream, c.N, (int)c.nb, c.A.str, c.B.str, R,
+                     c.hii, E, c.mlp.prm, Cout);
   check_launch("k_mlp_precond");
+}
+
+// SBDX_PRECOND_WAVES=8|12 picks the block shape (default 8)
+void mlp_precond(Context &c, const double *R, const double *E, double *Cout) {
+  static const int waves = [] {
+    const char *e = std::getenv("SBDX_PRECOND_WAVES");
+    return e && std::string(e) == "12" ? 12 : 8;
+  }();
+  if (waves == 12)
+    launch_precond<12>(c, R, E, Cout);
+  else
+    launch_precond<8>(c, R, E, Cout);
 }
 
 // Features of every determinant for the residual R and Ritz value E (BF16,
