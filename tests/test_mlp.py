@@ -154,8 +154,8 @@ def test_shipped_network_gain_on_held_out_instance():
     """What the trained network (the shipped weights) gains over the diagonal
     preconditioner on the held-out norb-22 instance (not among the training
     instances): both solves reach the same energy; the products and times of
-    both are printed, and the network must not cost more than 10% extra
-    products (measured gain so far: about 1.03x fewer products)."""
+    both are printed, and the network must take no more products than the
+    diagonal step (measured: 128 against 132)."""
     import os
     import sbdx
     from sbdx import learned
@@ -166,4 +166,4 @@ def test_shipped_network_gain_on_held_out_instance():
     assert out["diag"]["converged"] and out["mlp"]["converged"]
     assert out["dE"] < 1e-8
     assert out["mlp"]["steps_by_precond"]["mlp"] > 0
-    assert out["mlp"]["products"] <= 1.1 * out["diag"]["products"]
+    assert out["mlp"]["products"] <= out["diag"]["products"]
