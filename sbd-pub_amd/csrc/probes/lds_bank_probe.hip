@@ -40,6 +40,25 @@ __global__ void __launch_bounds__(256) k_lds_probe(const unsigned *__restrict__ 
   out[blockIdx.x * 256 + threadIdx.x] = acc;
 }
 
+// The same for 4-byte reads (ds_read_b32): every lane reads one float at a
+// fixed per-lane element index (below 16384, 64 KB of LDS).
+__global__ void __launch_bounds__(256) k_lds_probe32(const unsigned *__restrict__ pat, int iters, float *out) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  typedef __attribute__((address_space(3))) const float lds_f;
+  for (int i = threadIdx.x; i < 16384; i += 256) reinterpret_cast<float *>(lds)[i] = (float)i;
+  __syncthreads();
+  unsigned a[8];
+  for (int k = 0; k < 8; k++) a[k] = (pat[threadIdx.x & 63] & 16383u) * 4u;
+  float acc = 0.f;
+  for (int it = 0; it < iters; it += 8) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) asm volatile("" : "+v"(a[k]));
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc += *reinterpret_cast<lds_f *>((size_t)a[k]);
+  }
+  out[blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
 int main() {
   std::vector<std::pair<std::string, std::vector<unsigned>>> pats;
   std::vector<unsigned> e(64);
@@ -84,6 +103,32 @@ int main() {
     // wave instructions per CU: blocks * 4 waves * iters / 256 CUs, clock 2.4 GHz
     const double cyc = ms * 1e-3 * 2.4e9 / ((double)blocks * 4 * iters / 256);
     printf("%-28s %6.2f CU-cycles per 16-byte wave read\n", pats[p].first.c_str(), cyc);
+  }
+  // 4-byte reads: which lanes share a conflict domain, and the modulus
+  std::vector<std::pair<std::string, std::vector<unsigned>>> p32;
+  for (int l = 0; l < 64; l++) e[l] = l;
+  p32.push_back({"b32 consecutive", e});
+  for (int l = 0; l < 64; l++) e[l] = 64 * l;
+  p32.push_back({"b32 one bank (x64)", e});
+  for (int l = 0; l < 64; l++) e[l] = (l % 32) + 64 * (l / 32);  // lane l, l+32 same bank, other address
+  p32.push_back({"b32 halves share banks", e});
+  for (int l = 0; l < 64; l++) e[l] = (l % 16) + 32 * ((l / 16) % 2) + 128 * (l / 32);  // l, l+16: equal mod 32
+  p32.push_back({"b32 equal mod 32 in a half", e});
+  for (int l = 0; l < 64; l++) e[l] = (l % 16) + 64 * ((l / 16) % 2) + 128 * (l / 32);  // l, l+16: equal mod 64
+  p32.push_back({"b32 equal mod 64 in a half", e});
+  for (int l = 0; l < 64; l++) e[l] = l + 64 * (l % 2);  // distinct banks mod 64, l and l+32 equal mod 32
+  p32.push_back({"b32 distinct banks, spread", e});
+  for (size_t p = 0; p < p32.size(); p++) {
+    if (hipMemcpy(dp, p32[p].second.data(), sizeof(unsigned) * 64, hipMemcpyHostToDevice) != hipSuccess) return 1;
+    hipLaunchKernelGGL(k_lds_probe32, dim3(blocks), dim3(256), 65536, 0, dp, iters, dout);
+    (void)hipEventRecord(t0, 0);
+    hipLaunchKernelGGL(k_lds_probe32, dim3(blocks), dim3(256), 65536, 0, dp, iters, dout);
+    (void)hipEventRecord(t1, 0);
+    if (hipEventSynchronize(t1) != hipSuccess) return 1;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    const double cyc = ms * 1e-3 * 2.4e9 / ((double)blocks * 4 * iters / 256);
+    printf("%-28s %6.2f CU-cycles per 4-byte wave read\n", p32[p].first.c_str(), cyc);
   }
   return 0;
 }

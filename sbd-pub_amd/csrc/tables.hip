@@ -361,43 +361,49 @@ static void ell_layout(Context &c, const SpinTables &t, const std::vector<uint64
 // times in the group, trying (1,1), then (1,2) and (2,1), and so on; an
 // unbounded greedy choice ends it.  On the norb-36 workload this takes the
 // modelled cost of one read from 8.9 cycles (dealing that ignores the lane
-// grouping) to 5.2 (4 without conflicts).
+// grouping) to 5.2 (4 without conflicts).  4- and 8-byte reads follow
+// another rule (schedule_slice_packed), dealt by the same search.
 namespace qsched {
+// NL lanes per group, NR residues (a power of two, <= 32; NL == NR here)
+template <int NR>
 struct Lane {
   std::vector<uint32_t> ent;
-  uint16_t cnt[256];  // entries per (x, w) residue pair
-  uint16_t xw[16];    // bit w of xw[x]: some entry has residues (x, w)
+  uint16_t cnt[NR * NR];  // entries per (x, w) residue pair
+  uint32_t xw[NR];        // bit w of xw[x]: some entry has residues (x, w)
 };
-static inline int rx(uint32_t w) { return (w >> 4) & 15; }
-static inline int rw(uint32_t w) { return (w >> 19) & 15; }
+template <int NR>
+static inline int rx(uint32_t w) { return (w >> 4) & (NR - 1); }
+template <int NR>
+static inline int rw(uint32_t w) { return (w >> 19) & (NR - 1); }
+template <int NL, int NR>
 struct Search {
-  Lane *L;
-  int remx[16], remw[16];
-  uint8_t ux[16], uw[16];
+  Lane<NR> *L;
+  int remx[NR], remw[NR];
+  uint8_t ux[NR], uw[NR];
   int cx, cw;
   int padx, padw;
   bool padused;
   long nodes, limit;
-  int pick[16];  // pair index x*16+w, or -1 idle
-  bool need[16], done[16];
+  int pick[NL];  // pair index x*NR+w, or -1 idle
+  bool need[NL], done[NL];
   bool fits(int x, int w) const { return ux[x] < cx && uw[w] < cw; }
   int options(int l, uint32_t wfree) const {
     int n = 0;
-    for (int x = 0; x < 16; x++)
+    for (int x = 0; x < NR; x++)
       if (ux[x] < cx) n += __builtin_popcount(L[l].xw[x] & wfree);
     return n;
   }
   uint32_t wfree_mask() const {
     uint32_t m = 0;
-    for (int w = 0; w < 16; w++) if (uw[w] < cw) m |= 1u << w;
+    for (int w = 0; w < NR; w++) if (uw[w] < cw) m |= 1u << w;
     return m;
   }
   bool dfs(int depth) {
-    if (depth == 16) return true;
+    if (depth == NL) return true;
     if (++nodes > limit) return false;
     int bl = -1, bn = 1 << 30;
     const uint32_t wf = wfree_mask();
-    for (int l = 0; l < 16; l++) {
+    for (int l = 0; l < NL; l++) {
       if (done[l]) continue;
       int n = options(l, wf) + (need[l] ? 0 : 1);
       if (n < bn) { bn = n; bl = l; }
@@ -405,20 +411,20 @@ struct Search {
     if (bn == 0) return false;
     const int l = bl;
     done[l] = true;
-    std::array<std::pair<int, int>, 256> cand;
+    std::array<std::pair<int, int>, NR * NR> cand;
     int nc = 0;
-    for (int x = 0; x < 16; x++) {
+    for (int x = 0; x < NR; x++) {
       if (ux[x] >= cx) continue;
       uint32_t m = L[l].xw[x] & wf;
       while (m) {
         const int w = __builtin_ctz(m);
         m &= m - 1;
-        cand[nc++] = {-(remx[x] + remw[w]), x * 16 + w};
+        cand[nc++] = {-(remx[x] + remw[w]), x * NR + w};
       }
     }
     std::sort(cand.begin(), cand.begin() + nc);
     for (int i = 0; i < nc; i++) {
-      const int p = cand[i].second, x = p >> 4, w = p & 15;
+      const int p = cand[i].second, x = p / NR, w = p % NR;
       ux[x]++; uw[w]++;
       pick[l] = p;
       if (dfs(depth + 1)) return true;
@@ -438,39 +444,40 @@ struct Search {
     return false;
   }
 };
-// rows[lanes[i]]: the 16 lanes' entries (consumed); out[64*k + lanes[i]]
-static void deal_quarter(std::vector<uint32_t> *rows, const int *lanes, uint32_t len, uint32_t pad, uint32_t *out,
-                         long limit) {
-  Lane L[16];
-  Search s;
-  s.L = L;
-  std::fill(s.remx, s.remx + 16, 0);
-  std::fill(s.remw, s.remw + 16, 0);
-  for (int l = 0; l < 16; l++) {
+// rows[lanes[i]]: the NL lanes' entries (consumed); out[64*k + lanes[i]]
+template <int NL, int NR>
+static void deal_group(std::vector<uint32_t> *rows, const int *lanes, uint32_t len, uint32_t pad, uint32_t *out,
+                       long limit) {
+  std::vector<Lane<NR>> L(NL);
+  Search<NL, NR> s;
+  s.L = L.data();
+  std::fill(s.remx, s.remx + NR, 0);
+  std::fill(s.remw, s.remw + NR, 0);
+  for (int l = 0; l < NL; l++) {
     L[l].ent = rows[lanes[l]];
-    std::fill(L[l].cnt, L[l].cnt + 256, 0);
-    std::fill(L[l].xw, L[l].xw + 16, 0);
+    std::fill(L[l].cnt, L[l].cnt + NR * NR, 0);
+    std::fill(L[l].xw, L[l].xw + NR, 0);
     for (uint32_t w : L[l].ent) {
-      L[l].cnt[rx(w) * 16 + rw(w)]++;
-      L[l].xw[rx(w)] |= (uint16_t)(1u << rw(w));
-      s.remx[rx(w)]++;
-      s.remw[rw(w)]++;
+      L[l].cnt[rx<NR>(w) * NR + rw<NR>(w)]++;
+      L[l].xw[rx<NR>(w)] |= 1u << rw<NR>(w);
+      s.remx[rx<NR>(w)]++;
+      s.remw[rw<NR>(w)]++;
     }
   }
-  s.padx = rx(pad);
-  s.padw = rw(pad);
+  s.padx = rx<NR>(pad);
+  s.padw = rw<NR>(pad);
   static const int caps[][2] = {{1, 1}, {1, 2}, {2, 1}, {2, 2}, {2, 3}, {3, 2}, {3, 3}};
   const int ncaps = sizeof(caps) / sizeof(caps[0]);
   for (uint32_t k = 0; k < len; k++) {
     const uint32_t left = len - k;
-    for (int l = 0; l < 16; l++) s.need[l] = L[l].ent.size() >= left;
+    for (int l = 0; l < NL; l++) s.need[l] = L[l].ent.size() >= left;
     bool ok = false;
     for (int c = 0; c < ncaps && !ok; c++) {
       s.cx = caps[c][0];
       s.cw = caps[c][1];
-      std::fill(s.ux, s.ux + 16, 0);
-      std::fill(s.uw, s.uw + 16, 0);
-      std::fill(s.done, s.done + 16, false);
+      std::fill(s.ux, s.ux + NR, 0);
+      std::fill(s.uw, s.uw + NR, 0);
+      std::fill(s.done, s.done + NL, false);
       s.padused = false;
       s.nodes = 0;
       s.limit = limit;
@@ -478,10 +485,10 @@ static void deal_quarter(std::vector<uint32_t> *rows, const int *lanes, uint32_t
     }
     if (!ok) {
       // unbounded: lane by lane, the pair minimizing the larger use count
-      std::fill(s.ux, s.ux + 16, 0);
-      std::fill(s.uw, s.uw + 16, 0);
+      std::fill(s.ux, s.ux + NR, 0);
+      std::fill(s.uw, s.uw + NR, 0);
       bool pu = false;
-      for (int l = 0; l < 16; l++) {
+      for (int l = 0; l < NL; l++) {
         if (L[l].ent.empty()) {
           s.pick[l] = -1;
           if (!pu) { s.ux[s.padx]++; s.uw[s.padw]++; pu = true; }
@@ -489,28 +496,28 @@ static void deal_quarter(std::vector<uint32_t> *rows, const int *lanes, uint32_t
         }
         long best = 1L << 60;
         int bp = 0;
-        for (int p = 0; p < 256; p++) {
+        for (int p = 0; p < NR * NR; p++) {
           if (!L[l].cnt[p]) continue;
-          const int x = p >> 4, w = p & 15;
+          const int x = p / NR, w = p % NR;
           const long sc = (long)std::max(s.ux[x], s.uw[w]) * 1000000 + (long)(s.ux[x] + s.uw[w]) * 1000 -
                           (s.remx[x] + s.remw[w]);
           if (sc < best) { best = sc; bp = p; }
         }
         s.pick[l] = bp;
-        s.ux[bp >> 4]++;
-        s.uw[bp & 15]++;
+        s.ux[bp / NR]++;
+        s.uw[bp % NR]++;
       }
     }
-    for (int l = 0; l < 16; l++) {
+    for (int l = 0; l < NL; l++) {
       uint32_t w = pad;
       if (s.pick[l] >= 0 && !L[l].ent.empty()) {
         const int p = s.pick[l];
         auto &E = L[l].ent;
         for (size_t i = 0; i < E.size(); i++)
-          if (rx(E[i]) * 16 + rw(E[i]) == p) { w = E[i]; E[i] = E.back(); E.pop_back(); break; }
-        if (--L[l].cnt[p] == 0) L[l].xw[p >> 4] &= (uint16_t)~(1u << (p & 15));
-        s.remx[p >> 4]--;
-        s.remw[p & 15]--;
+          if (rx<NR>(E[i]) * NR + rw<NR>(E[i]) == p) { w = E[i]; E[i] = E.back(); E.pop_back(); break; }
+        if (--L[l].cnt[p] == 0) L[l].xw[p / NR] &= ~(1u << (p % NR));
+        s.remx[p / NR]--;
+        s.remw[p % NR]--;
       }
       out[(size_t)64 * k + lanes[l]] = w;
     }
@@ -519,13 +526,26 @@ static void deal_quarter(std::vector<uint32_t> *rows, const int *lanes, uint32_t
 }  // namespace qsched
 
 // The packed entries of one slice (rows: the 64 lanes' entries, consumed),
-// lane group by lane group.
-static void schedule_slice_packed(std::vector<uint32_t> *rows, uint32_t len, uint32_t pad, uint32_t *out) {
-  for (int c = 0; c < 4; c++) {
-    int lanes[16], n = 0;
-    for (int l = 0; l < 64; l++)
-      if (((l >> 5) & 1) * 2 + (((l >> 4) ^ (l >> 3) ^ (l >> 2)) & 1) == c) lanes[n++] = l;
-    qsched::deal_quarter(rows, lanes, len, pad, out, 200);
+// group by group: for 16-byte reads the 4 measured lane groups of 16 lanes
+// with residues mod 16; for 4- and 8-byte reads the two half-waves with
+// residues mod 32 (measured with the same probe: lanes of different
+// half-waves never conflict, and two distinct addresses of one half-wave
+// conflict when their element indices agree mod 32).
+static void schedule_slice_packed(std::vector<uint32_t> *rows, uint32_t len, uint32_t pad, uint32_t *out,
+                                  bool wide) {
+  if (wide) {
+    for (int c = 0; c < 4; c++) {
+      int lanes[16], n = 0;
+      for (int l = 0; l < 64; l++)
+        if (((l >> 5) & 1) * 2 + (((l >> 4) ^ (l >> 3) ^ (l >> 2)) & 1) == c) lanes[n++] = l;
+      qsched::deal_group<16, 16>(rows, lanes, len, pad, out, 200);
+    }
+  } else {
+    for (int c = 0; c < 2; c++) {
+      int lanes[32];
+      for (int l = 0; l < 32; l++) lanes[l] = 32 * c + l;
+      qsched::deal_group<32, 32>(rows, lanes, len, pad, out, 200);
+    }
   }
 }
 
@@ -568,6 +588,9 @@ void build_singles_ell(Context &c, SpinTables &t) {
   t.has_ell2 = false;
   if (c.K > 4096 || (size_t)c.K + (size_t)n > 32767) return;
   const uint32_t pad2 = ((uint32_t)(c.K + n)) << 4;
+  // the reads the kernel will make: 16-byte values when four FP32 pairs fit
+  // in LDS (the autotuner's choice whenever they do), else 4- or 8-byte ones
+  const bool wide = (size_t)4 * ((size_t)c.K + n + 1) * sizeof(float) <= c.lds_limit;
   // groups of 4 positions: slice lengths rounded up to a multiple of 4, each
   // group stored lane-major (lane l's 4 words contiguous, one 16-byte load),
   // plus 2 groups of padding past the end for the kernel's prefetch
@@ -600,7 +623,7 @@ void build_singles_ell(Context &c, SpinTables &t) {
       }
       const uint32_t len2 = sl_len2[sl];
       dealt.assign((size_t)len2 * 64, pad2);
-      schedule_slice_packed(rows, sl_len[sl], pad2, dealt.data());
+      schedule_slice_packed(rows, sl_len[sl], pad2, dealt.data(), wide);
       uint32_t *o = ell2.data() + sl_off2[sl];
       for (uint32_t k = 0; k < len2; k++)
         for (int l = 0; l < 64; l++) o[256 * (k / 4) + 4 * l + (k % 4)] = dealt[(size_t)64 * k + l];
