@@ -85,7 +85,10 @@ def lds_bytes_mixed(wc, na, nb, n_elec, eb, e=4.0, symmetric=True):
     eb staged alpha singles (plus one spectator round per bra), every slot
     reads two eb-wide values (staged ket row, staged ERI row) per beta single
     and n_elec + 1 eb-wide values for its dressing; the symmetric form visits
-    half the slots."""
+    half the slots.  Width 16 is the signed-triangle kernel: 16-byte pairs,
+    16 / e values each."""
+    if eb == 16:
+        eb = int(16 // e)
     rounds = (wc["alpha_singles"] + na) / eb
     slots = nb / 2 if symmetric else nb
     singles = wc["beta_singles"] / 2 if symmetric else wc["beta_singles"]

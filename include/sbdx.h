@@ -179,15 +179,18 @@ int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_spa
 /* Configuration of the mixed-term kernel chosen for `precision` 0 (FP64) or
  * 1 (FP32) of the general sigma, or 2 (FP64) or 3 (FP32) of the spin-flip
  * symmetric sigma, on the first product of that form by timing every
- * staging width (alpha singles staged in LDS per round: 1, 2, 4) and slots
- * per thread (1, 2, 4, 8; a row's slots are split over blocks accordingly)
- * that fit (no counterpart in the reference): *width 0 before that, -1 when
- * no width fits in LDS (row parts or the CSR/global variants run), else 1,
- * 2 or 4; *slots_per_thread, *ms the winner and its time; *multi (forms 0
+ * staging width (alpha singles staged in LDS per round: 1, 2, 4, or 16 for
+ * the signed-triangle kernel, whose LDS elements are 16-byte pairs: 4 FP32
+ * or 2 FP64 values, and whose staged ERI rows hold the r >= s triangle with
+ * both signs) and slots per thread (1, 2, 4, 8; a row's slots are split
+ * over blocks accordingly) that fit (no counterpart in the reference):
+ * *width 0 before that, -1 when no width fits in LDS (row parts or the
+ * CSR/global variants run), else 1, 2, 4 or 16; *slots_per_thread, *ms the
+ * winner and its time; *multi (forms 0
  * and 1) whether block products use the multi-vector kernel (1) or
  * per-vector launches (0), decided by timing on the first block product
  * (-1 before, and always -1 for forms 2 and 3).
- * SBDX_MIXED_EB=1|2|4 and SBDX_MIXED_NT=1|2|4|8 fix the coordinates. */
+ * SBDX_MIXED_EB=1|2|4|16 and SBDX_MIXED_NT=1|2|4|8 fix the coordinates. */
 int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, int *slots_per_thread,
                        float *ms, int *multi);
 

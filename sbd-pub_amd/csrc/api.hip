@@ -189,6 +189,24 @@ static Context *create_core(int L, std::vector<uint64_t> as, std::vector<uint64_
     c.eri = (double *)dalloc(c, sizeof(double) * K * K);
     SBDX_HIP(hipMemcpyAsync(c.h1, h1.data(), sizeof(double) * K, hipMemcpyHostToDevice, c.stream));
     SBDX_HIP(hipMemcpyAsync(c.eri, eri.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, c.stream));
+    c.eri_rs_sym = true;
+    for (size_t pq = 0; pq < K && c.eri_rs_sym; pq++)
+      for (int a = 1; a < L && c.eri_rs_sym; a++)
+        for (int b = 0; b < a; b++)
+          if (eri[pq * K + (size_t)a * L + b] != eri[pq * K + (size_t)b * L + a]) {
+            c.eri_rs_sym = false;
+            break;
+          }
+    c.Kt = L * (L + 1) / 2;
+    {
+      std::vector<uint16_t> tri((size_t)c.Kt);
+      for (int a = 0; a < L; a++)
+        for (int b = 0; b <= a; b++) tri[(size_t)a * (a + 1) / 2 + b] = (uint16_t)(a * L + b);
+      c.tri_rs = (uint16_t *)dalloc(c, sizeof(uint16_t) * tri.size());
+      SBDX_HIP(hipMemcpyAsync(c.tri_rs, tri.data(), sizeof(uint16_t) * tri.size(), hipMemcpyHostToDevice,
+                              c.stream));
+      SBDX_HIP(hipStreamSynchronize(c.stream));
+    }
     build_spin_tables(c, c.A, as);
     build_singles_ell(c, c.A);
     if (as == bs) {

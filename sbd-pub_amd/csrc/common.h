@@ -73,6 +73,16 @@ struct SpinTables {
   uint64_t *sl_off2 = nullptr;  // [nslice]
   uint32_t *sl_len2 = nullptr;  // [nslice]
   bool has_ell2 = false;
+  // The same entries for the signed-triangle kernel k_mixed_sw (same
+  // slices, lengths and group layout as ell2): ket << 4 | widx << 21, where
+  // widx = tri(pq) for a positive single and Kt + tri(pq) for a negative one
+  // (tri(a*L+b) = max(a,b)(max(a,b)+1)/2 + min(a,b), Kt = L(L+1)/2): w >> 17
+  // is the byte offset of the signed staged ERI value, w & 0x1fff0 that of
+  // the ket value relative to the ket rows; padding (n << 4).  Built when
+  // the ERI matrix is symmetric in its second index pair, Kt <= 1024 and
+  // n < 8192.
+  uint32_t *ell3 = nullptr;
+  bool has_ell3 = false;
   // The same singles split by ket range into `nparts` parts of `part_h`
   // kets (ket stored relative to its part), same perm and slices; built on
   // demand when one ket row does not fit in LDS.
@@ -119,6 +129,11 @@ struct Context {
   size_t na = 0, nb = 0, N = 0;
   double I0 = 0.0;
   double *eri = nullptr;   // [K*K]
+  // (pq|rs) == (pq|sr) bitwise for every pq, r, s (real orbitals): the
+  // mixed kernels may stage each ERI row over the triangle r >= s only
+  bool eri_rs_sym = false;
+  int Kt = 0;                  // L(L+1)/2
+  uint16_t *tri_rs = nullptr;  // [Kt]: rs = a*L + b of triangle element a(a+1)/2 + b, a >= b
   double *h1 = nullptr;    // [K] spatial one-body
   SpinTables A, B;
   double *hii = nullptr;   // [N]

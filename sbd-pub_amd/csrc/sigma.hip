@@ -353,6 +353,211 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
   }
 }
 
+// Byte offset of the ket rows in k_mixed_sw's LDS image: the signed ERI
+// triangle (2*Kt 16-byte elements, Kt <= 1024) lies below it, and the
+// constant fits a DS instruction's 16-bit offset field.
+constexpr uint32_t kSwKetBase = 32768;
+
+// sum_{k in occ} v[tri(k,k)], tri(k,k) = k(k+3)/2 (the staged triangle's
+// diagonal: the dressing sum of k_mixed_sw)
+template <typename T, int EB>
+__device__ __forceinline__ typename Pack<T, EB>::V diag_sum_tri(const typename Pack<T, EB>::V *v, uint64_t occ) {
+  typename Pack<T, EB>::V s;
+#pragma unroll
+  for (int q = 0; q < EB; q++) lane_of<T, EB>(s, q) = T(0);
+  while (occ) {
+    const int k = __builtin_ctzll(occ);
+    s += v[k * (k + 3) / 2];
+    occ &= occ - 1;
+  }
+  return s;
+}
+
+// Mixed alpha-beta term with signed-triangle staging (SpinTables::ell3) and
+// 16-byte pairs (EB = 4 FP32 or 2 FP64 values per LDS element).  Rounds,
+// slots, slices, dressing and the SYM triangle are those of k_mixed_ell; the
+// staged ERI row of each pair is kept over the triangle r >= s only (the ERI
+// matrix being symmetric in its second index pair) and twice, with both
+// signs: element t holds v[tri_rs[t]] and element Kt + t holds -v[tri_rs[t]].
+// A beta single's sign is then part of its ERI address, and one entry costs
+// one mask, one shift, two 16-byte LDS reads and EB fused multiply-adds into
+// one of four running sums (no sign flip, no separate dot product).  The ket
+// rows start at byte kSwKetBase, folded into the ket read's offset field.
+template <typename T, int NT, bool SYM>
+__global__ void __launch_bounds__(1024) k_mixed_sw(int nb, int K, int L, int Kt, int nslots,
+                                                   const uint64_t *__restrict__ a_off,
+                                                   const uint64_t *__restrict__ a_ent,
+                                                   const uint64_t *__restrict__ a_str,
+                                                   const uint64_t *__restrict__ b_str,
+                                                   const uint32_t *__restrict__ ell,
+                                                   const uint32_t *__restrict__ perm,
+                                                   const uint64_t *__restrict__ sl_off,
+                                                   const uint32_t *__restrict__ sl_len,
+                                                   const uint16_t *__restrict__ tri_rs,
+                                                   const T *__restrict__ eri,
+                                                   const T *__restrict__ x, T *__restrict__ y, int odd) {
+  constexpr int EB = 16 / sizeof(T);
+  typedef typename Pack<T, EB>::V V;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  V *W = reinterpret_cast<V *>(lds_raw);               // 2*Kt signed ERI values per pair
+  V *X = reinterpret_cast<V *>(lds_raw + kSwKetBase);  // nb ket values per pair, then a zero
+  // the kernel has no static LDS, so the dynamic array starts at LDS
+  // address 0 (checked by the launcher) and an offset is an address
+  typedef __attribute__((address_space(3))) const V lds_v;
+  auto at = [](uint32_t off) { return *reinterpret_cast<lds_v *>((size_t)off); };
+  const int L1 = L + 1;
+  const int pos = blockIdx.x;
+  const int I = SYM ? (int)perm[pos] : blockIdx.x;
+  const int base = (SYM ? (pos & ~63) : 0) + blockIdx.y * (1024 * NT);
+  if (SYM && base >= nslots) return;  // uniform per block, before any barrier
+  const int tid = base + threadIdx.x;
+  T acc[NT];
+  uint64_t occ[NT];
+  int rr[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    acc[t] = T(0);
+    const int slot = tid + t * 1024;
+    const uint32_t r = slot < nslots ? perm[slot] : 0xffffffffu;
+    rr[t] = r != 0xffffffffu ? (int)r : 0;
+    occ[t] = r != 0xffffffffu ? b_str[r] : 0ull;  // padding slots: no dressing
+  }
+  const uint64_t occ_a = a_str[I];
+  const uint64_t e0 = a_off[I], e1 = a_off[I + 1] + 1;  // + the spectator pair
+  for (uint64_t e = e0; e < e1; e += EB) {
+    // pair q of this round: kind 1 = alpha single, 2 = spectator, 0 = empty
+    int kind[EB];
+    const T *xsrc[EB];
+    const T *vsrc[EB];
+    T sgn[EB];
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+      kind[q] = 0;
+      xsrc[q] = x;
+      vsrc[q] = eri;
+      sgn[q] = T(1);
+      if (e + q + 1 < e1) {
+        const uint64_t ent = a_ent[e + q];
+        kind[q] = 1;
+        xsrc[q] = x + (size_t)ent_ket(ent) * nb;
+        vsrc[q] = eri + (size_t)ent_pq(ent) * K;
+        sgn[q] = ent_neg(ent) ? T(-1) : T(1);
+      } else if (e + q + 1 == e1) {
+        kind[q] = 2;
+        xsrc[q] = x + (size_t)I * nb;
+      }
+    }
+    // every staged value of the round is loaded before any is stored
+    // (nb + 1 <= 8192 ket values: at most 8 per thread; Kt <= 1024 ERI
+    // values: at most one), so the round waits for one memory latency
+    // rather than one per loop trip
+    V xs[8], ws;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int i = threadIdx.x + k * 1024;
+#pragma unroll
+      for (int q = 0; q < EB; q++) lane_of<T, EB>(xs[k], q) = (kind[q] && i < nb) ? xsrc[q][i] : T(0);
+    }
+    const bool wl = (int)threadIdx.x < Kt;
+    {
+      const int rs = wl ? tri_rs[threadIdx.x] : 0;
+#pragma unroll
+      for (int q = 0; q < EB; q++) {
+        T w = T(0);
+        if (wl && kind[q] == 1) w = sgn[q] * vsrc[q][rs];
+        else if (wl && kind[q] == 2) w = spectator_val(eri, occ_a, L1, K, rs);
+        lane_of<T, EB>(ws, q) = w;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int i = threadIdx.x + k * 1024;
+      if (i <= nb) X[i] = xs[k];
+    }
+    if (wl) {
+      W[threadIdx.x] = ws;
+      W[Kt + threadIdx.x] = -ws;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int slot = tid + t * 1024;
+      if (slot < nslots) {
+        // a wavefront's 64 slots lie in one slice (slot bases are multiples
+        // of 64), so the slice's group count and offset are wave-uniform
+        const int sl = __builtin_amdgcn_readfirstlane(slot >> 6);
+        const uint32_t ng = __builtin_amdgcn_readfirstlane(sl_len[sl]) / 4;
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 *p = reinterpret_cast<const u32x4 *>(ell + sl_off[sl]) + (slot & 63);
+        T s4[4] = {T(0), T(0), T(0), T(0)};
+        // one group: 8 LDS reads issued together, then 4*EB FMAs into four
+        // independent running sums
+        auto group = [&](const u32x4 &cur) {
+          const uint32_t w4[4] = {cur.x, cur.y, cur.z, cur.w};
+          V wv[4], xv[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            wv[u] = at(w4[u] >> 17);
+            xv[u] = at((w4[u] & 0x1fff0u) + kSwKetBase);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int qq = 0; qq < EB; qq++)
+              s4[u] = fma(lane_c<T, EB>(wv[u], qq), lane_c<T, EB>(xv[u], qq), s4[u]);
+        };
+        if (ng > 0) {
+          // two table-word buffers in turn: the words of group g + 2 load
+          // into the buffer group g has just used, so each load has a whole
+          // group of LDS reads and FMAs to land and no register copies are
+          // needed.  The loads run at most one group past the slice (into
+          // the next slice or the table's two groups of padding); those
+          // words are never used.
+          u32x4 b0 = p[0], b1 = p[64];
+          typedef __attribute__((address_space(1))) const u32x4 gl_u4;
+          gl_u4 *q = (gl_u4 *)(p + 128);
+          uint32_t g = 0;
+          for (; g + 2 <= ng; g += 2) {
+            // an opaque address (an empty asm statement, no instruction):
+            // otherwise the compiler re-loads each group where it is used
+            asm volatile("" : "+v"(q));
+            group(b0);
+            b0 = q[0];
+            group(b1);
+            b1 = q[64];
+            q += 128;
+          }
+          if (g < ng) group(b0);
+        }
+        const V dv = diag_sum_tri<T, EB>(W, occ[t]) * X[rr[t]];
+        T a = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+#pragma unroll
+        for (int qq = 0; qq < EB; qq++) a += lane_c<T, EB>(dv, qq);
+        acc[t] += a;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int slot = tid + t * 1024;
+    if (slot < nslots) {
+      const T v = acc[t];
+      const uint32_t r = perm[slot];
+      if (SYM) {
+        if (r != 0xffffffffu && slot > pos) {
+          y[(size_t)I * nb + r] += v;
+          y[(size_t)r * nb + I] += odd ? -v : v;
+        } else if (r != 0xffffffffu && slot == pos && !odd) {
+          y[(size_t)I * nb + r] += v;  // odd sector: the diagonal stays 0
+        }
+      } else {
+        if (r != 0xffffffffu) y[(size_t)I * nb + r] += v;
+      }
+    }
+  }
+}
+
 // Mixed term for KV vectors at once (x_v = x + v*N): one ERI row and the KV
 // ket rows x_v[J,:] staged per alpha single (then the spectator pair), so
 // each sliced-ELL entry costs one table load and 1 + KV LDS reads for KV
@@ -895,6 +1100,41 @@ static bool mixed_ell_dispatch(Context &c, int nt, const T *eri, const T *x, T *
   return true;
 }
 
+// Staging width code of the signed-triangle kernel in the autotuner's
+// (width, slots) choice: 16-byte pairs, 4 FP32 or 2 FP64 values.
+constexpr int kEbSigned = 16;
+
+template <typename T, int NT, bool SYM>
+static void launch_mixed_sw(Context &c, const T *eri, const T *x, T *y, size_t lds) {
+  auto fn = k_mixed_sw<T, NT, SYM>;
+  static bool lds_base_checked = false;
+  if (!lds_base_checked) {
+    hipFuncAttributes attr;
+    SBDX_HIP(hipFuncGetAttributes(&attr, (const void *)fn));
+    if (attr.sharedSizeBytes != 0) throw Error(2, "k_mixed_sw: static LDS would offset the dynamic array");
+    lds_base_checked = true;
+  }
+  SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int nslots = c.B.nslice * 64;
+  const unsigned nsplit = cdiv(nslots, 1024 * NT);
+  hipLaunchKernelGGL(fn, dim3((unsigned)c.na, nsplit), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L, c.Kt,
+                     nslots, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, c.B.ell3, c.B.perm, c.B.sl_off2,
+                     c.B.sl_len2, c.tri_rs, eri, x, y, (SYM && c.sym_par < 0) ? 1 : 0);
+}
+
+// The signed-triangle kernel, NT slots per thread; false when its table was
+// not built or its LDS image does not fit.
+template <typename T, bool SYM = false>
+static bool mixed_sw_dispatch(Context &c, int nt, const T *eri, const T *x, T *y) {
+  const size_t lds = kSwKetBase + (size_t)16 * ((size_t)c.nb + 1);
+  if (!c.B.has_ell3 || (size_t)32 * c.Kt > kSwKetBase || c.nb + 1 > 8192 || lds > c.lds_limit) return false;
+  if (nt <= 1) launch_mixed_sw<T, 1, SYM>(c, eri, x, y, lds);
+  else if (nt <= 2) launch_mixed_sw<T, 2, SYM>(c, eri, x, y, lds);
+  else if (nt <= 4) launch_mixed_sw<T, 4, SYM>(c, eri, x, y, lds);
+  else launch_mixed_sw<T, 8, SYM>(c, eri, x, y, lds);
+  return true;
+}
+
 template <typename T, int NT>
 static void launch_mixed_lds(Context &c, const T *eri, const T *x, T *y, size_t lds) {
   auto fn = k_mixed_lds<T, NT>;
@@ -943,13 +1183,15 @@ static bool mixed_parts_dispatch(Context &c, const T *eri, const T *x, T *y) {
 
 template <bool SYM = false>
 static bool mixed_ell_cfg(Context &c, int eb, int nt, const double *eri, const double *x, double *y) {
-  return eb == 4 ? mixed_ell_dispatch<double, 4, SYM>(c, nt, eri, x, y)
+  return eb == kEbSigned ? mixed_sw_dispatch<double, SYM>(c, nt, eri, x, y)
+       : eb == 4 ? mixed_ell_dispatch<double, 4, SYM>(c, nt, eri, x, y)
        : eb == 2 ? mixed_ell_dispatch<double, 2, SYM>(c, nt, eri, x, y)
                  : mixed_ell_dispatch<double, 1, SYM>(c, nt, eri, x, y);
 }
 template <bool SYM = false>
 static bool mixed_ell_cfg(Context &c, int eb, int nt, const float *eri, const float *x, float *y) {
-  return eb == 4 ? mixed_ell_dispatch<float, 4, SYM>(c, nt, eri, x, y)
+  return eb == kEbSigned ? mixed_sw_dispatch<float, SYM>(c, nt, eri, x, y)
+       : eb == 4 ? mixed_ell_dispatch<float, 4, SYM>(c, nt, eri, x, y)
        : eb == 2 ? mixed_ell_dispatch<float, 2, SYM>(c, nt, eri, x, y)
                  : mixed_ell_dispatch<float, 1, SYM>(c, nt, eri, x, y);
 }
@@ -957,12 +1199,13 @@ static double *scratch_vec(Context &c, const double *) { return c.T2; }
 static float *scratch_vec(Context &c, const float *) { return c.f32.T2; }
 
 // Autotuner for the sliced-ELL mixed kernel: every (staging width EB in
-// {1, 2, 4}, slots per thread NT in {1, 2, 4, 8}) whose LDS footprint fits
+// {1, 2, 4, kEbSigned (k_mixed_sw)}, slots per thread NT in {1, 2, 4, 8})
+// whose LDS footprint fits
 // and that does not cover fewer than half of a row's slots per thread
 // budget runs twice on this x into a scratch vector (the kernel
 // accumulates, so y is untouched); HIP events on the stream time the second
 // run and the fastest is kept for this precision (-1: nothing fits).
-// SBDX_MIXED_EB=1|2|4 and SBDX_MIXED_NT=1|2|4|8 fix either coordinate.
+// SBDX_MIXED_EB=1|2|4|16 and SBDX_MIXED_NT=1|2|4|8 fix either coordinate.
 template <typename T, bool SYM = false>
 static int mixed_cfg(Context &c, const T *eri, const T *x) {
   const int pi = (sizeof(T) == 4 ? 1 : 0) + (SYM ? 2 : 0);
@@ -971,16 +1214,18 @@ static int mixed_cfg(Context &c, const T *eri, const T *x) {
   const char *env_eb = std::getenv("SBDX_MIXED_EB");
   const char *env_nt = std::getenv("SBDX_MIXED_NT");
   const int fix_eb = env_eb ? std::atoi(env_eb) : 0, fix_nt = env_nt ? std::atoi(env_nt) : 0;
-  if (env_eb && fix_eb != 1 && fix_eb != 2 && fix_eb != 4) throw Error(1, "SBDX_MIXED_EB must be 1, 2 or 4");
+  if (env_eb && fix_eb != 1 && fix_eb != 2 && fix_eb != 4 && fix_eb != kEbSigned)
+    throw Error(1, "SBDX_MIXED_EB must be 1, 2, 4 or 16");
   if (env_nt && fix_nt != 1 && fix_nt != 2 && fix_nt != 4 && fix_nt != 8)
     throw Error(1, "SBDX_MIXED_NT must be 1, 2, 4 or 8");
   T *scratch = scratch_vec(c, x);
   hipEvent_t e0, e1;
   SBDX_HIP(hipEventCreate(&e0));
   SBDX_HIP(hipEventCreate(&e1));
-  // every timed configuration, in the fixed order (eb, then nt, ascending)
+  // every timed configuration, in the fixed order (the signed-triangle
+  // kernel, then eb descending; nt ascending)
   std::vector<std::array<float, 3>> timed;
-  for (int eb : {1, 2, 4}) {
+  for (int eb : {kEbSigned, 4, 2, 1}) {
     if (fix_eb && eb != fix_eb) continue;
     for (int nt : {1, 2, 4, 8}) {
       if (fix_nt && nt != fix_nt) continue;
@@ -1716,7 +1961,8 @@ static void sigma_sym_impl(Context &c, const T *x, T *y) {
     const int eb = mixed_cfg<T, true>(c, o.eri, x);
     const int nt = c.mixed_nt[pi];
     bool ok = false;
-    if (eb == 4) ok = mixed_ell_dispatch<T, 4, true>(c, nt, o.eri, x, y);
+    if (eb == kEbSigned) ok = mixed_sw_dispatch<T, true>(c, nt, o.eri, x, y);
+    else if (eb == 4) ok = mixed_ell_dispatch<T, 4, true>(c, nt, o.eri, x, y);
     else if (eb == 2) ok = mixed_ell_dispatch<T, 2, true>(c, nt, o.eri, x, y);
     else if (eb == 1) ok = mixed_ell_dispatch<T, 1, true>(c, nt, o.eri, x, y);
     if (!ok) throw Error(3, "symmetric mixed kernel: staging does not fit");

@@ -586,6 +586,7 @@ void build_singles_ell(Context &c, SpinTables &t) {
   t.has_ell = true;
   // the packed form for k_mixed_ell (same rows, slices and lengths)
   t.has_ell2 = false;
+  t.has_ell3 = false;
   if (c.K > 4096 || (size_t)c.K + (size_t)n > 32767) return;
   const uint32_t pad2 = ((uint32_t)(c.K + n)) << 4;
   // the reads the kernel will make: 16-byte values when four FP32 pairs fit
@@ -602,37 +603,68 @@ void build_singles_ell(Context &c, SpinTables &t) {
     sl_off2[sl] = total2;
     total2 += (uint64_t)sl_len2[sl] * 64;
   }
-  std::vector<uint32_t> ell2(total2 + 2 * 256, pad2);
-  // slices are independent: dealt on up to 16 host threads
-  const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  std::atomic<int> next{0};
-  auto worker = [&]() {
-    std::vector<uint32_t> rows[64];
-    std::vector<uint32_t> dealt;
-    for (int sl = next++; sl < nsl; sl = next++) {
-      for (int l = 0; l < 64; l++) {
-        rows[l].clear();
-        const uint32_t r = perm[(size_t)sl * 64 + l];
-        if (r == 0xffffffffu) continue;
-        for (uint64_t k = off[r]; k < off[r + 1]; k++) {
-          const uint64_t e = ent[k];
+  // deal every slice's entries (word_of: the packed word of one singles
+  // entry, pad the padding word) into the group layout; slices are
+  // independent and dealt on up to 16 host threads
+  auto deal_all = [&](auto word_of, uint32_t pad, std::vector<uint32_t> &dst) {
+    dst.assign(total2 + 2 * 256, pad);
+    const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+      std::vector<uint32_t> rows[64];
+      std::vector<uint32_t> dealt;
+      for (int sl = next++; sl < nsl; sl = next++) {
+        for (int l = 0; l < 64; l++) {
+          rows[l].clear();
+          const uint32_t r = perm[(size_t)sl * 64 + l];
+          if (r == 0xffffffffu) continue;
+          for (uint64_t k = off[r]; k < off[r + 1]; k++) rows[l].push_back(word_of(ent[k]));
+        }
+        const uint32_t len2 = sl_len2[sl];
+        dealt.assign((size_t)len2 * 64, pad);
+        schedule_slice_packed(rows, sl_len[sl], pad, dealt.data(), wide);
+        uint32_t *o = dst.data() + sl_off2[sl];
+        for (uint32_t k = 0; k < len2; k++)
+          for (int l = 0; l < 64; l++) o[256 * (k / 4) + 4 * l + (k % 4)] = dealt[(size_t)64 * k + l];
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nthr; i++) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+  };
+  std::vector<uint32_t> ell2;
+  deal_all(
+      [&](uint64_t e) {
+        const uint32_t ket = (uint32_t)(e & 0xffffffffull);
+        const uint32_t pq = (uint32_t)((e >> 32) & 0x7fffffffull);
+        return ((uint32_t)(c.K + ket) << 4) | (pq << 19) | ((e >> 63) ? 0x80000000u : 0u);
+      },
+      pad2, ell2);
+  // the signed-triangle form: dealt in the ell2 word layout with the ket
+  // element at 2048 + ket (the kernel's ket rows start at byte 32768 = 2048
+  // 16-byte elements, so residues match the LDS addresses) and the signed
+  // triangle index in the pq field, then repacked
+  if (c.eri_rs_sym && c.Kt <= 1024 && n < 8192 && wide) {
+    const uint32_t Kt = (uint32_t)c.Kt;
+    const int L = c.L;
+    std::vector<uint32_t> ell3;
+    deal_all(
+        [&](uint64_t e) {
           const uint32_t ket = (uint32_t)(e & 0xffffffffull);
           const uint32_t pq = (uint32_t)((e >> 32) & 0x7fffffffull);
-          rows[l].push_back(((uint32_t)(c.K + ket) << 4) | (pq << 19) | ((e >> 63) ? 0x80000000u : 0u));
-        }
-      }
-      const uint32_t len2 = sl_len2[sl];
-      dealt.assign((size_t)len2 * 64, pad2);
-      schedule_slice_packed(rows, sl_len[sl], pad2, dealt.data(), wide);
-      uint32_t *o = ell2.data() + sl_off2[sl];
-      for (uint32_t k = 0; k < len2; k++)
-        for (int l = 0; l < 64; l++) o[256 * (k / 4) + 4 * l + (k % 4)] = dealt[(size_t)64 * k + l];
-    }
-  };
-  std::vector<std::thread> pool;
-  for (int i = 1; i < nthr; i++) pool.emplace_back(worker);
-  worker();
-  for (auto &th : pool) th.join();
+          const uint32_t a = pq / L, b = pq % L, hi = a > b ? a : b, lo = a > b ? b : a;
+          const uint32_t widx = hi * (hi + 1) / 2 + lo + ((e >> 63) ? Kt : 0u);
+          return ((2048u + ket) << 4) | (widx << 19);
+        },
+        (2048u + (uint32_t)n) << 4, ell3);
+    for (uint32_t &w : ell3) w = ((((w >> 4) & 0x7fffu) - 2048u) << 4) | (((w >> 19) & 0xfffu) << 21);
+    t.ell3 = (uint32_t *)dalloc(c, sizeof(uint32_t) * ell3.size());
+    SBDX_HIP(hipMemcpyAsync(t.ell3, ell3.data(), sizeof(uint32_t) * ell3.size(), hipMemcpyHostToDevice,
+                            c.stream));
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+    t.has_ell3 = true;
+  }
   t.ell2 = (uint32_t *)dalloc(c, sizeof(uint32_t) * ell2.size());
   t.sl_off2 = (uint64_t *)dalloc(c, sizeof(uint64_t) * nsl);
   t.sl_len2 = (uint32_t *)dalloc(c, sizeof(uint32_t) * nsl);
