@@ -186,7 +186,9 @@ int sbdx_same_spin_form(sbdx_handle_t h, int precision, int *form, float *ms_spa
  * over blocks accordingly) that fit (no counterpart in the reference):
  * *width 0 before that, -1 when no width fits in LDS (row parts or the
  * CSR/global variants run), else 1, 2, 4 or 16; *slots_per_thread, *ms the
- * winner and its time; *multi (forms 0
+ * winner and its time (with width -1: the row-parts kernel's slots per
+ * thread, timed the same way on its first use; SBDX_PARTS_NT fixes it);
+ * *multi (forms 0
  * and 1) whether block products use the multi-vector kernel (1) or
  * per-vector launches (0), decided by timing on the first block product
  * (-1 before, and always -1 for forms 2 and 3).

@@ -572,8 +572,11 @@ int sbdx_mixed_staging(sbdx_handle_t h, int precision, int *width, int *slots_pe
   SBDX_GUARD_BEGIN
   Context &c = ctx_of(h);
   if (precision < 0 || precision > 3) throw Error(SBDX_ERR_ARG, "precision must be 0, 1, 2 or 3");
-  if (width) *width = c.mixed_eb[precision];
-  if (slots_per_thread) *slots_per_thread = c.mixed_nt[precision];
+  // the row-parts kernel (no staging width fits) reports width -1 and its
+  // own tuned slots per thread
+  const bool parts = c.mixed_eb[precision] <= 0 && c.parts_nt[precision] > 0;
+  if (width) *width = parts ? -1 : c.mixed_eb[precision];
+  if (slots_per_thread) *slots_per_thread = parts ? c.parts_nt[precision] : c.mixed_nt[precision];
   if (ms) *ms = c.mixed_best_ms[precision];
   if (multi) *multi = precision < 2 ? c.multi_mode[precision] : -1;
   return SBDX_OK;

@@ -179,6 +179,9 @@ struct Context {
   int mixed_eb[4] = {0, 0, 0, 0};
   int mixed_nt[4] = {0, 0, 0, 0};
   float mixed_best_ms[4] = {0.f, 0.f, 0.f, 0.f};
+  // slots per thread of the row-parts mixed kernel, per form as mixed_eb
+  // (0 until timed on first use)
+  int parts_nt[4] = {0, 0, 0, 0};
   // block products: 1 when the multi-vector kernel beat per-vector launches
   // on first use, 0 when not, -1 not decided
   int multi_mode[2] = {-1, -1};

@@ -268,23 +268,50 @@ __global__ void __launch_bounds__(1024) k_mixed_ell(int nb, int K, int L, int ns
         xsrc[q] = x + (size_t)I * nb;
       }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i <= nb; i += 1024) {
-      V v;
+    // the ket rows are staged CH loop trips at a time (16 registers), all
+    // loads of a chunk before its stores, so a chunk waits for one memory
+    // latency rather than one per trip; the first chunk loads before the
+    // barrier that ends the previous round's reads
+    constexpr int CH = sizeof(V) >= 16 ? 4 : sizeof(V) == 8 ? 8 : 16;
+    V xs[CH];
+    auto load_x = [&](int i0) {
 #pragma unroll
-      for (int q = 0; q < EB; q++) lane_of<T, EB>(v, q) = (kind[q] && i < nb) ? xsrc[q][i] : T(0);
-      X[i] = v;
-    }
-    for (int i = threadIdx.x; i < K; i += 1024) {
-      V v;
+      for (int k = 0; k < CH; k++) {
+        const int i = i0 + k * 1024;
 #pragma unroll
-      for (int q = 0; q < EB; q++) {
-        T w = T(0);
-        if (kind[q] == 1) w = sgn[q] * vsrc[q][i];
-        else if (kind[q] == 2) w = spectator_val(eri, occ_a, L1, K, i);
-        lane_of<T, EB>(v, q) = w;
+        for (int q = 0; q < EB; q++) lane_of<T, EB>(xs[k], q) = (kind[q] && i < nb) ? xsrc[q][i] : T(0);
       }
-      W[i] = v;
+    };
+    auto store_x = [&](int i0) {
+#pragma unroll
+      for (int k = 0; k < CH; k++) {
+        const int i = i0 + k * 1024;
+        if (i <= nb) X[i] = xs[k];
+      }
+    };
+    load_x(threadIdx.x);
+    __syncthreads();
+    store_x(threadIdx.x);
+    for (int i0 = threadIdx.x + CH * 1024; i0 <= nb; i0 += CH * 1024) {
+      load_x(i0);
+      store_x(i0);
+    }
+    for (int i0 = threadIdx.x; i0 < K; i0 += 4 * 1024) {
+      V ws[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int i = i0 + k * 1024;
+#pragma unroll
+        for (int q = 0; q < EB; q++) {
+          T w = T(0);
+          if (i < K && kind[q] == 1) w = sgn[q] * vsrc[q][i];
+          else if (i < K && kind[q] == 2) w = spectator_val(eri, occ_a, L1, K, i);
+          lane_of<T, EB>(ws[k], q) = w;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (i0 + k * 1024 < K) W[i0 + k * 1024] = ws[k];
     }
     __syncthreads();
 #pragma unroll
@@ -676,8 +703,11 @@ __global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int L, int 
   const int L1 = L + 1;
   const int pos = blockIdx.x;
   const int I = SYM ? (int)perm[pos] : blockIdx.x;
-  // SYM: slots from pos's slice on (as in k_mixed_ell); NT covers the rest
-  const int tid = (SYM ? (pos & ~63) : 0) + threadIdx.x;
+  // SYM: slots from pos's slice on (as in k_mixed_ell); the row's slots are
+  // split over gridDim.y blocks of 1024*NT slots each
+  const int base = (SYM ? (pos & ~63) : 0) + blockIdx.y * (1024 * NT);
+  if (base >= nslots) return;  // uniform per block, before any barrier
+  const int tid = base + threadIdx.x;
   T acc[NT];
   uint64_t occ[NT];
   int rr[NT];
@@ -695,12 +725,22 @@ __global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int L, int 
     const bool spect = e == e1;
     size_t J = I;
     __syncthreads();
+    // staged rows go CH loop trips at a time (16 registers), all loads of a
+    // chunk before its stores: one memory latency per chunk, not per trip
+    constexpr int CH = sizeof(T) == 8 ? 8 : 16;
+    T st[CH];
     if (!spect) {
       const uint64_t ent = a_ent[e];
       J = ent_ket(ent);
       const T sa = ent_neg(ent) ? T(-1) : T(1);
       const T *vsrc = eri + (size_t)ent_pq(ent) * K;
-      for (int i = threadIdx.x; i < K; i += 1024) vrow[i] = sa * vsrc[i];
+      for (int i0 = threadIdx.x; i0 < K; i0 += CH * 1024) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) st[k] = i0 + k * 1024 < K ? sa * vsrc[i0 + k * 1024] : T(0);
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+          if (i0 + k * 1024 < K) vrow[i0 + k * 1024] = st[k];
+      }
     } else {
       for (int i = threadIdx.x; i < K; i += 1024) vrow[i] = spectator_val(eri, occ_a, L1, K, i);
     }
@@ -708,8 +748,27 @@ __global__ void __launch_bounds__(1024) k_mixed_parts(int nb, int K, int L, int 
     for (int p = 0; p < nparts; p++) {
       const int lo = p * h;
       const int len_p = (nb - lo) < h ? (nb - lo) : h;
+      const T *src = x + J * nb + lo;
+      // the part's first chunk loads before the barrier that ends the
+      // previous part's reads
+#pragma unroll
+      for (int k = 0; k < CH; k++) {
+        const int i = threadIdx.x + k * 1024;
+        st[k] = i < len_p ? src[i] : T(0);
+      }
       __syncthreads();
-      for (int i = threadIdx.x; i < len_p; i += 1024) crow[i] = x[J * nb + lo + i];
+#pragma unroll
+      for (int k = 0; k < CH; k++) {
+        const int i = threadIdx.x + k * 1024;
+        if (i < len_p) crow[i] = st[k];
+      }
+      for (int i0 = threadIdx.x + CH * 1024; i0 < len_p; i0 += CH * 1024) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) st[k] = i0 + k * 1024 < len_p ? src[i0 + k * 1024] : T(0);
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+          if (i0 + k * 1024 < len_p) crow[i0 + k * 1024] = st[k];
+      }
       __syncthreads();
       const uint32_t *ellp = pt.ell[p];
       const uint64_t *offp = pt.off[p];
@@ -1153,16 +1212,74 @@ static void launch_mixed_parts(Context &c, const T *eri, const T *x, T *y, size_
     pt.off[p] = c.B.psl_off[p];
     pt.len[p] = c.B.psl_len[p];
   }
-  hipLaunchKernelGGL(fn, dim3((unsigned)c.na), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L,
-                     c.B.nslice * 64, c.B.nparts, c.B.part_h, c.A.s_off, c.A.s_ent, c.A.str, c.B.str,
-                     pt, c.B.perm, eri, x, y, (SYM && c.sym_par < 0) ? 1 : 0);
+  const int nslots = c.B.nslice * 64;
+  const unsigned nsplit = cdiv(nslots, 1024 * NT);
+  hipLaunchKernelGGL(fn, dim3((unsigned)c.na, nsplit), dim3(1024), lds, c.stream, (int)c.nb, c.K, c.L, nslots,
+                     c.B.nparts, c.B.part_h, c.A.s_off, c.A.s_ent, c.A.str, c.B.str, pt, c.B.perm, eri, x,
+                     y, (SYM && c.sym_par < 0) ? 1 : 0);
+}
+
+template <typename T, bool SYM = false>
+static void launch_mixed_parts_nt(Context &c, int nt, const T *eri, const T *x, T *y, size_t lds) {
+  if (nt <= 1) launch_mixed_parts<T, 1, SYM>(c, eri, x, y, lds);
+  else if (nt <= 2) launch_mixed_parts<T, 2, SYM>(c, eri, x, y, lds);
+  else if (nt <= 4) launch_mixed_parts<T, 4, SYM>(c, eri, x, y, lds);
+  else if (nt <= 8) launch_mixed_parts<T, 8, SYM>(c, eri, x, y, lds);
+  else if (nt <= 16) launch_mixed_parts<T, 16, SYM>(c, eri, x, y, lds);
+  else launch_mixed_parts<T, 32, SYM>(c, eri, x, y, lds);
+}
+
+static double *scratch_vec(Context &c, const double *) { return c.T2; }
+static float *scratch_vec(Context &c, const float *) { return c.f32.T2; }
+
+// Slots per thread NT of the row-parts kernel (a row's slots split over
+// ceil(slots / (1024 NT)) blocks, each staging the same rows): every power
+// of two up to the one that covers a row in one block (at most 32) runs
+// twice on this x into a scratch vector on the first product of this
+// precision and form, HIP events time the second run, and the first within
+// 3% of the fastest is kept (large NT saves staging, small NT registers).
+// SBDX_PARTS_NT=1|2|4|8|16|32 fixes it.
+template <typename T, bool SYM = false>
+static int parts_nt(Context &c, const T *eri, const T *x, size_t lds) {
+  const int pi = (sizeof(T) == 4 ? 1 : 0) + (SYM ? 2 : 0);
+  if (c.parts_nt[pi] > 0) return c.parts_nt[pi];
+  const int nt_max = (int)cdiv((size_t)c.B.nslice * 64, 1024);
+  if (const char *ev = std::getenv("SBDX_PARTS_NT")) {
+    const int v = std::atoi(ev);
+    if (v != 1 && v != 2 && v != 4 && v != 8 && v != 16 && v != 32)
+      throw Error(1, "SBDX_PARTS_NT must be 1, 2, 4, 8, 16 or 32");
+    return c.parts_nt[pi] = v;
+  }
+  T *scratch = scratch_vec(c, x);
+  hipEvent_t e0, e1;
+  SBDX_HIP(hipEventCreate(&e0));
+  SBDX_HIP(hipEventCreate(&e1));
+  std::vector<std::pair<int, float>> timed;
+  for (int nt : {1, 2, 4, 8, 16, 32}) {
+    if (nt > 1 && nt / 2 >= nt_max) break;  // a smaller NT already covers the row
+    float ms = 0.f;
+    for (int rep = 0; rep < 2; rep++) {
+      SBDX_HIP(hipEventRecord(e0, c.stream));
+      launch_mixed_parts_nt<T, SYM>(c, nt, eri, x, scratch, lds);
+      SBDX_HIP(hipEventRecord(e1, c.stream));
+      SBDX_HIP(hipEventSynchronize(e1));
+      SBDX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    }
+    check_launch("k_mixed_parts");
+    timed.push_back({nt, ms});
+  }
+  SBDX_HIP(hipEventDestroy(e0));
+  SBDX_HIP(hipEventDestroy(e1));
+  float fastest = timed[0].second;
+  for (const auto &t : timed) fastest = std::min(fastest, t.second);
+  for (const auto &t : timed)
+    if (t.second <= 1.03f * fastest) return c.parts_nt[pi] = t.first;
+  return c.parts_nt[pi] = timed.back().first;
 }
 
 // Ket row staged in parts: smallest part count (<= 8) whose piece and ERI row fit.
 template <typename T, bool SYM = false>
 static bool mixed_parts_dispatch(Context &c, const T *eri, const T *x, T *y) {
-  const int nt = (c.B.nslice * 64 + 1023) / 1024;
-  if (nt > 32) return false;
   // Parts already built (e.g. for the other precision) are kept when they fit.
   const int first = c.B.nparts >= 2 ? c.B.nparts : 2;
   for (int P = first; P <= 8; P++) {
@@ -1170,12 +1287,7 @@ static bool mixed_parts_dispatch(Context &c, const T *eri, const T *x, T *y) {
     const size_t lds = sizeof(T) * (h + c.K + 1);
     if (lds > c.lds_limit) continue;
     build_singles_ell_parts(c, c.B, P);
-    if (nt <= 1) launch_mixed_parts<T, 1, SYM>(c, eri, x, y, lds);
-    else if (nt <= 2) launch_mixed_parts<T, 2, SYM>(c, eri, x, y, lds);
-    else if (nt <= 4) launch_mixed_parts<T, 4, SYM>(c, eri, x, y, lds);
-    else if (nt <= 8) launch_mixed_parts<T, 8, SYM>(c, eri, x, y, lds);
-    else if (nt <= 16) launch_mixed_parts<T, 16, SYM>(c, eri, x, y, lds);
-    else launch_mixed_parts<T, 32, SYM>(c, eri, x, y, lds);
+    launch_mixed_parts_nt<T, SYM>(c, parts_nt<T, SYM>(c, eri, x, lds), eri, x, y, lds);
     return true;
   }
   return false;
@@ -1195,9 +1307,6 @@ static bool mixed_ell_cfg(Context &c, int eb, int nt, const float *eri, const fl
        : eb == 2 ? mixed_ell_dispatch<float, 2, SYM>(c, nt, eri, x, y)
                  : mixed_ell_dispatch<float, 1, SYM>(c, nt, eri, x, y);
 }
-static double *scratch_vec(Context &c, const double *) { return c.T2; }
-static float *scratch_vec(Context &c, const float *) { return c.f32.T2; }
-
 // Autotuner for the sliced-ELL mixed kernel: every (staging width EB in
 // {1, 2, 4, kEbSigned (k_mixed_sw)}, slots per thread NT in {1, 2, 4, 8})
 // whose LDS footprint fits
