@@ -84,16 +84,34 @@ __global__ void __launch_bounds__(kRedThreads) k_dots(VecList X, int nv, const d
   }
 }
 
-// Fixed-order sum of the partials of column k; written to H[k + ld*col] and
-// H[col + ld*k] (the symmetric subspace matrix, column-major).
-__global__ void k_finish_hcol(int nv, int nblk, const double *__restrict__ part, double *H, int ld,
-                              int col) {
-  const int k = threadIdx.x;
-  if (k >= nv) return;
+// Fixed-order sum of part[b*stride + off] over b < nblk by one whole
+// wavefront: lane l adds b = l, l + 64, ... in order, then a butterfly over
+// the 64 lanes (each step adds a pair of lanes' values in both lanes, so
+// every lane ends with the same total).  A single thread summing the 1024
+// partials one load at a time took ~0.1 ms.
+__device__ __forceinline__ double wave_sum(const double *part, int nblk, int stride, int off) {
+  const int lane = threadIdx.x & 63;
   double s = 0.0;
-  for (int b = 0; b < nblk; b++) s += part[(size_t)b * nv + k];
-  H[k + (size_t)ld * col] = s;
-  H[col + (size_t)ld * k] = s;
+  for (int b = lane; b < nblk; b += 64) s += part[(size_t)b * stride + off];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+// threads of the one-block finishing kernels (4 wavefronts)
+constexpr int kFinishThreads = 256;
+
+// Fixed-order sum of the partials of column k; written to H[k + ld*col] and
+// H[col + ld*k] (the symmetric subspace matrix, column-major).  Wavefront w
+// sums columns w, w + 4, ...
+__global__ void __launch_bounds__(kFinishThreads) k_finish_hcol(int nv, int nblk, const double *__restrict__ part,
+                                                                double *H, int ld, int col) {
+  for (int k = threadIdx.x >> 6; k < nv; k += kFinishThreads / 64) {
+    const double s = wave_sum(part, nblk, nv, k);
+    if ((threadIdx.x & 63) == 0) {
+      H[k + (size_t)ld * col] = s;
+      H[col + (size_t)ld * k] = s;
+    }
+  }
 }
 
 // ---- subspace eigensolver (host): eigenpairs of the leading n x n block of
@@ -260,10 +278,110 @@ __global__ void __launch_bounds__(kRedThreads) k_ritz(VecList Cv, VecList HCv, i
   }
 }
 
-__device__ __forceinline__ double sum_part(const double *part, int nblk, int stride, int off) {
-  double s = 0.0;
-  for (int b = 0; b < nblk; b++) s += part[(size_t)b * stride + off];
-  return s;
+// Basis size up to which the thick solver's expansion steps use the fused
+// pass k_ritz_expand (its per-thread dot accumulators are registers).
+constexpr int kFuseMax = 16;
+// k_ritz_expand's |W|^2, |R|^2 partials, past its dot partials in the
+// context's partial-sum buffer
+constexpr size_t kPwOffset = (size_t)kRedBlocks * (kFuseMax + 1);
+
+// One pass over the basis for an expansion step of the thick solver with
+// the diagonal preconditioner: W = sum_k U[k] C_k and R = E0 W -
+// sum_k U[k] HC_k as k_ritz (R is not stored), the diagonal step
+// T = R/(E0 - d) with k_precond's regularization (T is the unnormalized
+// new direction: Gram-Schmidt normalizes it), and per block the partial
+// dots <C_k|T> (k < nv) and <T|T> at part[b*(nv+1) + k] (the layout
+// k_finish_dgks reads) and |W|^2, |R|^2 at pw[2b], pw[2b+1].  Replaces
+// k_ritz, k_scale2, k_precond and the first Gram-Schmidt k_dots (their
+// separate reads of R, T and the basis).  nv <= kFuseMax.
+__global__ void __launch_bounds__(kRedThreads) k_ritz_expand(VecList Cv, VecList HCv, int nv,
+                                                             const double *__restrict__ U,
+                                                             const double *__restrict__ E,
+                                                             const double *__restrict__ hii, double eps_reg,
+                                                             size_t n, double *__restrict__ W,
+                                                             double *__restrict__ T, double *__restrict__ part,
+                                                             double *__restrict__ pw) {
+  constexpr int NS = kFuseMax + 3;  // dots with C_k, then |T|^2, |W|^2, |R|^2
+  __shared__ double sh[kRedThreads / 64][NS];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const double E0 = E[0];
+  double acc[NS];
+#pragma unroll
+  for (int k = 0; k < NS; k++) acc[k] = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double cv[kFuseMax];
+    double w = 0.0, r = 0.0;
+    // basis loads in groups of 8 issued together, sums in index order (as
+    // k_ritz); lanes past nv reload vector 0 and are not summed
+#pragma unroll
+    for (int k0 = 0; k0 < kFuseMax; k0 += 8) {
+      if (k0 < nv) {
+        double hv[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const int k = k0 + j < nv ? k0 + j : 0;
+          cv[k0 + j] = Cv.p[k][i];
+          hv[j] = HCv.p[k][i];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          if (k0 + j < nv) {
+            w += cv[k0 + j] * U[k0 + j];
+            r += hv[j] * (-U[k0 + j]);
+          }
+      }
+    }
+    r += E0 * w;
+    const double den = E0 - hii[i];
+    const double t = fabs(den) > eps_reg ? r / den : r / (den - eps_reg);
+    W[i] = w;
+    T[i] = t;
+#pragma unroll
+    for (int k = 0; k < kFuseMax; k++)
+      if (k < nv) acc[k] += cv[k] * t;
+    acc[kFuseMax] += t * t;
+    acc[kFuseMax + 1] += w * w;
+    acc[kFuseMax + 2] += r * r;
+  }
+  // fixed-order block sums: butterfly within each wavefront, then the 4
+  // wavefronts in order
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    double v = acc[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) sh[wv][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NS) {
+    const int k = threadIdx.x;
+    double v = sh[0][k];
+    for (int q = 1; q < kRedThreads / 64; q++) v += sh[q][k];
+    if (k < nv) part[(size_t)blockIdx.x * (nv + 1) + k] = v;
+    else if (k == kFuseMax) part[(size_t)blockIdx.x * (nv + 1) + nv] = v;
+    else if (k == kFuseMax + 1) pw[2 * (size_t)blockIdx.x] = v;
+    else if (k == kFuseMax + 2) pw[2 * (size_t)blockIdx.x + 1] = v;
+  }
+}
+
+// scal[0] = |R|, scal[1] = |W| from k_ritz_expand's block partials
+__global__ void __launch_bounds__(64) k_finish_norms(int nblk, const double *__restrict__ pw,
+                                                     double *__restrict__ scal) {
+  const double nw = wave_sum(pw, nblk, 2, 0);
+  const double nr = wave_sum(pw, nblk, 2, 1);
+  if (threadIdx.x == 0) {
+    scal[0] = sqrt(nr);
+    scal[1] = sqrt(nw);
+  }
+}
+
+// v /= *nrm
+__global__ void __launch_bounds__(kRedThreads) k_scale_by(double *__restrict__ v, size_t n,
+                                                          const double *__restrict__ nrm) {
+  const double f = 1.0 / nrm[0];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) v[i] *= f;
 }
 
 // W /= |W| ; R /= |R| ; scal[0] = |R|, scal[1] = |W|.
@@ -271,14 +389,16 @@ __global__ void __launch_bounds__(kRedThreads) k_scale2(double *__restrict__ W, 
                                                         size_t n, const double *__restrict__ part,
                                                         int nblk, double *__restrict__ scal) {
   __shared__ double f[2];
-  if (threadIdx.x == 0) {
-    const double nw = sqrt(sum_part(part, nblk, 2, 0));
-    const double nr = sqrt(sum_part(part, nblk, 2, 1));
-    f[0] = 1.0 / nw;
-    f[1] = 1.0 / nr;
-    if (blockIdx.x == 0) {
-      scal[0] = nr;
-      scal[1] = nw;
+  if (threadIdx.x < 64) {
+    const double nw = sqrt(wave_sum(part, nblk, 2, 0));
+    const double nr = sqrt(wave_sum(part, nblk, 2, 1));
+    if (threadIdx.x == 0) {
+      f[0] = 1.0 / nw;
+      f[1] = 1.0 / nr;
+      if (blockIdx.x == 0) {
+        scal[0] = nr;
+        scal[1] = nw;
+      }
     }
   }
   __syncthreads();
@@ -309,7 +429,10 @@ __global__ void __launch_bounds__(kRedThreads) k_axpy_part(double *__restrict__ 
                                                            const double *__restrict__ part,
                                                            int nblk) {
   __shared__ double ol;
-  if (threadIdx.x == 0) ol = -sum_part(part, nblk, 1, 0);
+  if (threadIdx.x < 64) {
+    const double v = wave_sum(part, nblk, 1, 0);
+    if (threadIdx.x == 0) ol = -v;
+  }
   __syncthreads();
   const double o = ol;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -322,15 +445,20 @@ __global__ void __launch_bounds__(kRedThreads) k_scale_part(double *__restrict__
                                                             const double *__restrict__ part,
                                                             int nblk) {
   __shared__ double f;
-  if (threadIdx.x == 0) f = 1.0 / sqrt(sum_part(part, nblk, 1, 0));
+  if (threadIdx.x < 64) {
+    const double v = wave_sum(part, nblk, 1, 0);
+    if (threadIdx.x == 0) f = 1.0 / sqrt(v);
+  }
   __syncthreads();
   const double g = f;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) c[i] *= g;
 }
 
-__global__ void k_finish_scalar(int nblk, const double *__restrict__ part, double *out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *out = sum_part(part, nblk, 1, 0);
+// *out = fixed-order sum of the nblk partials (one wavefront)
+__global__ void __launch_bounds__(64) k_finish_scalar(int nblk, const double *__restrict__ part, double *out) {
+  const double v = wave_sum(part, nblk, 1, 0);
+  if (threadIdx.x == 0) *out = v;
 }
 
 static void dots(Context &c, const double *const *X, int nv, const double *y, double *part,
@@ -346,7 +474,7 @@ double device_dot(Context &c, const double *x, const double *y, size_t n) {
   const double *X[1] = {x};
   dots(c, X, 1, y, c.part);
   double *scal = c.dsmall + kScalarOffset + 8;
-  hipLaunchKernelGGL(k_finish_scalar, dim3(1), dim3(1), 0, c.stream, kRedBlocks, c.part, scal);
+  hipLaunchKernelGGL(k_finish_scalar, dim3(1), dim3(64), 0, c.stream, kRedBlocks, c.part, scal);
   check_launch("k_finish_scalar");
   double r = 0.0;
   SBDX_HIP(hipMemcpyAsync(&r, scal, sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -355,13 +483,13 @@ double device_dot(Context &c, const double *x, const double *y, size_t n) {
 }
 
 // out[k] = fixed-order sum over blocks of part[blk*nv + k]
-__global__ void k_finish_vec(int nv, int nblk, const double *__restrict__ part, double *__restrict__ out,
-                             const int *__restrict__ gate) {
-  const int k = threadIdx.x;
-  if (k >= nv || (gate && *gate == 0)) return;
-  double s = 0.0;
-  for (int b = 0; b < nblk; b++) s += part[(size_t)b * nv + k];
-  out[k] = s;
+__global__ void __launch_bounds__(kFinishThreads) k_finish_vec(int nv, int nblk, const double *__restrict__ part,
+                                                               double *__restrict__ out, const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;
+  for (int k = threadIdx.x >> 6; k < nv; k += kFinishThreads / 64) {
+    const double s = wave_sum(part, nblk, nv, k);
+    if ((threadIdx.x & 63) == 0) out[k] = s;
+  }
 }
 
 // t -= sum_k coef[k] X_k (one classical Gram-Schmidt pass with coefficients
@@ -394,19 +522,19 @@ __global__ void __launch_bounds__(kRedThreads) k_sub_combo(VecList X, int nv, co
 // coef[k] = <X_k|t>; *gate = 1 when the pass leaves less than half of
 // |t|^2 (|t|^2 - sum coef^2 < |t|^2 / 2), so that a second pass is needed
 // to keep t orthogonal (Daniel, Gragg, Kaufman and Stewart 1976,
-// threshold 1/sqrt(2) on the norm), else 0.  One block of kNbMax threads.
-__global__ void k_finish_dgks(int nv, int nblk, const double *__restrict__ part, double *__restrict__ coef,
-                              int *__restrict__ gate) {
+// threshold 1/sqrt(2) on the norm), else 0.  One block of kFinishThreads.
+__global__ void __launch_bounds__(kFinishThreads) k_finish_dgks(int nv, int nblk, const double *__restrict__ part,
+                                                                double *__restrict__ coef, int *__restrict__ gate) {
   __shared__ double s[kNbMax + 1];
-  const int k = threadIdx.x;
-  if (k <= nv) {
-    double a = 0.0;
-    for (int b = 0; b < nblk; b++) a += part[(size_t)b * (nv + 1) + k];
-    s[k] = a;
-    if (k < nv) coef[k] = a;
+  for (int k = threadIdx.x >> 6; k <= nv; k += kFinishThreads / 64) {
+    const double a = wave_sum(part, nblk, nv + 1, k);
+    if ((threadIdx.x & 63) == 0) {
+      s[k] = a;
+      if (k < nv) coef[k] = a;
+    }
   }
   __syncthreads();
-  if (k == 0) {
+  if (threadIdx.x == 0) {
     double hh = 0.0;
     for (int j = 0; j < nv; j++) hh += s[j] * s[j];
     *gate = (s[nv] - hh < 0.5 * s[nv]) ? 1 : 0;
@@ -450,9 +578,12 @@ __global__ void __launch_bounds__(kRedThreads) k_exact_resid(const double *__res
                                                              double *__restrict__ Eout) {
   __shared__ double sh[kRedThreads];
   __shared__ double Es;
-  if (threadIdx.x == 0) {
-    Es = sum_part(epart, nblk, 1, 0);
-    if (blockIdx.x == 0) *Eout = Es;
+  if (threadIdx.x < 64) {
+    const double v = wave_sum(epart, nblk, 1, 0);
+    if (threadIdx.x == 0) {
+      Es = v;
+      if (blockIdx.x == 0) *Eout = v;
+    }
   }
   __syncthreads();
   const double E = Es;
@@ -514,7 +645,7 @@ int davidson_solve(Context &c, int max_it, int nb, double eps, double max_time, 
           dots(c, c.HC.data(), ib + 1, c.C[ib], c.part);
         else
           dots(c, c.C.data(), ib + 1, c.HC[ib], c.part);
-        hipLaunchKernelGGL(k_finish_hcol, dim3(1), dim3(kNbMax), 0, c.stream, ib + 1, kRedBlocks,
+        hipLaunchKernelGGL(k_finish_hcol, dim3(1), dim3(kFinishThreads), 0, c.stream, ib + 1, kRedBlocks,
                            c.part, H, ld, ib);
         check_launch("k_finish_hcol");
         subspace_eig(c, ib + 1, ld, H, U, E);
@@ -627,7 +758,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
   // coupling <W|H|t> = t . r near convergence.
   auto add_column = [&](int col) {
     dots(c, c.HC.data(), col + 1, c.C[col], c.part);
-    hipLaunchKernelGGL(k_finish_hcol, dim3(1), dim3(kNbMax), 0, c.stream, col + 1, kRedBlocks, c.part,
+    hipLaunchKernelGGL(k_finish_hcol, dim3(1), dim3(kFinishThreads), 0, c.stream, col + 1, kRedBlocks, c.part,
                        H, ld, col);
     check_launch("k_finish_hcol");
   };
@@ -732,6 +863,15 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       ProfScope ps(c, "dav_subspace");
       subspace_eig(c, d + 1, ld, H, U, E);
     }
+    // An expansion step with the diagonal step takes the new direction and
+    // its first Gram-Schmidt dots from the Ritz pass itself (k_ritz_expand):
+    // no restart is due at this basis size, no training pairs are being
+    // recorded (they need R) and the network is not the preconditioner.
+    // W then stays unnormalized (unit up to rounding) until the step ends
+    // other than by expansion, when normalize_w scales it exactly.
+    const bool fuse = d + 1 <= kFuseMax && d < nb - 1 && c.mlp.record_cap == 0 &&
+                      !(c.precond == 1 && c.mlp.valid);
+    bool have_cn = false;
     {
       ProfScope ps(c, "dav_ritz");
       VecList Cv{}, HCv{};
@@ -739,11 +879,24 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
         Cv.p[k] = c.C[k];
         HCv.p[k] = c.HC[k];
       }
-      hipLaunchKernelGGL(k_ritz, g, b, 0, c.stream, Cv, HCv, d + 1, U, E, N, c.W, c.R, c.part);
-      check_launch("k_ritz");
-      hipLaunchKernelGGL(k_scale2, g, b, 0, c.stream, c.W, c.R, N, c.part, kRedBlocks, scal);
-      check_launch("k_scale2");
+      if (fuse) {
+        hipLaunchKernelGGL(k_ritz_expand, g, b, 0, c.stream, Cv, HCv, d + 1, U, E, c.hii, eps_reg, N, c.W,
+                           c.C[d + 1], c.part, c.part + kPwOffset);
+        check_launch("k_ritz_expand");
+        hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(64), 0, c.stream, kRedBlocks, c.part + kPwOffset, scal);
+        check_launch("k_finish_norms");
+      } else {
+        hipLaunchKernelGGL(k_ritz, g, b, 0, c.stream, Cv, HCv, d + 1, U, E, N, c.W, c.R, c.part);
+        check_launch("k_ritz");
+        hipLaunchKernelGGL(k_scale2, g, b, 0, c.stream, c.W, c.R, N, c.part, kRedBlocks, scal);
+        check_launch("k_scale2");
+      }
     }
+    auto normalize_w = [&]() {
+      if (!fuse) return;
+      hipLaunchKernelGGL(k_scale_by, g, b, 0, c.stream, c.W, N, scal + 1);
+      check_launch("k_scale_by");
+    };
     host2(E, scal, E0, nr);
     c.trace.push_back(E0);
     c.trace.push_back(nr);
@@ -753,6 +906,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
     }
     const bool exact_now = !mixed;  // every product in the basis is FP64
     if (nr < eps && exact_now) {
+      normalize_w();
       done = true;
       break;
     }
@@ -763,6 +917,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
     // use for it (the same premise as any Davidson solve: the iteration
     // converges to the lowest state its start vector reaches)
     if (E0 - c.abandon_above > 10.0 * nr) {
+      normalize_w();
       abandoned = true;
       break;
     }
@@ -770,6 +925,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
     if (d == nb - 1 || (nr < eps && !fresh) || timeout) {
       // ---- restart with an exact residual check
       ProfScope ps(c, "dav_restart");
+      normalize_w();
       double *HW = c.spareHY[0];
       apply(c.W, HW, false);
       steps++;
@@ -779,7 +935,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       hipLaunchKernelGGL(k_exact_resid, g, b, 0, c.stream, c.W, HW, N, c.part, kRedBlocks, c.R,
                          c.part + kRedBlocks, Eex);
       check_launch("k_exact_resid");
-      hipLaunchKernelGGL(k_finish_scalar, dim3(1), dim3(1), 0, c.stream, kRedBlocks,
+      hipLaunchKernelGGL(k_finish_scalar, dim3(1), dim3(64), 0, c.stream, kRedBlocks,
                          c.part + kRedBlocks, scal + 3);
       check_launch("k_finish_scalar");
       double e_ex, r2;
@@ -850,6 +1006,8 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       fresh = true;
       // expand from the exact residual (E0 = exact energy for the preconditioner)
       SBDX_HIP(hipMemcpyAsync(E, Eex, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    } else {
+      have_cn = fuse;  // the Ritz pass wrote the direction and its dots
     }
     // ---- expand: preconditioned residual, MGS against the basis, normalize
     rhist.push_back(nr);
@@ -878,8 +1036,10 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
         if (sym) symmetrize(c, cn, c.sym_par);
         c.mlp.used_mlp++;
       } else {
-        hipLaunchKernelGGL(k_precond, g, b, 0, c.stream, c.R, c.hii, E, N, eps_reg, cn);
-        check_launch("k_precond");
+        if (!have_cn) {
+          hipLaunchKernelGGL(k_precond, g, b, 0, c.stream, c.R, c.hii, E, N, eps_reg, cn);
+          check_launch("k_precond");
+        }
         c.mlp.used_diag++;
       }
     }
@@ -897,13 +1057,13 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       const double *Xw[kNbMax];
       for (int k = 0; k <= d; k++) Xw[k] = c.C[k];
       Xw[d + 1] = cn;  // d + 2 <= nb <= kNbMax
-      dots(c, Xw, d + 2, cn, c.part);
-      hipLaunchKernelGGL(k_finish_dgks, dim3(1), dim3(kNbMax), 0, c.stream, d + 1, kRedBlocks, c.part, coef, gate);
+      if (!have_cn) dots(c, Xw, d + 2, cn, c.part);  // else k_ritz_expand wrote them
+      hipLaunchKernelGGL(k_finish_dgks, dim3(1), dim3(kFinishThreads), 0, c.stream, d + 1, kRedBlocks, c.part, coef, gate);
       check_launch("k_finish_dgks");
       hipLaunchKernelGGL(k_sub_combo, g, b, 0, c.stream, Cv, d + 1, coef, cn, N, (const int *)nullptr);
       check_launch("k_sub_combo");
       dots(c, c.C.data(), d + 1, cn, c.part, gate);
-      hipLaunchKernelGGL(k_finish_vec, dim3(1), dim3(kNbMax), 0, c.stream, d + 1, kRedBlocks, c.part, coef,
+      hipLaunchKernelGGL(k_finish_vec, dim3(1), dim3(kFinishThreads), 0, c.stream, d + 1, kRedBlocks, c.part, coef,
                          (const int *)gate);
       check_launch("k_finish_vec");
       hipLaunchKernelGGL(k_sub_combo, g, b, 0, c.stream, Cv, d + 1, coef, cn, N, (const int *)gate);
