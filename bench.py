@@ -197,7 +197,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1, help="timed Davidson solves")
     ap.add_argument("--warmup", type=int, default=1, help="untimed sigma applications")
     ap.add_argument("--workload", default="c36")
-    ap.add_argument("--nb", type=int, default=12)
+    ap.add_argument("--nb", type=int, default=16)
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--max-it", type=int, default=200)
     ap.add_argument("--max-time", type=float, default=900.0)
