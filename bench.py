@@ -218,7 +218,7 @@ def main():
                          "sigma")
     ap.add_argument("--no-symmetry", dest="sectors", action="store_const", const="off",
                     help="same as --sectors off")
-    ap.add_argument("--coarse", type=int, default=500,
+    ap.add_argument("--coarse", type=int, default=1000,
                     help="coarse start on the first M strings of each set (optimized solver; 0: off)")
     ap.add_argument("--coarse-eps", type=float, default=1e-2)
     ap.add_argument("--no-graph-run", dest="graph_run", action="store_false",
