@@ -254,6 +254,7 @@ Context *make_sub_context(const Context &c, size_t m) {
 
 static void destroy(Context *c) {
   if (!c) return;
+  if (c->coarse_sub) destroy(c->coarse_sub);
   (void)hipStreamSynchronize(c->stream);
   prof_clear(*c);
   graphs_release(*c);

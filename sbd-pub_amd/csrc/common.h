@@ -182,6 +182,10 @@ struct Context {
   // slots per thread of the row-parts mixed kernel, per form as mixed_eb
   // (0 until timed on first use)
   int parts_nt[4] = {0, 0, 0, 0};
+  // the coarse start's sub-problem context (first coarse_sub_m strings of
+  // each set), kept between solves of the same size; owned by this context
+  Context *coarse_sub = nullptr;
+  size_t coarse_sub_m = 0;
   // block products: 1 when the multi-vector kernel beat per-vector launches
   // on first use, 0 when not, -1 not decided
   int multi_mode[2] = {-1, -1};

@@ -26,6 +26,9 @@ constexpr int kRedBlocks = 1024;
 // FP32 noise level of a product relative to |E0| (1000 x 2^-24): below
 // kFp32Floor |E0| a stalled mixed-precision solve switches to FP64 products
 constexpr double kFp32Floor = 1000.0 * 5.9604644775390625e-8;
+// distance above another sector's result, relative to |E0|, beyond which an
+// abandonable stage keeps its anchor products in FP32
+constexpr double kAnchorMargin = 1e-3;
 constexpr int kRedThreads = 256;
 
 struct OutList {
@@ -801,11 +804,18 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
   if (mc > 0 && mc < c.na && mc < c.nb) {
     ProfScope ps(c, "coarse_start");
     const auto tc = std::chrono::steady_clock::now();
-    Context *sub = make_sub_context(c, mc);
-    struct Owner {
-      Context *p;
-      ~Owner() { destroy_context(p); }
-    } owner{sub};
+    // the sub-problem's context (tables, dense blocks, tuned kernels) is kept
+    // for the next solve of the same size, e.g. the other spin-flip sector
+    if (c.coarse_sub && c.coarse_sub_m != mc) {
+      destroy_context(c.coarse_sub);
+      c.coarse_sub = nullptr;
+    }
+    if (!c.coarse_sub) {
+      c.coarse_sub = make_sub_context(c, mc);
+      c.coarse_sub_m = mc;
+    }
+    Context *sub = c.coarse_sub;
+    sub->mixed_form = c.mixed_form;
     sub->precision = c.precision;
     sub->restart_keep = c.restart_keep;
     // nested levels: the sub-problem starts from a quarter of its strings
@@ -846,7 +856,19 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
     hipLaunchKernelGGL(k_scale_part, g, b, 0, c.stream, c.C[0], N, c.part, kRedBlocks);
     check_launch("k_scale_part");
   }
-  apply(c.C[0], c.HC[0], false);
+  // An abandonable stage (abandon_above set: another spin-flip sector's
+  // result lies at that level) is only needed to show that its lowest state
+  // lies above the level, and FP32 products (errors ~1e-7 |H|) show that as
+  // well as FP64 ones: its anchor products (the start and the restarts)
+  // stay FP32 while E0 lies more than kAnchorMargin |E0| above the level.
+  // Within the margin the stage may hold the lower state, and its anchors
+  // are exact as in any solve.
+  auto approx_anchor = [&](double e) {
+    return mixed && c.abandon_above < 1e299 && e - c.abandon_above > kAnchorMargin * std::max(1.0, std::fabs(e));
+  };
+  // the lifted coarse vector's Rayleigh quotient is the coarse energy (the
+  // sub-problem is H restricted to the coarse block)
+  apply(c.C[0], c.HC[0], c.coarse_products > 0 && approx_anchor(c.coarse_energy));
   add_column(0);
   int d = 0, steps = 1;
   bool done = false, fresh = true, abandoned = false;
@@ -927,7 +949,8 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       ProfScope ps(c, "dav_restart");
       normalize_w();
       double *HW = c.spareHY[0];
-      apply(c.W, HW, false);
+      const bool approx = approx_anchor(E0);
+      apply(c.W, HW, approx);
       steps++;
       const double *X[1] = {c.W};
       dots(c, X, 1, HW, c.part);
@@ -948,7 +971,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
         fprintf(stderr, "[sbdx] restart step %d E0 = %.15f exact |r| = %.3e\n", steps, E0, nr);
         fflush(stderr);
       }
-      if (nr < eps) {
+      if (nr < eps && !approx) {
         done = true;
         break;
       }
@@ -960,7 +983,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       // the FP32 noise level of this spectrum, the rest of the solve uses
       // FP64 products, and this restart keeps W alone (its product exact).
       int keep_now = m;
-      if (mixed && prev_exact > 0.0 && nr > 0.5 * prev_exact && nr < kFp32Floor * std::fabs(E0)) {
+      if (mixed && !approx && prev_exact > 0.0 && nr > 0.5 * prev_exact && nr < kFp32Floor * std::fabs(E0)) {
         mixed = false;
         keep_now = 1;
         c.fp64_switch_step = steps;
@@ -970,7 +993,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
           fflush(stderr);
         }
       }
-      prev_exact = nr;
+      prev_exact = approx ? -1.0 : nr;  // an FP32 anchor residual is no baseline
       // R /= |R| for the preconditioner (the reference normalizes R too)
       {
         const double *Y[1] = {c.R};
