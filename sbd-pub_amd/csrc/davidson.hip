@@ -932,12 +932,13 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       done = true;
       break;
     }
-    // The Ritz value E0 only decreases, and near convergence it lies above
-    // the eigenvalue it approaches by about |r|^2 / (gap to the next one):
-    // once E0 exceeds abandon_above by ten times |r|, the state this
-    // iteration converges to lies above that value and the caller has no
-    // use for it (the same premise as any Davidson solve: the iteration
-    // converges to the lowest state its start vector reaches)
+    // With the unit Ritz vector u = sum c_i v_i over this sector's
+    // eigenvectors, |r|^2 >= |c_1|^2 (E0 - lambda_1)^2, so E0 - lambda_1 <=
+    // |r| / |c_1|.  Were the sector's lowest eigenvalue below abandon_above,
+    // E0 - abandon_above > 10 |r| would need |c_1| < 1/10: while u keeps an
+    // overlap of at least 1/10 with the lowest state (the state the
+    // iteration converges to), the test shows that this sector's lowest
+    // state lies above the level and the caller has no use for it
     if (E0 - c.abandon_above > 10.0 * nr) {
       normalize_w();
       abandoned = true;
