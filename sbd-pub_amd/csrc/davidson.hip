@@ -856,19 +856,26 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
     hipLaunchKernelGGL(k_scale_part, g, b, 0, c.stream, c.C[0], N, c.part, kRedBlocks);
     check_launch("k_scale_part");
   }
-  // An abandonable stage (abandon_above set: another spin-flip sector's
-  // result lies at that level) is only needed to show that its lowest state
-  // lies above the level, and FP32 products (errors ~1e-7 |H|) show that as
-  // well as FP64 ones: its anchor products (the start and the restarts)
-  // stay FP32 while E0 lies more than kAnchorMargin |E0| above the level.
-  // Within the margin the stage may hold the lower state, and its anchors
-  // are exact as in any solve.
-  auto approx_anchor = [&](double e) {
-    return mixed && c.abandon_above < 1e299 && e - c.abandon_above > kAnchorMargin * std::max(1.0, std::fabs(e));
+  // Anchor products (the start and the restarts) are exact so that the
+  // subspace couples every correction to an exact product once the
+  // residual nears the FP32 floor.  They stay FP32 (errors ~1e-7 |H|)
+  //  - while the residual r of the step lies ten times above that floor
+  //    (kFp32Floor |E0|): the corrections' own FP32 products dominate the
+  //    subspace's error there, and a later restart anchors exactly;
+  //  - in an abandonable stage (abandon_above set: another spin-flip
+  //    sector's result lies at that level) while E0 lies more than
+  //    kAnchorMargin |E0| above the level: the stage is only needed to show
+  //    that its lowest state lies above, which FP32 values show as well.
+  // Convergence is only declared on an exact anchor.
+  auto approx_anchor = [&](double e, double r) {
+    if (!mixed) return false;
+    const double ae = std::max(1.0, std::fabs(e));
+    return r > 10.0 * kFp32Floor * ae || (c.abandon_above < 1e299 && e - c.abandon_above > kAnchorMargin * ae);
   };
   // the lifted coarse vector's Rayleigh quotient is the coarse energy (the
-  // sub-problem is H restricted to the coarse block)
-  apply(c.C[0], c.HC[0], c.coarse_products > 0 && approx_anchor(c.coarse_energy));
+  // sub-problem is H restricted to the coarse block); its residual is not
+  // known before this product, so only the second rule applies
+  apply(c.C[0], c.HC[0], c.coarse_products > 0 && approx_anchor(c.coarse_energy, 0.0));
   add_column(0);
   int d = 0, steps = 1;
   bool done = false, fresh = true, abandoned = false;
@@ -950,7 +957,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       ProfScope ps(c, "dav_restart");
       normalize_w();
       double *HW = c.spareHY[0];
-      const bool approx = approx_anchor(E0);
+      const bool approx = approx_anchor(E0, nr);
       apply(c.W, HW, approx);
       steps++;
       const double *X[1] = {c.W};
