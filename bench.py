@@ -223,7 +223,7 @@ def main():
     ap.add_argument("--coarse-eps", type=float, default=1e-2)
     ap.add_argument("--no-graph-run", dest="graph_run", action="store_false",
                     help="skip the extra timed solve with the FP32 sigma core replayed as a hipGraph")
-    ap.add_argument("--keep", type=int, default=4,
+    ap.add_argument("--keep", type=int, default=3,
                     help="Ritz vectors kept at a thick restart (thick / mixed solvers)")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
                     help="skip the plain FP64 reference solve")
