@@ -234,6 +234,8 @@ def main():
     ap.add_argument("--factored", action="store_true",
                     help="also time one product per factored (pq|rs)-GEMM form of the mixed term")
     ap.add_argument("--quiet", action="store_true", help="no per-step solver lines on stderr")
+    ap.add_argument("--rdm", action="store_true",
+                    help="also compute the 1-/2-RDM of the solution after the timed solves (not timed in value)")
     ap.add_argument("--precond", choices=["diag", "mlp"], default="diag",
                     help="preconditioner of the timed solver (mlp: learned, weights below)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "sbd-pub_amd", "sbdx", "data",
@@ -540,6 +542,17 @@ def main():
         "same_spin_form": op.same_spin_forms(),
         "build_s": round(t_build, 3),
     }
+    if args.rdm:
+        # one- and two-particle RDMs of the solution, after the timed solves
+        # (the vector goes in and the RDMs come out through host buffers)
+        Wsol = op.get_vector()
+        t0 = time.perf_counter()
+        one, two = op.correlation(Wsol)
+        L = inst["norb"]
+        out["rdm"] = dict(seconds=round(time.perf_counter() - t0, 4),
+                          trace_one=float(sum(one[s * L * L + i * (L + 1)] for s in (0, 1) for i in range(L))),
+                          note="sbdx_rdm of the solution vector, host buffers included")
+        log(f"rdm: {out['rdm']['seconds']} s, trace of the 1-RDM {out['rdm']['trace_one']:.10f}")
     if args.factored:
         out["factored_mixed"] = factored_leg(op, W0 if args.init == "random" else
                                              np.random.default_rng(2).standard_normal(op.ndet), inst["norb"])
