@@ -83,6 +83,18 @@ struct SpinTables {
   // n < 8192.
   uint32_t *ell3 = nullptr;
   bool has_ell3 = false;
+  // The singles grouped by pair index pq (for the RDM's mixed term):
+  // pm_off[K+1]; pm_ent = bra | ket << 32 | (sign<0) << 63, rows ascending
+  // within a pair.  The same grouping as a sliced ELL over the pair bins
+  // (n < 32768): bins by entry count into slices of 64 lanes, entry k of
+  // lane l of slice s at pb[pb_off[s] + 64 k + l] = bra | ket << 15 |
+  // (sign<0) << 31 (padding: bra = ket = n), pb_len[s] entries per lane,
+  // pb_perm[64 s + l] the lane's bin (0xffffffff past K).
+  uint64_t *pm_off = nullptr;
+  uint64_t *pm_ent = nullptr;
+  uint32_t *pb = nullptr, *pb_len = nullptr, *pb_perm = nullptr;
+  uint64_t *pb_off = nullptr;
+  int pb_nslice = 0;
   // The same singles split by ket range into `nparts` parts of `part_h`
   // kets (ket stored relative to its part), same perm and slices; built on
   // demand when one ket row does not fit in LDS.

@@ -292,6 +292,87 @@ __global__ void __launch_bounds__(1024) k_rdm_mixed(int nb, int K, const uint64_
   }
 }
 
+// The same mixed term with no atomics: one block per alpha pair index pq
+// loops over the alpha singles (I -> J, s_a) with that pq (a_pm), staging
+// x[I,:] and x[J,:] in LDS (each followed by a zero).  The beta singles are
+// grouped by their pair index rs into bins laid out as a sliced ELL (b_pb:
+// 64 bins per slice, entry k of lane l at b_off[s] + 64 k + l, packed
+// ib | jb << 15 | (s_b<0) << 31, padding reading the zeros): wavefront w
+// owns slices w, w + 16, ..., each lane one bin, and adds s_a times
+// sum s_b x[I,ib] x[J,jb] over its bin to a register; after the last alpha
+// single the registers are Tm's row pq.  Table loads are coalesced (one
+// 256-byte row per wavefront step) and no two lanes share a bin.  Needs
+// nb < 8192 (the staged rows: at most 8 values per thread) and at most
+// 16 * MS slices.
+template <int MS>
+__global__ void __launch_bounds__(1024) k_rdm_mixed_pq(int nb, int K, int nslice,
+                                                       const uint64_t *__restrict__ a_off,
+                                                       const uint64_t *__restrict__ a_ent,
+                                                       const uint32_t *__restrict__ b_pb,
+                                                       const uint64_t *__restrict__ b_off,
+                                                       const uint32_t *__restrict__ b_len,
+                                                       const uint32_t *__restrict__ b_perm,
+                                                       const double *__restrict__ w, double *__restrict__ Tm) {
+  extern __shared__ double lds[];
+  double *xI = lds, *xJ = lds + nb + 1;
+  const int pq = blockIdx.x;
+  const uint64_t e0 = a_off[pq], e1 = a_off[pq + 1];
+  if (e0 == e1) return;  // uniform per block, before any barrier (Tm's row stays 0)
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  double acc[MS];
+#pragma unroll
+  for (int j = 0; j < MS; j++) acc[j] = 0.0;
+  for (uint64_t e = e0; e < e1; e++) {
+    const uint64_t ent = a_ent[e];
+    const size_t I = (uint32_t)(ent & 0xffffffffull);
+    const size_t J = (uint32_t)((ent >> 32) & 0x7fffffffull);
+    const double sa = (ent >> 63) ? -1.0 : 1.0;
+    // both rows loaded before the barrier that ends the previous alpha
+    // single's reads (nb + 1 <= 8192: at most 8 values per thread and row)
+    double vi[8], vj[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int i = tid + k * 1024;
+      vi[k] = i < nb ? w[I * nb + i] : 0.0;
+      vj[k] = i < nb ? w[J * nb + i] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int i = tid + k * 1024;
+      if (i <= nb) {
+        xI[i] = vi[k];
+        xJ[i] = vj[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < MS; j++) {
+      const int sl = wv + 16 * j;
+      if (sl < nslice) {
+        const uint32_t len = b_len[sl];
+        const uint32_t *p = b_pb + b_off[sl] + lane;
+        double s = 0.0;
+#pragma unroll 4
+        for (uint32_t k = 0; k < len; k++) {
+          const uint32_t wd = p[64 * (size_t)k];
+          const double v = xI[wd & 0x7fffu] * xJ[(wd >> 15) & 0x7fffu];
+          s += (wd >> 31) ? -v : v;
+        }
+        acc[j] += sa * s;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < MS; j++) {
+    const int sl = wv + 16 * j;
+    if (sl < nslice) {
+      const uint32_t bin = b_perm[sl * 64 + lane];
+      if (bin != 0xffffffffu) Tm[(size_t)pq * K + bin] = acc[j];
+    }
+  }
+}
+
 // Place Tm[pq][rs] (alpha p->q, beta r->s) with TwoDiffCorrelation's rules on
 // the sorted spin orbitals I<J (bra-only 2p, 2r+1) and A<B (ket-only 2q, 2s+1).
 __global__ void k_rdm_mixed_place(int L, const double *__restrict__ Tm, double *__restrict__ two) {
@@ -357,7 +438,10 @@ static void rdm_run(Context &c, const double *Wp, double *one, double *two, cons
   const int na = (int)c.na, nb = (int)c.nb;
   const size_t lds = sizeof(double) * ((size_t)nb + K);
   const int nt = (nb + 1023) / 1024;
-  if (lds > c.lds_limit || nt > 32)
+  // the pair-major kernel stages two rows, each followed by a zero
+  const size_t lds_pq = sizeof(double) * (2 * (size_t)nb + 2);
+  const bool by_pair = c.A.pm_ent && c.B.pb && c.B.pb_nslice <= 64 && nb < 8192 && lds_pq <= c.lds_limit;
+  if (!by_pair && (lds > c.lds_limit || nt > 32))
     throw Error(3, "rdm: beta string count too large for the LDS-staged mixed RDM kernel");
   zero(c, one, 2 * L2);
   zero(c, two, 4 * L4);
@@ -401,7 +485,18 @@ static void rdm_run(Context &c, const double *Wp, double *one, double *two, cons
     }
   }
   // mixed alpha-single x beta-single
-  if (nt <= 1) launch_rdm_mixed<1>(c, Wp, wk.Tm, lds);
+  if (by_pair) {
+    const int ms = (c.B.pb_nslice + 15) / 16;
+    auto go = [&](auto fn) {
+      SBDX_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pq));
+      hipLaunchKernelGGL(fn, dim3((unsigned)K), dim3(1024), lds_pq, c.stream, nb, (int)K, c.B.pb_nslice,
+                         c.A.pm_off, c.A.pm_ent, c.B.pb, c.B.pb_off, c.B.pb_len, c.B.pb_perm, Wp, wk.Tm);
+    };
+    if (ms <= 1) go(k_rdm_mixed_pq<1>);
+    else if (ms <= 2) go(k_rdm_mixed_pq<2>);
+    else go(k_rdm_mixed_pq<4>);
+    check_launch("k_rdm_mixed_pq");
+  } else if (nt <= 1) launch_rdm_mixed<1>(c, Wp, wk.Tm, lds);
   else if (nt <= 2) launch_rdm_mixed<2>(c, Wp, wk.Tm, lds);
   else if (nt <= 4) launch_rdm_mixed<4>(c, Wp, wk.Tm, lds);
   else if (nt <= 8) launch_rdm_mixed<8>(c, Wp, wk.Tm, lds);

@@ -564,6 +564,80 @@ void build_singles_ell(Context &c, SpinTables &t) {
   const int n = t.n;
   std::vector<uint64_t> off, ent;
   download_singles(c, t, off, ent);
+  // the singles grouped by pair index (counting sort, row order kept
+  // within a pair) for the RDM's mixed term
+  {
+    const size_t K = (size_t)c.K;
+    std::vector<uint64_t> poff(K + 1, 0);
+    for (size_t k = 0; k < t.ns; k++) poff[((ent[k] >> 32) & 0x7fffffffull) + 1]++;
+    for (size_t pq = 0; pq < K; pq++) poff[pq + 1] += poff[pq];
+    std::vector<uint64_t> pent(t.ns);
+    std::vector<uint64_t> fill(poff.begin(), poff.end() - 1);
+    for (int r = 0; r < n; r++)
+      for (uint64_t k = off[r]; k < off[r + 1]; k++) {
+        const uint64_t e = ent[k];
+        const uint64_t dst = fill[(e >> 32) & 0x7fffffffull]++;
+        const uint32_t ket = (uint32_t)(e & 0xffffffffull), neg = (uint32_t)(e >> 63);
+        pent[dst] = (uint64_t)r | ((uint64_t)ket << 32) | ((uint64_t)neg << 63);
+      }
+    t.pm_off = (uint64_t *)dalloc(c, sizeof(uint64_t) * (K + 1));
+    t.pm_ent = (uint64_t *)dalloc(c, sizeof(uint64_t) * std::max<size_t>(t.ns, 1));
+    SBDX_HIP(hipMemcpyAsync(t.pm_off, poff.data(), sizeof(uint64_t) * (K + 1), hipMemcpyHostToDevice, c.stream));
+    SBDX_HIP(hipMemcpyAsync(t.pm_ent, pent.data(), sizeof(uint64_t) * t.ns, hipMemcpyHostToDevice, c.stream));
+    // the same grouping as a sliced ELL over the pair bins (n <= 32767):
+    // bins sorted by entry count (descending) into slices of 64 lanes,
+    // entry k of lane l of slice s at pb_off[s] + 64 k + l, packed as
+    // bra | ket << 15 | (sign<0) << 31; padding entries bra = ket = n (a
+    // staged zero); pb_perm[64 s + l] = the lane's bin (0xffffffff past K)
+    t.pb_nslice = 0;
+    if (n < 32768) {
+      std::vector<uint32_t> bins(K);
+      for (size_t i = 0; i < K; i++) bins[i] = (uint32_t)i;
+      std::stable_sort(bins.begin(), bins.end(), [&](uint32_t a, uint32_t b) {
+        return (poff[a + 1] - poff[a]) > (poff[b + 1] - poff[b]);
+      });
+      const size_t nsl = (K + 63) / 64;
+      std::vector<uint32_t> perm(nsl * 64, 0xffffffffu), len(nsl);
+      std::vector<uint64_t> soff(nsl);
+      uint64_t tot = 0;
+      for (size_t sl = 0; sl < nsl; sl++) {
+        uint64_t mx = 0;
+        for (size_t l = 0; l < 64; l++) {
+          const size_t slot = sl * 64 + l;
+          if (slot < K) {
+            perm[slot] = bins[slot];
+            mx = std::max<uint64_t>(mx, poff[bins[slot] + 1] - poff[bins[slot]]);
+          }
+        }
+        soff[sl] = tot;
+        len[sl] = (uint32_t)mx;
+        tot += mx * 64;
+      }
+      const uint32_t pad = (uint32_t)n | ((uint32_t)n << 15);
+      std::vector<uint32_t> pb(std::max<uint64_t>(tot, 1), pad);
+      for (size_t sl = 0; sl < nsl; sl++)
+        for (size_t l = 0; l < 64; l++) {
+          const uint32_t bin = perm[sl * 64 + l];
+          if (bin == 0xffffffffu) continue;
+          for (uint64_t k = poff[bin]; k < poff[bin + 1]; k++) {
+            const uint64_t pe = pent[k];
+            const uint32_t bra = (uint32_t)(pe & 0xffffffffull), ket = (uint32_t)((pe >> 32) & 0x7fffffffull);
+            pb[soff[sl] + 64 * (k - poff[bin]) + l] = bra | (ket << 15) | ((uint32_t)(pe >> 63) << 31);
+          }
+        }
+      t.pb = (uint32_t *)dalloc(c, sizeof(uint32_t) * pb.size());
+      t.pb_off = (uint64_t *)dalloc(c, sizeof(uint64_t) * nsl);
+      t.pb_len = (uint32_t *)dalloc(c, sizeof(uint32_t) * nsl);
+      t.pb_perm = (uint32_t *)dalloc(c, sizeof(uint32_t) * perm.size());
+      SBDX_HIP(hipMemcpyAsync(t.pb, pb.data(), sizeof(uint32_t) * pb.size(), hipMemcpyHostToDevice, c.stream));
+      SBDX_HIP(hipMemcpyAsync(t.pb_off, soff.data(), sizeof(uint64_t) * nsl, hipMemcpyHostToDevice, c.stream));
+      SBDX_HIP(hipMemcpyAsync(t.pb_len, len.data(), sizeof(uint32_t) * nsl, hipMemcpyHostToDevice, c.stream));
+      SBDX_HIP(hipMemcpyAsync(t.pb_perm, perm.data(), sizeof(uint32_t) * perm.size(), hipMemcpyHostToDevice,
+                              c.stream));
+      t.pb_nslice = (int)nsl;
+    }
+    SBDX_HIP(hipStreamSynchronize(c.stream));
+  }
   std::vector<uint32_t> order(n);
   for (int i = 0; i < n; i++) order[i] = (uint32_t)i;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
