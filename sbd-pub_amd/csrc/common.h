@@ -258,6 +258,7 @@ struct Context {
   // Davidson state
   int dav_nb = 0;
   std::vector<double *> C, HC;
+  std::vector<float *> HCf;  // FP32 storage of FP32 products (thick solver, symmetric mixed mode)
   double *W = nullptr, *R = nullptr;
   double *dsmall = nullptr;                 // small device matrices/scalars
   double *hpin = nullptr;                   // pinned host staging for the subspace eigensolver
@@ -328,6 +329,7 @@ void sigma_block(Context &c, int k, const double *x, double *y);
 bool sigma_sym_available(Context &c);
 void sigma_sym(Context &c, const double *x, double *y);
 void sigma_sym_f32(Context &c, const double *x, double *y);
+void sigma_sym_f32_to(Context &c, const double *x, float *y);
 void sigma_sym_f32_raw(Context &c, const float *x, float *y);  // FP32 vectors in and out, no graph
 double sym_defect(Context &c, const double *v, int p);  // max|v - p v^T| / max|v|, p = +-1
 void symmetrize(Context &c, double *v, int p);         // v = (v + p v^T)/2

@@ -35,9 +35,17 @@ struct OutList {
   double *p[8];
 };
 
+// Basis columns for the streaming kernels: column k is p[k] (FP64) or, when
+// bit k of f32 is set, pf[k] (FP32 storage of an FP32 product, widened on
+// load: the values are those of the FP64 copy it replaces).
 struct VecList {
   const double *p[kNbMax];
+  const float *pf[kNbMax];
+  uint64_t f32;
 };
+__device__ __forceinline__ double vget(const VecList &X, int k, size_t i) {
+  return ((X.f32 >> k) & 1ull) ? (double)X.pf[k][i] : X.p[k][i];
+}
 
 __device__ __forceinline__ double block_sum(double v, double *sh) {
   // deterministic LDS tree over 256 threads
@@ -67,18 +75,14 @@ __global__ void __launch_bounds__(kRedThreads) k_dots(VecList X, int nv, const d
   const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int k0 = 0; k0 < nv; k0 += 8) {
     const int m = nv - k0 < 8 ? nv - k0 : 8;
-    const double *xk[8];
     double s[8];
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      xk[j] = X.p[j < m ? k0 + j : k0];
-      s[j] = 0.0;
-    }
+    for (int j = 0; j < 8; j++) s[j] = 0.0;
     for (size_t i = i0; i < n; i += stride) {
       const double yi = y[i];
 #pragma unroll
       for (int j = 0; j < 8; j++)
-        if (j < m) s[j] += xk[j][i] * yi;
+        if (j < m) s[j] += vget(X, k0 + j, i) * yi;
     }
     for (int j = 0; j < m; j++) {
       const double r = block_sum(s[j], sh);
@@ -258,7 +262,7 @@ __global__ void __launch_bounds__(kRedThreads) k_ritz(VecList Cv, VecList HCv, i
       for (int j = 0; j < 8; j++) {
         const int k = j < m ? k0 + j : k0;
         cv[j] = Cv.p[k][i];
-        hv[j] = HCv.p[k][i];
+        hv[j] = vget(HCv, k, i);
       }
 #pragma unroll
       for (int j = 0; j < 8; j++)
@@ -324,7 +328,7 @@ __global__ void __launch_bounds__(kRedThreads) k_ritz_expand(VecList Cv, VecList
         for (int j = 0; j < 8; j++) {
           const int k = k0 + j < nv ? k0 + j : 0;
           cv[k0 + j] = Cv.p[k][i];
-          hv[j] = HCv.p[k][i];
+          hv[j] = vget(HCv, k, i);
         }
 #pragma unroll
         for (int j = 0; j < 8; j++)
@@ -471,6 +475,11 @@ static void dots(Context &c, const double *const *X, int nv, const double *y, do
   hipLaunchKernelGGL(k_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, L, nv, y, c.N, part, gate);
   check_launch("k_dots");
 }
+static void dots(Context &c, const VecList &L, int nv, const double *y, double *part) {
+  hipLaunchKernelGGL(k_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, L, nv, y, c.N, part,
+                     (const int *)nullptr);
+  check_launch("k_dots");
+}
 
 double device_dot(Context &c, const double *x, const double *y, size_t n) {
   (void)n;
@@ -557,7 +566,7 @@ __global__ void __launch_bounds__(kRedThreads) k_combine(VecList in, int nin, co
       const int m = nin - k0 < 8 ? nin - k0 : 8;
       double v[8];
 #pragma unroll
-      for (int q = 0; q < 8; q++) v[q] = in.p[q < m ? k0 + q : k0][i];
+      for (int q = 0; q < 8; q++) v[q] = vget(in, q < m ? k0 + q : k0, i);
 #pragma unroll
       for (int q = 0; q < 8; q++)
         if (q < m) {
@@ -759,8 +768,22 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
   // the restart vector (whose product is exact FP64) stays exact in mixed
   // mode; an FP32 error of ~1e-7 |H| there would exceed the residual-sized
   // coupling <W|H|t> = t . r near convergence.
+  // Products of FP32-product columns are kept in FP32 (c.HCf) in the
+  // symmetric mixed solve: bit k of hc32 marks column k.  The values are
+  // those an FP64 copy would hold (an FP32 result widened), at half the
+  // bytes for every later pass over the products.
+  uint64_t hc32 = 0;
+  auto hc_list = [&](int nv) {
+    VecList L{};
+    for (int k = 0; k < nv; k++) {
+      L.p[k] = c.HC[k];
+      L.pf[k] = k < (int)c.HCf.size() ? c.HCf[k] : nullptr;
+    }
+    L.f32 = nv >= 64 ? hc32 : (hc32 & ((1ull << nv) - 1ull));
+    return L;
+  };
   auto add_column = [&](int col) {
-    dots(c, c.HC.data(), col + 1, c.C[col], c.part);
+    dots(c, hc_list(col + 1), col + 1, c.C[col], c.part);
     hipLaunchKernelGGL(k_finish_hcol, dim3(1), dim3(kFinishThreads), 0, c.stream, col + 1, kRedBlocks, c.part,
                        H, ld, col);
     check_launch("k_finish_hcol");
@@ -876,6 +899,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
   // sub-problem is H restricted to the coarse block); its residual is not
   // known before this product, so only the second rule applies
   apply(c.C[0], c.HC[0], c.coarse_products > 0 && approx_anchor(c.coarse_energy, 0.0));
+  hc32 = 0;
   add_column(0);
   int d = 0, steps = 1;
   bool done = false, fresh = true, abandoned = false;
@@ -903,11 +927,9 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
     bool have_cn = false;
     {
       ProfScope ps(c, "dav_ritz");
-      VecList Cv{}, HCv{};
-      for (int k = 0; k <= d; k++) {
-        Cv.p[k] = c.C[k];
-        HCv.p[k] = c.HC[k];
-      }
+      VecList Cv{};
+      for (int k = 0; k <= d; k++) Cv.p[k] = c.C[k];
+      const VecList HCv = hc_list(d + 1);
       if (fuse) {
         hipLaunchKernelGGL(k_ritz_expand, g, b, 0, c.stream, Cv, HCv, d + 1, U, E, c.hii, eps_reg, N, c.W,
                            c.C[d + 1], c.part, c.part + kPwOffset);
@@ -1011,11 +1033,9 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       }
       // thick restart basis: W (exact product) + Ritz vectors 1..keep_now-1
       if (keep_now > 1) {
-        VecList Cv{}, HCv{};
-        for (int k = 0; k <= d; k++) {
-          Cv.p[k] = c.C[k];
-          HCv.p[k] = c.HC[k];
-        }
+        VecList Cv{};
+        for (int k = 0; k <= d; k++) Cv.p[k] = c.C[k];
+        const VecList HCv = hc_list(d + 1);
         OutList Yo{}, HYo{};
         for (int j = 1; j < keep_now; j++) {
           Yo.p[j - 1] = c.spareY[j];
@@ -1033,6 +1053,7 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
         std::swap(c.HC[j], c.spareHY[j]);
       }
       d = keep_now - 1;
+      hc32 &= ~((1ull << keep_now) - 1ull);  // the kept columns' products are FP64
       for (int j = 0; j <= d; j++) add_column(j);
       fresh = true;
       // expand from the exact residual (E0 = exact energy for the preconditioner)
@@ -1104,7 +1125,15 @@ int davidson_solve_thick(Context &c, int max_steps, int nb, int keep, double eps
       hipLaunchKernelGGL(k_scale_part, g, b, 0, c.stream, cn, N, c.part, kRedBlocks);
       check_launch("k_scale_part");
     }
-    apply(c.C[d + 1], c.HC[d + 1], mixed);
+    if (mixed && sym) {
+      while ((int)c.HCf.size() <= d + 1) c.HCf.push_back(nullptr);
+      if (!c.HCf[d + 1]) c.HCf[d + 1] = (float *)dalloc(c, sizeof(float) * N);
+      sigma_sym_f32_to(c, c.C[d + 1], c.HCf[d + 1]);
+      hc32 |= 1ull << (d + 1);
+    } else {
+      apply(c.C[d + 1], c.HC[d + 1], mixed);
+      hc32 &= ~(1ull << (d + 1));
+    }
     steps++;
     d++;
     add_column(d);

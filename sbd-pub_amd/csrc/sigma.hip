@@ -2128,6 +2128,17 @@ void sigma_sym_f32_raw(Context &c, const float *x, float *y) {
   sigma_sym_impl<float>(c, x, y);
 }
 
+// The FP32 symmetric product of an FP64 vector, left in FP32 (the values
+// sigma_sym_f32 widens).
+void sigma_sym_f32_to(Context &c, const double *x, float *y) {
+  ensure_f32(c);
+  const unsigned g = (unsigned)std::min<size_t>(cdiv(c.N, 256), 65536);
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(g), dim3(256), 0, c.stream, c.N, x, c.f32.x);
+  check_launch("k_cast");
+  sym_core_f32(c);
+  SBDX_HIP(hipMemcpyAsync(y, c.f32.y, sizeof(float) * c.N, hipMemcpyDeviceToDevice, c.stream));
+}
+
 void sigma_sym_f32(Context &c, const double *x, double *y) {
   ensure_f32(c);
   const unsigned g = (unsigned)std::min<size_t>(cdiv(c.N, 256), 65536);
