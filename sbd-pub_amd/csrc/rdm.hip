@@ -12,8 +12,11 @@
 //   TwoDiff   -> per same-spin double pair: T; mixed: T[pq][rs] summed over
 //                all (alpha single, beta single) pairs, then placed with the
 //                reference's spin-orbital ordering rules.
+#include <memory>
 #include <mutex>
 #include <string>
+
+#include <rocblas/rocblas.h>
 
 #include "common.h"
 
@@ -187,6 +190,28 @@ __global__ void __launch_bounds__(256) k_rdm_double_pairs(int n, int M, uint64_t
     out[e - e0] = sg * t;
   }
   }
+}
+
+// The same double pairs from the Gram matrix G = w w^T of this spin's n rows
+// (one library GEMM per spin instead of a dot product per pair):
+// out[e - e0] = sign * G[I, J]
+__global__ void __launch_bounds__(256) k_rdm_double_gram(int n, uint64_t e0, uint64_t e1,
+                                                         const uint64_t *__restrict__ d_off,
+                                                         const uint32_t *__restrict__ d_ket,
+                                                         const uint32_t *__restrict__ d_orb,
+                                                         const uint64_t *__restrict__ str,
+                                                         const double *__restrict__ G,
+                                                         double *__restrict__ out) {
+  const uint64_t e = e0 + (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= e1) return;
+  const int I = row_of(d_off, n, e);
+  const size_t J = d_ket[e];
+  const uint32_t o = d_orb[e];
+  const int i = o & 255, j = (o >> 8) & 255, a = (o >> 16) & 255, b = (o >> 24) & 255;
+  const uint64_t bra = str[I];
+  double sg = parity1r(bra, i, a) * parity1r(bra, j, b);
+  if (a > j || b < i) sg = -sg;
+  out[e - e0] = sg * G[(size_t)I * n + J];
 }
 
 // OneDiff scatter for the single pairs e0 <= e < e1 of one spin (si =
@@ -428,6 +453,7 @@ static void zero(Context &c, double *p, size_t n) {
 struct RdmWork {
   double *rho_a, *rho_b, *mrow, *Tm, *Ts, *Td;
   size_t ts_cap, td_cap;
+  double *G = nullptr;  // room for max(na, nb)^2 values, or null (dot products per double pair)
 };
 
 // Everything after the sign conversion: W' (alpha-first, device) to one and
@@ -473,12 +499,27 @@ static void rdm_run(Context &c, const double *Wp, double *one, double *two, cons
                          (uint64_t)e0, (uint64_t)e1, t.s_off, t.s_ent, t.str, wk.Ts, one, two);
       check_launch("k_rdm_single_scatter");
     }
+    const bool gram = wk.G && t.nd > 0;
+    if (gram) {
+      // G = w w^T: w is n rows of M values, i.e. the column-major M x n
+      // matrix B, and G = B^T B (n x n, symmetric)
+      const double one = 1.0, zero_d = 0.0;
+      if (rocblas_dgemm((rocblas_handle)blas_handle(c), rocblas_operation_transpose, rocblas_operation_none, t.n,
+                        t.n, M, &one, w, M, w, M, &zero_d, wk.G, t.n) != rocblas_status_success)
+        throw Error(3, "rdm: rocblas_dgemm (Gram matrix) failed");
+    }
     for (size_t e0 = 0; e0 < t.nd; e0 += wk.td_cap) {
       const size_t e1 = std::min(t.nd, e0 + wk.td_cap);
-      hipLaunchKernelGGL(k_rdm_double_pairs, dim3((unsigned)std::min<size_t>(e1 - e0, 1u << 20)), dim3(256), 0,
-                         c.stream, t.n, M, (uint64_t)e0, (uint64_t)e1, t.d_off, t.d_ket, t.d_orb, t.str, w,
-                         wk.Td);
-      check_launch("k_rdm_double_pairs");
+      if (gram) {
+        hipLaunchKernelGGL(k_rdm_double_gram, dim3((unsigned)cdiv(e1 - e0, 256)), dim3(256), 0, c.stream, t.n,
+                           (uint64_t)e0, (uint64_t)e1, t.d_off, t.d_ket, t.d_orb, t.str, wk.G, wk.Td);
+        check_launch("k_rdm_double_gram");
+      } else {
+        hipLaunchKernelGGL(k_rdm_double_pairs, dim3((unsigned)std::min<size_t>(e1 - e0, 1u << 20)), dim3(256),
+                           0, c.stream, t.n, M, (uint64_t)e0, (uint64_t)e1, t.d_off, t.d_ket, t.d_orb, t.str, w,
+                           wk.Td);
+        check_launch("k_rdm_double_pairs");
+      }
       hipLaunchKernelGGL(k_rdm_double_scatter, dim3(cdiv(e1 - e0, 256)), dim3(256), 0, c.stream, L, spin,
                          (uint64_t)e0, (uint64_t)e1, t.d_orb, wk.Td, two);
       check_launch("k_rdm_double_scatter");
@@ -514,8 +555,18 @@ void compute_rdm(Context &c, const double *Wp, double *one, double *two) {
   const size_t nd = std::max<size_t>(std::max(c.A.nd, c.B.nd), 1);
   TmpBuf b_rho_a(sizeof(double) * c.na), b_rho_b(sizeof(double) * c.nb), b_mrow(sizeof(double) * c.na * L),
       b_tm(sizeof(double) * K * K), b_ts(sizeof(double) * ns * (L + 1)), b_td(sizeof(double) * nd);
-  const RdmWork wk{(double *)b_rho_a.p, (double *)b_rho_b.p, (double *)b_mrow.p, (double *)b_tm.p,
-                   (double *)b_ts.p,    (double *)b_td.p,    ns,                  nd};
+  RdmWork wk{(double *)b_rho_a.p, (double *)b_rho_b.p, (double *)b_mrow.p, (double *)b_tm.p,
+             (double *)b_ts.p,    (double *)b_td.p,    ns,                  nd};
+  // the Gram matrices of the double pairs when a quarter of free memory
+  // holds one (else a dot product per pair)
+  const size_t mx = std::max(c.na, c.nb);
+  size_t free_b = 0, total_b = 0;
+  SBDX_HIP(hipMemGetInfo(&free_b, &total_b));
+  std::unique_ptr<TmpBuf> b_g;
+  if (mx < (1u << 31) && sizeof(double) * mx * mx < free_b / 4) {
+    b_g.reset(new TmpBuf(sizeof(double) * mx * mx));
+    wk.G = (double *)b_g->p;
+  }
   rdm_run(c, Wp, one, two, wk);
 }
 
